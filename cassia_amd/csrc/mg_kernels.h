@@ -1,0 +1,57 @@
+// Kernel entry points and parameter blocks of the MI355X mold renderer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mg_shared.h"
+
+namespace mg {
+
+struct PaintParams {
+    const uint64_t *segs;        // sorted render words
+    const uint32_t *row_start;   // global_rows + 1
+    const uint32_t *row_scene;   // global row -> scene
+    const SceneDesc *scenes;
+    const PropsEntry *props;
+    const float *stops;
+    int32_t *prev_layers;        // BufferLayerCache::layers_per_tile (-1 = None)
+    uint8_t *out;                // RGBA8 frame(s)
+    uint32_t col_bits, layer_bits;
+    uint32_t cap_ids, cap_runs;  // LDS capacities
+    uint32_t *overflow;
+    // debug: coverage records (A values) and per-tile write ops
+    int32_t *cov_a;
+    uint32_t *cov_key;
+    uint32_t *cov_count;
+    uint32_t cov_cap;
+    int8_t *tile_ops;
+};
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n);
+__global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n);
+__global__ void k_setup(const float4 *pts, const uint16_t *ids, const LayerEntry *layers,
+                        const SceneDesc *scenes, const uint32_t *scene_line_base, FrameDims dims,
+                        LineSetup *out, uint32_t *lengths, uint32_t *err);
+__global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *status,
+                           uint32_t *part_counter, uint64_t *total);
+__global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
+                       FrameDims dims, FrameCounters *counters, uint64_t *out, uint64_t capacity,
+                       uint64_t *status, uint32_t *part_counter, uint32_t *hist, uint32_t *row_count);
+__global__ void k_ingest(const uint64_t *in, uint64_t n_in, const SceneDesc *scenes, FrameDims dims,
+                         FrameCounters *counters, uint64_t *out, uint64_t *status, uint32_t *part_counter,
+                         uint32_t *hist, uint32_t *row_count);
+__global__ void k_digit_bases(const uint32_t *hist, uint32_t *bases, uint32_t radix);
+template <int R>
+__global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
+                            uint32_t shift, const uint32_t *digit_base, uint32_t *status,
+                            uint32_t *part_counter);
+__global__ void k_paint(PaintParams pp);
+
+size_t paint_lds_bytes_host(uint32_t cap, uint32_t rcap);
+
+constexpr uint32_t kEmitTileHost = 1024;
+constexpr uint32_t kSortTileHost = 4096;
+constexpr uint32_t kScanTileHost = 2048;
+
+}  // namespace mg

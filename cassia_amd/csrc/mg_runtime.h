@@ -1,0 +1,104 @@
+// Device runtime of the MI355X mold renderer: resident buffers, frame
+// execution (setup -> scan -> emit -> sort -> paint) and timing.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "mg_shared.h"
+
+namespace mg {
+
+// Growable device allocation.
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    bool reserve(size_t n);  // keeps contents only when not growing
+    void release();
+    template <class T>
+    T *as() const { return static_cast<T *>(ptr); }
+};
+
+// Host-side description of one frame (a batch of scenes).
+struct FrameSpec {
+    std::vector<SceneDesc> scenes;
+    std::vector<uint32_t> scene_line_base;
+    std::vector<LayerEntry> layers;     // concatenated per scene
+    std::vector<PropsEntry> props;      // concatenated per scene
+    std::vector<float> stops;
+    uint32_t max_order = 0;             // over all scenes
+    uint32_t mode = 0;                  // 0 render, 1 canonical stream, 2 render an ingested stream
+    const uint64_t *in_segs = nullptr;  // mode 2: device PixelSegment stream
+    uint64_t n_in = 0;
+    uint32_t radix_bits = 8;
+    bool want_tile_ops = false;
+    bool want_coverage = false;
+    uint32_t coverage_cap = 0;
+};
+
+struct FrameResult {
+    uint64_t line_pixels = 0, segments = 0, kept = 0;
+    uint32_t passes = 0, radix_bits = 0;
+    uint64_t tiles = 0;
+    double device_ms = 0.0;
+    const uint64_t *sorted = nullptr;   // device pointer to the sorted words (mode 1)
+};
+
+class Renderer {
+public:
+    Renderer();
+    ~Renderer();
+    bool init(std::string &err);
+
+    // Resident line store (float4 points + u16 layer ids).
+    bool upload_lines(const float *pts4, const uint16_t *ids, size_t first, size_t n, std::string &err);
+    size_t lines_capacity() const { return lines_cap_; }
+
+    // Per-tile previous layer counts of a BufferLayerCache (-1 = None).
+    int32_t *cache_array(int id, size_t tiles, bool &resized, std::string &err);
+    bool cache_clear(int id, std::string &err);
+    void cache_release(int id);
+
+    // The frame image: the caller's pixels are uploaded first so that tiles
+    // the painter leaves untouched keep their contents.
+    uint8_t *frame_image(size_t bytes, std::string &err);
+
+    bool run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_t *prev_layers,
+             FrameResult &res, std::string &err);
+    bool download_sorted(uint64_t *dst, size_t n, std::string &err);
+    bool download_tile_ops(int8_t *dst, size_t n, std::string &err);
+    bool download_coverage(std::vector<int32_t> &a, std::vector<uint32_t> &key, uint32_t &count,
+                           std::string &err);
+    hipStream_t stream() const { return stream_; }
+    int cu_count() const { return cus_; }
+
+private:
+    bool paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
+               const uint64_t *sorted, uint8_t *image, int32_t *prev_layers, size_t tiles,
+               std::string &err);
+
+    int device_ = -1;
+    int cus_ = 0;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    DevBuf pts_, ids_;
+    size_t lines_cap_ = 0;
+    DevBuf scenes_, scene_line_base_, layers_, props_, stops_;
+    DevBuf setup_, lengths_, offsets_, scan_status_, counters_, err_;
+    DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
+    DevBuf row_count_, row_start_, row_scene_, row_status_;
+    DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
+    std::vector<DevBuf> caches_;
+    std::vector<size_t> cache_tiles_;
+    const uint64_t *last_sorted_ = nullptr;
+    uint32_t cap_ids_ = 256, cap_runs_ = 256;
+};
+
+// Number of usable gfx950 devices.
+int device_count();
+
+}  // namespace mg

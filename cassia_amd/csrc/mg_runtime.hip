@@ -1,0 +1,472 @@
+// Device runtime of the MI355X mold renderer.
+#include "mg_runtime.h"
+
+#include <algorithm>
+#include <cstring>
+
+#include "mg_kernels.h"
+
+namespace mg {
+
+#define MG_HIP(call, what)                                                  \
+    do {                                                                    \
+        hipError_t e_ = (call);                                             \
+        if (e_ != hipSuccess) {                                             \
+            err = std::string(what) + ": " + hipGetErrorString(e_);         \
+            return false;                                                   \
+        }                                                                   \
+    } while (0)
+
+bool DevBuf::reserve(size_t n) {
+    if (n <= bytes) return true;
+    release();
+    size_t want = std::max<size_t>(n, 256);
+    if (hipMalloc(&ptr, want) != hipSuccess) {
+        ptr = nullptr;
+        bytes = 0;
+        return false;
+    }
+    bytes = want;
+    return true;
+}
+
+void DevBuf::release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+static bool is_gfx950(int dev) {
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, dev) != hipSuccess) return false;
+    return std::strncmp(p.gcnArchName, "gfx950", 6) == 0;
+}
+
+int device_count() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    int ok = 0;
+    for (int d = 0; d < n; ++d)
+        if (is_gfx950(d)) ++ok;
+    return ok;
+}
+
+Renderer::Renderer() {}
+
+Renderer::~Renderer() {
+    if (device_ >= 0) {
+        (void)hipSetDevice(device_);
+        if (stream_) (void)hipStreamSynchronize(stream_);
+    }
+    DevBuf *all[] = {&pts_, &ids_, &scenes_, &scene_line_base_, &layers_, &props_, &stops_,
+                     &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
+                     &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
+                     &row_count_, &row_start_, &row_scene_, &row_status_, &image_, &overflow_,
+                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_};
+    for (DevBuf *b : all) b->release();
+    for (DevBuf &b : caches_) b.release();
+    if (ev0_) (void)hipEventDestroy(ev0_);
+    if (ev1_) (void)hipEventDestroy(ev1_);
+    if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+bool Renderer::init(std::string &err) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+        err = "no HIP device";
+        return false;
+    }
+    for (int d = 0; d < n; ++d)
+        if (is_gfx950(d)) { device_ = d; break; }
+    if (device_ < 0) {
+        err = "no gfx950 device";
+        return false;
+    }
+    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+    hipDeviceProp_t p;
+    MG_HIP(hipGetDeviceProperties(&p, device_), "hipGetDeviceProperties");
+    cus_ = p.multiProcessorCount;
+    MG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+    MG_HIP(hipEventCreate(&ev0_), "hipEventCreate");
+    MG_HIP(hipEventCreate(&ev1_), "hipEventCreate");
+    if (!counters_.reserve(sizeof(FrameCounters)) || !err_.reserve(16) || !overflow_.reserve(16) ||
+        !part_counters_.reserve(64 * sizeof(uint32_t)) || !cov_count_.reserve(16)) {
+        err = "out of device memory";
+        return false;
+    }
+    return true;
+}
+
+bool Renderer::upload_lines(const float *pts4, const uint16_t *ids, size_t first, size_t n, std::string &err) {
+    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+    size_t need = first + n;
+    if (need > lines_cap_) {
+        size_t cap = std::max<size_t>(need, lines_cap_ * 2);
+        DevBuf np, ni;
+        if (!np.reserve(cap * 16) || !ni.reserve(cap * 2)) {
+            np.release();
+            ni.release();
+            err = "out of device memory (lines)";
+            return false;
+        }
+        if (first > 0 && pts_.ptr) {
+            MG_HIP(hipMemcpyAsync(np.ptr, pts_.ptr, first * 16, hipMemcpyDeviceToDevice, stream_), "copy lines");
+            MG_HIP(hipMemcpyAsync(ni.ptr, ids_.ptr, first * 2, hipMemcpyDeviceToDevice, stream_), "copy ids");
+            MG_HIP(hipStreamSynchronize(stream_), "sync");
+        }
+        pts_.release();
+        ids_.release();
+        pts_ = np;
+        ids_ = ni;
+        lines_cap_ = cap;
+    }
+    if (n) {
+        MG_HIP(hipMemcpyAsync(pts_.as<char>() + first * 16, pts4, n * 16, hipMemcpyHostToDevice, stream_),
+               "upload lines");
+        MG_HIP(hipMemcpyAsync(ids_.as<char>() + first * 2, ids, n * 2, hipMemcpyHostToDevice, stream_),
+               "upload ids");
+        MG_HIP(hipStreamSynchronize(stream_), "sync");
+    }
+    return true;
+}
+
+static void fill32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+    k_fill_u32<<<blocks, 256, 0, s>>>(p, v, n);
+}
+
+static void fill64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s) {
+    if (!n) return;
+    uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
+    k_fill_u64<<<blocks, 256, 0, s>>>(p, v, n);
+}
+
+int32_t *Renderer::cache_array(int id, size_t tiles, bool &resized, std::string &err) {
+    resized = false;
+    if (id < 0) return nullptr;
+    if ((size_t)id >= caches_.size()) {
+        caches_.resize(id + 1);
+        cache_tiles_.resize(id + 1, 0);
+    }
+    if (cache_tiles_[id] != tiles) {
+        if (!caches_[id].reserve(std::max<size_t>(tiles, 1) * 4)) {
+            err = "out of device memory (cache)";
+            return nullptr;
+        }
+        fill32(caches_[id].as<uint32_t>(), 0xFFFFFFFFu, tiles, stream_);
+        cache_tiles_[id] = tiles;
+        resized = true;
+    }
+    return caches_[id].as<int32_t>();
+}
+
+bool Renderer::cache_clear(int id, std::string &err) {
+    if (id >= 0 && (size_t)id < caches_.size() && caches_[id].ptr)
+        fill32(caches_[id].as<uint32_t>(), 0xFFFFFFFFu, cache_tiles_[id], stream_);
+    MG_HIP(hipGetLastError(), "cache clear");
+    return true;
+}
+
+void Renderer::cache_release(int id) {
+    if (id >= 0 && (size_t)id < caches_.size()) {
+        caches_[id].release();
+        cache_tiles_[id] = 0;
+    }
+}
+
+uint8_t *Renderer::frame_image(size_t bytes, std::string &err) {
+    if (!image_.reserve(bytes)) {
+        err = "out of device memory (image)";
+        return nullptr;
+    }
+    return image_.as<uint8_t>();
+}
+
+static uint32_t bits_for(uint64_t v) {  // bits to represent 0..v
+    uint32_t b = 0;
+    while (b < 64 && (v >> b)) ++b;
+    return b;
+}
+
+template <class T>
+static bool upload_vec(DevBuf &b, const std::vector<T> &v, hipStream_t s, std::string &err) {
+    if (!b.reserve(std::max<size_t>(v.size(), 1) * sizeof(T))) {
+        err = "out of device memory";
+        return false;
+    }
+    if (!v.empty() && hipMemcpyAsync(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s) != hipSuccess) {
+        err = "upload failed";
+        return false;
+    }
+    return true;
+}
+
+bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_t *prev_layers,
+                   FrameResult &res, std::string &err) {
+    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+    const uint32_t n_scenes = (uint32_t)spec.scenes.size();
+    uint32_t global_rows = 0;
+    uint32_t max_cols = 1;
+    uint64_t tiles = 0;
+    for (const SceneDesc &s : spec.scenes) {
+        global_rows = std::max(global_rows, s.row_base + s.tile_rows);
+        max_cols = std::max(max_cols, std::min(s.crop_h1, s.row_len));
+        tiles += (uint64_t)s.tile_rows * s.row_len;
+    }
+    const uint32_t R = spec.radix_bits;
+    const uint32_t radix = 1u << R;
+    FrameDims dims{};
+    dims.n_scenes = n_scenes;
+    dims.n_lines = (uint32_t)n_lines;
+    dims.global_rows = global_rows;
+    dims.col_bits = std::max(1u, bits_for(max_cols));
+    dims.layer_bits = std::max(1u, bits_for(spec.max_order));
+    const uint32_t row_bits = global_rows > 1 ? bits_for(global_rows - 1) : 0;
+    dims.key_bits = spec.mode != 1 ? row_bits + dims.col_bits + dims.layer_bits : 41;
+    dims.radix_bits = R;
+    dims.passes = (dims.key_bits + R - 1) / R;
+    dims.mode = spec.mode;
+    if (kPayloadBits + dims.key_bits > 64 || dims.passes > 8) {
+        err = "frame too large for the 64-bit render key";
+        return false;
+    }
+    res = FrameResult{};
+    res.passes = dims.passes;
+    res.radix_bits = R;
+    res.tiles = tiles;
+
+    if (!upload_vec(scenes_, spec.scenes, stream_, err) ||
+        !upload_vec(scene_line_base_, spec.scene_line_base, stream_, err) ||
+        !upload_vec(layers_, spec.layers, stream_, err) || !upload_vec(props_, spec.props, stream_, err) ||
+        !upload_vec(stops_, spec.stops, stream_, err))
+        return false;
+    // global row -> scene
+    {
+        std::vector<uint32_t> rs(std::max<uint32_t>(global_rows, 1), 0);
+        for (uint32_t s = 0; s < n_scenes; ++s)
+            for (uint32_t r = 0; r < spec.scenes[s].tile_rows; ++r) rs[spec.scenes[s].row_base + r] = s;
+        if (!upload_vec(row_scene_, rs, stream_, err)) return false;
+    }
+    const size_t L = n_lines;
+    if (!setup_.reserve(std::max<size_t>(L, 1) * sizeof(LineSetup)) || !lengths_.reserve(std::max<size_t>(L, 1) * 4) ||
+        !offsets_.reserve(std::max<size_t>(L, 1) * 4)) {
+        err = "out of device memory (lines)";
+        return false;
+    }
+    FrameCounters *ctr = counters_.as<FrameCounters>();
+    uint32_t *pc = part_counters_.as<uint32_t>();
+    MG_HIP(hipEventRecord(ev0_, stream_), "event");
+    fill64((uint64_t *)ctr, 0, sizeof(FrameCounters) / 8, stream_);
+    fill32(err_.as<uint32_t>(), 0, 1, stream_);
+    fill32(pc, 0, 64, stream_);
+    const bool ingest = spec.mode == 2;
+    if (L && !ingest) {
+        k_setup<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
+            pts_.as<float4>(), ids_.as<uint16_t>(), layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(),
+            scene_line_base_.as<uint32_t>(), dims, setup_.as<LineSetup>(), lengths_.as<uint32_t>(),
+            err_.as<uint32_t>());
+        const uint64_t nparts = (L + kScanTileHost - 1) / kScanTileHost;
+        if (!scan_status_.reserve(nparts * 8)) { err = "out of device memory"; return false; }
+        fill64(scan_status_.as<uint64_t>(), 0, nparts, stream_);
+        k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(lengths_.as<uint32_t>(), offsets_.as<uint32_t>(), L,
+                                                        scan_status_.as<uint64_t>(), pc + 0, &ctr->line_pixels);
+    }
+    MG_HIP(hipGetLastError(), "setup/scan launch");
+    FrameCounters hc{};
+    uint32_t herr = 0;
+    uint64_t P = 0;
+    if (!ingest) {
+        MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read P");
+        MG_HIP(hipMemcpyAsync(&herr, err_.ptr, 4, hipMemcpyDeviceToHost, stream_), "read err");
+        MG_HIP(hipStreamSynchronize(stream_), "setup/scan");
+        if (herr) {
+            err = "a line spans more pixels than an i32 can index";
+            return false;
+        }
+        P = hc.line_pixels;
+    } else {
+        P = (spec.n_in + 1) / 2;  // capacity: n_in segments
+    }
+    if (2 * P >= (1ull << 30)) {
+        err = "frame exceeds 2^29 line-pixels";
+        return false;
+    }
+    res.line_pixels = ingest ? 0 : P;
+    const uint64_t cap_segs = std::max<uint64_t>(2 * P, 1);
+    const uint64_t nparts_emit = ((ingest ? spec.n_in : P) + kEmitTileHost - 1) / kEmitTileHost;
+    const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
+    if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) ||
+        !emit_status_.reserve(std::max<uint64_t>(nparts_emit, 1) * 8) || !hist_.reserve(8 * radix * 4) ||
+        !bases_.reserve(8 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4) ||
+        !row_count_.reserve(((size_t)global_rows + 1) * 4) || !row_start_.reserve(((size_t)global_rows + 1) * 4) ||
+        !row_status_.reserve(((global_rows + 1 + kScanTileHost - 1) / kScanTileHost) * 8)) {
+        err = "out of device memory (segments)";
+        return false;
+    }
+    fill32(hist_.as<uint32_t>(), 0, dims.passes * radix, stream_);
+    fill32(row_count_.as<uint32_t>(), 0, global_rows + 1, stream_);
+    if (ingest && spec.n_in) {
+        fill64(emit_status_.as<uint64_t>(), 0, nparts_emit, stream_);
+        uint32_t grid = (uint32_t)std::min<uint64_t>(nparts_emit, (uint64_t)cus_ * 8);
+        k_ingest<<<grid, 256, 0, stream_>>>(spec.in_segs, spec.n_in, scenes_.as<SceneDesc>(), dims, ctr,
+                                            segs_a_.as<uint64_t>(), emit_status_.as<uint64_t>(), pc + 1,
+                                            hist_.as<uint32_t>(), row_count_.as<uint32_t>());
+    } else if (P) {
+        fill64(emit_status_.as<uint64_t>(), 0, nparts_emit, stream_);
+        uint32_t grid = (uint32_t)std::min<uint64_t>(nparts_emit, (uint64_t)cus_ * 8);
+        k_emit<<<grid, 256, 0, stream_>>>(setup_.as<LineSetup>(), offsets_.as<uint32_t>(), scenes_.as<SceneDesc>(),
+                                          dims, ctr, segs_a_.as<uint64_t>(), cap_segs,
+                                          emit_status_.as<uint64_t>(), pc + 1, hist_.as<uint32_t>(),
+                                          row_count_.as<uint32_t>());
+    }
+    k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
+    if (spec.mode != 1) {
+        const uint64_t nr = (uint64_t)global_rows + 1;
+        fill64(row_status_.as<uint64_t>(), 0, (nr + kScanTileHost - 1) / kScanTileHost, stream_);
+        k_scan_u32<<<(uint32_t)((nr + kScanTileHost - 1) / kScanTileHost), 256, 0, stream_>>>(
+            row_count_.as<uint32_t>(), row_start_.as<uint32_t>(), nr, row_status_.as<uint64_t>(), pc + 2,
+            &ctr->rows_total);
+    }
+    MG_HIP(hipGetLastError(), "emit launch");
+    uint64_t *in = segs_a_.as<uint64_t>(), *out = segs_b_.as<uint64_t>();
+    const uint32_t sort_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nparts_sort, 1), (uint64_t)cus_ * 4);
+    for (uint32_t p = 0; p < dims.passes; ++p) {
+        fill32(sort_status_.as<uint32_t>(), 0, nparts_sort * radix, stream_);
+        const uint32_t shift = kPayloadBits + p * R;
+        uint32_t *counter = pc + 8 + p;
+        if (R == 8)
+            k_sort_pass<8><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
+                                                            sort_status_.as<uint32_t>(), counter);
+        else if (R == 6)
+            k_sort_pass<6><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
+                                                            sort_status_.as<uint32_t>(), counter);
+        else
+            k_sort_pass<4><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
+                                                            sort_status_.as<uint32_t>(), counter);
+        std::swap(in, out);
+    }
+    MG_HIP(hipGetLastError(), "sort launch");
+    last_sorted_ = in;
+    res.sorted = in;
+    if (spec.mode != 1) {
+        if (spec.want_tile_ops) {
+            if (!tile_ops_.reserve(std::max<uint64_t>(tiles, 1))) { err = "out of device memory"; return false; }
+            MG_HIP(hipMemsetAsync(tile_ops_.ptr, 0xFF, std::max<uint64_t>(tiles, 1), stream_), "tile ops");
+        }
+        if (spec.want_coverage) {
+            if (!cov_a_.reserve((size_t)spec.coverage_cap * 64 * 4) || !cov_key_.reserve((size_t)spec.coverage_cap * 16)) {
+                err = "out of device memory";
+                return false;
+            }
+            fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
+        }
+        if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, image, prev_layers, tiles, err))
+            return false;
+    }
+    MG_HIP(hipEventRecord(ev1_, stream_), "event");
+    MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
+    MG_HIP(hipStreamSynchronize(stream_), "frame");
+    float ms = 0.0f;
+    MG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+    res.segments = hc.segments;
+    res.kept = hc.kept;
+    res.device_ms = ms;
+    return true;
+}
+
+bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
+                     const uint64_t *sorted, uint8_t *image, int32_t *prev_layers, size_t tiles,
+                     std::string &err) {
+    if (global_rows == 0) return true;
+    PaintParams pp{};
+    pp.segs = sorted;
+    pp.row_start = row_start_.as<uint32_t>();
+    pp.row_scene = row_scene_.as<uint32_t>();
+    pp.scenes = scenes_.as<SceneDesc>();
+    pp.props = props_.as<PropsEntry>();
+    pp.stops = stops_.as<float>();
+    pp.prev_layers = prev_layers;
+    pp.out = image;
+    pp.col_bits = col_bits;
+    pp.layer_bits = layer_bits;
+    pp.overflow = overflow_.as<uint32_t>();
+    pp.tile_ops = spec.want_tile_ops ? tile_ops_.as<int8_t>() : nullptr;
+    if (spec.want_coverage) {
+        pp.cov_a = cov_a_.as<int32_t>();
+        pp.cov_key = cov_key_.as<uint32_t>();
+        pp.cov_count = cov_count_.as<uint32_t>();
+        pp.cov_cap = spec.coverage_cap;
+    }
+    // BufferLayerCache state is updated by the painter: keep a copy so an
+    // overflow retry starts from the same state.
+    DevBuf saved;
+    if (prev_layers) {
+        if (!saved.reserve(std::max<size_t>(tiles, 1) * 4)) { err = "out of device memory"; return false; }
+        MG_HIP(hipMemcpyAsync(saved.ptr, prev_layers, tiles * 4, hipMemcpyDeviceToDevice, stream_), "save cache");
+    }
+    static const uint32_t caps[][2] = {{256, 256}, {1024, 384}, {2048, 512}, {3072, 640}};
+    size_t lds_max = 65536;
+    {
+        hipDeviceProp_t p;
+        if (hipGetDeviceProperties(&p, device_) == hipSuccess) lds_max = p.sharedMemPerBlock;
+    }
+    for (int attempt = 0; attempt < 4; ++attempt) {
+        uint32_t cap = std::max(caps[attempt][0], cap_ids_), rcap = std::max(caps[attempt][1], cap_runs_);
+        size_t lds = paint_lds_bytes_host(cap, rcap);
+        if (lds > lds_max) break;
+        if (attempt > 0 && prev_layers)
+            MG_HIP(hipMemcpyAsync(prev_layers, saved.ptr, tiles * 4, hipMemcpyDeviceToDevice, stream_), "restore cache");
+        if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
+        fill32(overflow_.as<uint32_t>(), 0, 1, stream_);
+        pp.cap_ids = cap;
+        pp.cap_runs = rcap;
+        MG_HIP(hipFuncSetAttribute((const void *)k_paint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+               "paint lds");
+        k_paint<<<global_rows, 64, lds, stream_>>>(pp);
+        MG_HIP(hipGetLastError(), "paint launch");
+        uint32_t ov = 0;
+        MG_HIP(hipMemcpyAsync(&ov, overflow_.ptr, 4, hipMemcpyDeviceToHost, stream_), "read overflow");
+        MG_HIP(hipStreamSynchronize(stream_), "paint");
+        if (!ov) {
+            cap_ids_ = cap;
+            cap_runs_ = rcap;
+            saved.release();
+            return true;
+        }
+    }
+    saved.release();
+    err = "a tile holds more layers than the painter's LDS capacity";
+    return false;
+}
+
+bool Renderer::download_sorted(uint64_t *dst, size_t n, std::string &err) {
+    if (!n) return true;
+    MG_HIP(hipMemcpyAsync(dst, last_sorted_, n * 8, hipMemcpyDeviceToHost, stream_), "download sorted");
+    MG_HIP(hipStreamSynchronize(stream_), "sync");
+    return true;
+}
+
+bool Renderer::download_tile_ops(int8_t *dst, size_t n, std::string &err) {
+    if (!n) return true;
+    MG_HIP(hipMemcpyAsync(dst, tile_ops_.ptr, n, hipMemcpyDeviceToHost, stream_), "download tile ops");
+    MG_HIP(hipStreamSynchronize(stream_), "sync");
+    return true;
+}
+
+bool Renderer::download_coverage(std::vector<int32_t> &a, std::vector<uint32_t> &key, uint32_t &count,
+                                 std::string &err) {
+    MG_HIP(hipMemcpyAsync(&count, cov_count_.ptr, 4, hipMemcpyDeviceToHost, stream_), "cov count");
+    MG_HIP(hipStreamSynchronize(stream_), "sync");
+    size_t n = std::min<size_t>(count, cov_a_.bytes / 256);
+    a.resize(n * 64);
+    key.resize(n * 4);
+    if (n) {
+        MG_HIP(hipMemcpyAsync(a.data(), cov_a_.ptr, n * 256, hipMemcpyDeviceToHost, stream_), "cov a");
+        MG_HIP(hipMemcpyAsync(key.data(), cov_key_.ptr, n * 16, hipMemcpyDeviceToHost, stream_), "cov key");
+        MG_HIP(hipStreamSynchronize(stream_), "sync");
+    }
+    return true;
+}
+
+}  // namespace mg

@@ -1,0 +1,94 @@
+// Host/device shared definitions of the MI355X mold renderer.
+//
+// PixelSegment wire format (surpass/src/rasterizer/raster_segment.rs:142-154),
+// MSB -> LSB:
+//   is_none:1 | tile_j:12 (signed) | tile_i:13 (max(tx,-1)+256) | layer:16 |
+//   tile_y:3 | tile_x:3 | area:10 (signed) | cover:6 (signed)
+//
+// Render-order word used between emission, sort and paint (not exported):
+//   [63 .. 22] key = global_tile_row << (col_bits + layer_bits)
+//                    | (col + 1) << layer_bits | layer      (col = -1: left of canvas)
+//   [21 ..  0] the PixelSegment's tile_y | tile_x | area | cover bits, unchanged.
+#pragma once
+
+#include <stdint.h>
+
+namespace mg {
+
+constexpr int kPayloadBits = 22;
+constexpr uint64_t kNone = 1ull << 63;
+constexpr uint32_t kTileW = 8;
+constexpr uint32_t kTileH = 8;
+
+// Layer table entry, indexed by scene layer id (surpass/src/layer.rs:5-20).
+enum : uint32_t { LF_EXISTS = 1, LF_ENABLED = 2, LF_AFFINE = 4, LF_ORDER = 8 };
+struct LayerEntry {
+    uint32_t flags;
+    uint32_t order;
+    float affine[6];
+};
+
+// Props entry, indexed by scene order (painter/mod.rs:137-141, style.rs:426-432).
+enum : uint32_t {
+    PF_DEFINED = 1u << 0,
+    PF_EVENODD = 1u << 1,
+    PF_CLIP = 1u << 2,      // Func::Clip
+    PF_CLIPPED = 1u << 3,   // Style::is_clipped
+    PF_GRADIENT = 1u << 4,  // Fill::Gradient
+    PF_UNCHANGED = 1u << 5, // Layer::is_unchanged(cache id) for this frame
+    PF_OPAQUE = 1u << 6,    // Draw + Solid + Over + alpha == 1.0
+    PF_RADIAL = 1u << 7,
+    PF_BLEND_SHIFT = 8,     // 4 bits of BlendMode
+};
+struct PropsEntry {
+    uint32_t flags;
+    uint32_t clip_layers;
+    float color[4];
+    float grad_start[2];
+    float grad_end[2];
+    uint32_t stops_offset;  // into the stops array, 5 floats per stop
+    uint32_t n_stops;
+};
+
+// One scene of a frame (a plain render is a batch of one scene).
+enum : uint32_t { SF_CLIPS = 1, SF_CACHE = 2, SF_GRADIENTS = 4 };
+struct SceneDesc {
+    uint32_t line_base, n_lines;
+    uint32_t layer_base, n_layer_entries;
+    uint32_t props_base, n_props;
+    uint32_t width, rows_px, stride;  // pixels
+    uint32_t row_len, tile_rows;
+    uint32_t row_base;                // first global tile row of the scene
+    uint32_t crop_h0, crop_h1, crop_v0, crop_v1;  // tiles
+    uint32_t cache_base;              // into previous_layers (SF_CACHE)
+    uint32_t flags;
+    uint64_t out_offset;              // byte offset of pixel (0, 0) in the output
+    float clear[4];
+};
+
+// Per-line setup record written by the setup kernel (segment.rs:144-219).
+struct LineSetup {
+    int32_t start;
+    float start_f32, end_f32;
+    float slope, slope_recip, py_slope_px, px_slope_recip_py;
+    uint32_t packed;  // order | octant << 16 | scene << 19
+};
+
+struct FrameDims {
+    uint32_t n_scenes;
+    uint32_t n_lines;
+    uint32_t global_rows;
+    uint32_t col_bits, layer_bits, key_bits;
+    uint32_t radix_bits, passes;
+    uint32_t mode;  // 0: render (cull + render key), 1: canonical full stream
+};
+
+// Device-side counters of a frame.
+struct FrameCounters {
+    uint64_t line_pixels;     // P
+    uint64_t segments;        // non-NONE pixel segments
+    uint64_t kept;            // segments emitted for sorting
+    uint64_t rows_total;      // sum of per-tile-row counts (== kept in render mode)
+};
+
+}  // namespace mg

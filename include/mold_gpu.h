@@ -1,0 +1,234 @@
+/*
+ * mold_gpu.h -- C ABI of the MI355X-native mold/surpass renderer.
+ *
+ * The entry points mirror mold's public scene API (mold/src/lib.rs:15-25) as
+ * opaque handles plus plain pointers and sizes, so a Rust, Python or C caller
+ * can bind them directly (the reference's only FFI precedent is the dlopen'd
+ * cassia ABI, cassia/src/Cassia.h:32-41, bound by demo/src/cassia.rs:15-56;
+ * its three entry points are reproduced at the end of this file).
+ *
+ * Conventions
+ *   - Every function returns an mg_status (MG_OK == 0). No exception or panic
+ *     crosses the ABI; conditions where the reference panics return an error
+ *     code and mg_last_error() describes them.
+ *   - Colours are linear f32 RGBA; buffers are RGBA8, row-major, 4 bytes per
+ *     pixel, `width_stride` pixels per row.
+ *   - A composition is single-owner (the reference's Composition is !Send,
+ *     composition.rs:33): use one per thread or serialise calls.
+ *   - Rendering runs on the GPU (gfx950). There is no CPU fallback: without a
+ *     usable device, mg_render returns MG_ERR_DEVICE.
+ */
+#ifndef MOLD_GPU_H
+#define MOLD_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define MG_EXPORT __attribute__((visibility("default")))
+#else
+#define MG_EXPORT
+#endif
+
+typedef enum mg_status {
+    MG_OK = 0,
+    MG_ERR_INVALID_ARG = 1,      /* bad pointer / size / enum value                     */
+    MG_ERR_LAYER_SCALE = 2,      /* Layer::set_transform scale check (layer.rs:186-190)  */
+    MG_ERR_IDS_EXHAUSTED = 3,    /* create_layer -> None (layer.rs:61-72)                */
+    MG_ERR_NO_LAYER = 4,         /* unknown layer id                                     */
+    MG_ERR_OOM = 5,              /* host or device allocation failed                     */
+    MG_ERR_DEVICE = 6,           /* no gfx950 device, or a HIP call failed               */
+    MG_ERR_BUFFER = 7,           /* buffer/width/stride mismatch (buffer_layout.rs:81-92,
+                                    180-185)                                             */
+    MG_ERR_GRADIENT = 8,         /* gradient stop outside [0, 1] (style.rs:58-60)        */
+    MG_ERR_CAPACITY = 9,         /* a tile exceeds the painter's layer capacity          */
+    MG_ERR_NO_CACHE = 10,        /* all 32 buffer layer caches in use (layer.rs:137-141) */
+} mg_status;
+
+/* Last error message of the calling thread ("" if none). */
+MG_EXPORT const char *mg_last_error(void);
+
+/* Number of usable gfx950 devices (0 when none). */
+MG_EXPORT int mg_device_count(void);
+
+/* ---------------------------------------------------------------- styles */
+
+typedef enum mg_fill_rule { MG_NONZERO = 0, MG_EVENODD = 1 } mg_fill_rule;   /* style.rs:9-19 */
+
+typedef enum mg_blend_mode {                                                  /* style.rs:189-203 */
+    MG_BLEND_OVER = 0, MG_BLEND_MULTIPLY, MG_BLEND_SCREEN, MG_BLEND_OVERLAY, MG_BLEND_DARKEN,
+    MG_BLEND_LIGHTEN, MG_BLEND_COLOR_DODGE, MG_BLEND_COLOR_BURN, MG_BLEND_HARD_LIGHT,
+    MG_BLEND_SOFT_LIGHT, MG_BLEND_DIFFERENCE, MG_BLEND_EXCLUSION
+} mg_blend_mode;
+
+typedef enum mg_func { MG_FUNC_DRAW = 0, MG_FUNC_CLIP = 1 } mg_func;          /* painter/mod.rs:125-135 */
+typedef enum mg_fill_kind { MG_FILL_SOLID = 0, MG_FILL_GRADIENT = 1 } mg_fill_kind;
+typedef enum mg_gradient_type { MG_GRADIENT_LINEAR = 0, MG_GRADIENT_RADIAL = 1 } mg_gradient_type;
+
+/* surpass::painter::Props (painter/mod.rs:137-141) with its Style
+   (style.rs:426-432) flattened. Gradient stops are (r, g, b, a, stop)
+   quintuples; a stop of -0.0 (or +0.0) is resolved to i / (n - 1) as
+   GradientBuilder::build does (style.rs:66-84). */
+typedef struct mg_props {
+    uint32_t fill_rule;        /* mg_fill_rule, the painter's rule            */
+    uint32_t func;             /* mg_func                                     */
+    uint32_t clip_layers;      /* Func::Clip(n)                               */
+    uint32_t is_clipped;       /* Style::is_clipped                           */
+    uint32_t style_fill_rule;  /* Style::fill_rule (kept, unused by painting) */
+    uint32_t blend_mode;       /* mg_blend_mode                               */
+    uint32_t fill_kind;        /* mg_fill_kind                                */
+    float color[4];            /* Fill::Solid                                 */
+    uint32_t gradient_type;    /* mg_gradient_type                            */
+    float gradient_start[2];
+    float gradient_end[2];
+    uint32_t n_stops;          /* >= 2 for a gradient fill                    */
+    const float *stops;        /* n_stops x 5 floats                          */
+} mg_props;
+
+/* ------------------------------------------------------------------ paths */
+
+typedef struct mg_path mg_path;
+
+/* Path::new (mold/src/path/mod.rs:38-40) */
+MG_EXPORT mg_status mg_path_create(mg_path **out);
+MG_EXPORT void mg_path_destroy(mg_path *path);
+/* Path::line(p0, p1): pts = {x0, y0, x1, y1} (path/mod.rs:53-55) */
+MG_EXPORT mg_status mg_path_line(mg_path *path, const float pts[4]);
+/* Path::quad: {x0,y0,x1,y1,x2,y2} (path/mod.rs:69-71) */
+MG_EXPORT mg_status mg_path_quad(mg_path *path, const float pts[6]);
+/* Path::rat_quad: {x0,y0,w0, x1,y1,w1, x2,y2,w2} (path/mod.rs:86-97) */
+MG_EXPORT mg_status mg_path_rat_quad(mg_path *path, const float pts[9]);
+/* Path::cubic: 4 points (path/mod.rs:112-114) */
+MG_EXPORT mg_status mg_path_cubic(mg_path *path, const float pts[8]);
+/* Path::rat_cubic: 4 weighted points (path/mod.rs:130-143) */
+MG_EXPORT mg_status mg_path_rat_cubic(mg_path *path, const float pts[12]);
+/* Path::close (path/mod.rs:168-170) */
+MG_EXPORT mg_status mg_path_close(mg_path *path);
+/* Flatten to line segments (PathSegments, path/segments.rs:233-263); `t9`
+   may be NULL. Writes at most `cap` segments (4 floats each) and returns the
+   total count in *n. */
+MG_EXPORT mg_status mg_path_flatten(const mg_path *path, const float *t9, float *out, size_t cap,
+                                    size_t *n);
+
+/* ------------------------------------------------------------ composition */
+
+typedef struct mg_composition mg_composition;
+typedef uint16_t mg_layer_id;
+
+/* Composition::new (composition.rs:36-48) */
+MG_EXPORT mg_status mg_composition_create(mg_composition **out);
+MG_EXPORT void mg_composition_destroy(mg_composition *comp);
+/* Composition::create_layer -> Option<LayerId> (composition.rs:54-56);
+   MG_ERR_IDS_EXHAUSTED for None. Ids are handed out smallest-free-first. */
+MG_EXPORT mg_status mg_create_layer(mg_composition *comp, mg_layer_id *out);
+/* Composition::insert_in_layer (composition.rs:82-84) */
+MG_EXPORT mg_status mg_insert_in_layer(mg_composition *comp, mg_layer_id id, const mg_path *path);
+/* Composition::insert_in_layer_transformed (composition.rs:86-94) */
+MG_EXPORT mg_status mg_insert_in_layer_transformed(mg_composition *comp, mg_layer_id id,
+                                                   const mg_path *path, const float t9[9]);
+/* Bulk form of insert_in_layer for a path made only of `n` Path::line
+   commands: lines = n x {x0, y0, x1, y1}. Same effect as the path form
+   (composition.rs:58-79). */
+MG_EXPORT mg_status mg_insert_lines(mg_composition *comp, mg_layer_id id, const float *lines,
+                                    size_t n);
+/* Bulk form over many layers: lines[i] goes to ids[i]; each distinct layer
+   is treated as an insert_in_layer call (order reset to its id). */
+MG_EXPORT mg_status mg_insert_lines_multi(mg_composition *comp, const mg_layer_id *ids,
+                                          const float *lines, size_t n);
+
+/* Layer accessors (mold/src/layer.rs:152-231). MG_ERR_NO_LAYER when the id
+   has no layer entry (Composition::get_mut -> None). */
+MG_EXPORT mg_status mg_layer_exists(const mg_composition *comp, mg_layer_id id, int *out);
+MG_EXPORT mg_status mg_layer_is_enabled(const mg_composition *comp, mg_layer_id id, int *out);
+MG_EXPORT mg_status mg_layer_set_enabled(mg_composition *comp, mg_layer_id id, int enabled);
+MG_EXPORT mg_status mg_layer_transform(const mg_composition *comp, mg_layer_id id, float out[6]);
+/* Layer::set_transform: MG_ERR_LAYER_SCALE where the reference panics */
+MG_EXPORT mg_status mg_layer_set_transform(mg_composition *comp, mg_layer_id id, const float t[6]);
+MG_EXPORT mg_status mg_layer_order(const mg_composition *comp, mg_layer_id id, uint16_t *out);
+MG_EXPORT mg_status mg_layer_set_order(mg_composition *comp, mg_layer_id id, uint16_t order);
+MG_EXPORT mg_status mg_layer_props(const mg_composition *comp, mg_layer_id id, mg_props *out);
+MG_EXPORT mg_status mg_layer_set_props(mg_composition *comp, mg_layer_id id, const mg_props *props);
+/* Composition::remove_disabled (composition.rs:123-142) */
+MG_EXPORT mg_status mg_remove_disabled(mg_composition *comp);
+/* Lines held by the builder / lines of enabled layers (composition.rs:116-121) */
+MG_EXPORT mg_status mg_lines_len(const mg_composition *comp, size_t *builder_len,
+                                 size_t *actual_len);
+
+/* Composition::create_buffer_layer_cache -> Option<BufferLayerCache>
+   (composition.rs:144-151). */
+MG_EXPORT mg_status mg_create_buffer_layer_cache(mg_composition *comp, int32_t *cache_id);
+/* BufferLayerCache::clear (buffer.rs:46-51) */
+MG_EXPORT mg_status mg_buffer_layer_cache_clear(mg_composition *comp, int32_t cache_id);
+/* Drop of the last BufferLayerCache clone (buffer.rs:53-63) */
+MG_EXPORT mg_status mg_buffer_layer_cache_release(mg_composition *comp, int32_t cache_id);
+
+/* Rect::new(horizontal, vertical) in pixels (buffer_layout.rs:43-50). */
+typedef struct mg_rect {
+    uint32_t x0, x1, y0, y1;
+} mg_rect;
+
+/* Called once per written tile slice (row of <= 8 pixels) after the frame is
+   in the caller's buffer (Flusher::flush, buffer_layout.rs:52-54). */
+typedef void (*mg_flusher_fn)(void *user, uint8_t *slice_rgba, size_t n_pixels);
+
+/* mold::Buffer (buffer.rs:17-25). `len_px` = buffer.len() in pixels. */
+typedef struct mg_buffer {
+    uint8_t *rgba;            /* host pointer, len_px * 4 bytes          */
+    size_t len_px;
+    size_t width;
+    size_t width_stride;      /* 0 = width                               */
+    int32_t layer_cache;      /* -1 = none                               */
+    mg_flusher_fn flusher;    /* may be NULL                             */
+    void *flusher_user;
+} mg_buffer;
+
+/* Composition::render(buffer, background_color, crop) (composition.rs:212-328).
+   Blocking: returns when the caller's buffer holds the frame. `crop` may be
+   NULL. */
+MG_EXPORT mg_status mg_render(mg_composition *comp, const mg_buffer *buffer,
+                              const float background[4], const mg_rect *crop);
+
+/* Composition::pixel_segments (composition.rs:153-188): the canonically
+   sorted PixelSegment stream (stable by `seg >> 22` over generation order,
+   NONE tail included). The pointer stays valid until the next call on `comp`. */
+MG_EXPORT mg_status mg_pixel_segments(mg_composition *comp, const uint64_t **out, size_t *n);
+
+/* Statistics of the last render: lines, line-pixels, non-NONE segments,
+   segments kept for painting, tiles, digit passes, radix bits. */
+typedef struct mg_frame_stats {
+    uint64_t lines, line_pixels, segments, segments_painted, tiles;
+    uint32_t sort_passes, radix_bits;
+    double device_ms;          /* GPU time of the frame (hipEvents)      */
+} mg_frame_stats;
+MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
+
+/* ------------------------------------------------------ cassia-compatible */
+
+/* cassia_init / cassia_render / cassia_shutdown (cassia/src/Cassia.h:32-41).
+   The reference paints into a window; this build paints into a device
+   RGBA8 frame of width x height read back with cassia_read_rgba8.
+   `psegments` is a PixelSegment stream (any order, NONE entries ignored);
+   stylings are indexed by the segment's layer field. */
+typedef struct CassiaStyling {
+    float fill[4];
+    uint32_t fillRule;
+    uint32_t blendMode;
+    uint32_t _padding[2];
+} CassiaStyling;
+
+MG_EXPORT void cassia_init(uint32_t width, uint32_t height);
+MG_EXPORT void cassia_render(const uint64_t *psegments, size_t psegmentCount,
+                             const CassiaStyling *stylings, size_t stylingCount);
+MG_EXPORT void cassia_shutdown(void);
+/* Not in the reference: copy the last cassia_render frame (width*height*4 bytes). */
+MG_EXPORT mg_status cassia_read_rgba8(uint8_t *out, size_t cap_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOLD_GPU_H */

@@ -1,0 +1,128 @@
+"""GPU parity: the HIP path (through the C ABI) against the CPU oracle.
+
+Bit-exact bar: PixelSegment stream (canonical order) and RGBA8 bytes."""
+import os
+
+import numpy as np
+import pytest
+
+import mold_scenes
+import scenegen
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import cassia_amd
+    assert cassia_amd.device_count() > 0, "no gfx950 device: the HIP path cannot run"
+    return cassia_amd
+
+
+@pytest.fixture(scope="module")
+def cpu():
+    from oracle import mold
+    return mold
+
+
+@pytest.mark.parametrize("scene", mold_scenes.ALL, ids=lambda f: f.__name__)
+def test_mold_kats_on_gpu(gpu, scene):
+    scene(gpu)
+
+
+def render_both(gpu, cpu, seed, w, h, n_layers, bg=(1.0, 1.0, 1.0, 1.0), stride=None, crop=None, **kw):
+    outs = []
+    for m in (cpu, gpu):
+        comp = scenegen.build_scene(m, seed, w, h, n_layers, **kw)
+        st = stride or w
+        rows = h
+        buf = np.full((st * rows, 4), 7, dtype=np.uint8)
+        c = m.Rect(range(crop[0], crop[1]), range(crop[2], crop[3])) if crop else None
+        comp.render(m.Buffer(buf, w, width_stride=stride), list(bg), c)
+        outs.append(buf)
+    return outs
+
+
+def assert_same_image(a, b, w):
+    if not np.array_equal(a, b):
+        d = np.nonzero((a != b).any(axis=1))[0]
+        y, x = d // w, d % w
+        raise AssertionError(f"{len(d)} pixels differ; first at (x={x[0]}, y={y[0]}): "
+                             f"oracle {a[d[0]].tolist()} gpu {b[d[0]].tolist()}")
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_scene_rgba8(gpu, cpu, seed):
+    w, h = 83, 61
+    a, b = render_both(gpu, cpu, seed, w, h, 24)
+    assert_same_image(a, b, w)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_scene_transforms_curves(gpu, cpu, seed):
+    w, h = 120, 96
+    a, b = render_both(gpu, cpu, 100 + seed, w, h, 40, transforms=True, curves=True)
+    assert_same_image(a, b, w)
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_scene_clips_gradients(gpu, cpu, seed):
+    w, h = 64, 72
+    a, b = render_both(gpu, cpu, 200 + seed, w, h, 30, clips=True, gradients=True)
+    assert_same_image(a, b, w)
+
+
+def test_stride_crop_and_transparent_background(gpu, cpu):
+    w, h = 77, 50
+    a, b = render_both(gpu, cpu, 300, w, h, 20, bg=(0.2, 0.4, 0.6, 0.5), stride=90, crop=(9, 60, 5, 41))
+    assert_same_image(a, b, 90)
+
+
+def test_many_layers_deep_tile(gpu, cpu):
+    # 300 overlapping layers over a small canvas: long id lists per tile
+    w, h = 48, 40
+    a, b = render_both(gpu, cpu, 400, w, h, 300)
+    assert_same_image(a, b, w)
+
+
+def test_pixel_segments_canonical_stream(gpu, cpu):
+    for seed in range(3):
+        s_cpu = scenegen.build_scene(cpu, 500 + seed, 90, 70, 25, transforms=True, curves=True).pixel_segments()
+        s_gpu = scenegen.build_scene(gpu, 500 + seed, 90, 70, 25, transforms=True, curves=True).pixel_segments()
+        assert len(s_cpu) == len(s_gpu)
+        assert np.array_equal(s_cpu, s_gpu)
+
+
+def test_empty_composition(gpu, cpu):
+    for m in (cpu, gpu):
+        buf = np.zeros((16 * 9, 4), np.uint8)
+        m.Composition().render(m.Buffer(buf, 16), [0.0, 1.0, 0.0, 1.0], None)
+        assert (buf == [0, 255, 0, 255]).all()
+
+
+def test_circles_fixture_via_cassia(gpu, orc):
+    """The reference-held circles.segs stream (with the +256 bias added,
+    cassia/tools/cassia_sim.py:37-43), painted by cassia_render vs the oracle."""
+    import ctypes as C
+    from oracle import mold as omold
+    from test_oracle_fixtures import biased_sorted_circles, load_circles
+    raw, sty = load_circles()
+    d = orc.decode(raw)
+    ti = np.maximum(d["tile_i"], -1) + 256
+    biased = (raw & ~np.uint64(0x1FFF << 38)) | ((ti.astype(np.int64) & 0x1FFF).astype(np.uint64) << np.uint64(38))
+    biased[d["is_none"] == 1] = np.uint64(orc.NONE)
+    W = H = 1000
+    gpu.cassia_init(W, H)
+    gpu.cassia_render(biased, [(sty[i, :4], 0, 0) for i in range(len(sty))])
+    img = gpu.cassia_read(W, H)
+    gpu.cassia_shutdown()
+    segs, _ = biased_sorted_circles(orc)
+    keep = []
+    pt = (orc.Props * 10)()
+    for i in range(10):
+        pt[i] = omold.c_props(omold.Props(func=omold.Func.Draw(omold.Style(fill=omold.Fill.Solid(sty[i, :4])))), keep=keep)
+    ref = np.zeros((W * H, 4), np.uint8)
+    clr = np.ones(4, np.float32)
+    assert orc.lib().or_print(orc.ptr(ref.reshape(-1), orc._u8), W * H, W, W, None, 0, orc.ptr(segs, orc._u64),
+                              len(segs), orc.ptr(clr, orc._f), None, pt, 10, None) == 0
+    assert_same_image(ref, img, W)
