@@ -1,0 +1,163 @@
+#!/usr/bin/env python3
+"""Benchmark of the mold rasterize -> sort -> paint path on one MI355X.
+
+Workload (default): the single large frame -- 3840x2160, 200,000 star /
+polygon paths of 5-50 edges, random affine transforms (rotation, scale
+0.5-2), 50/50 fill rules, uniform blend modes, alpha in [0.2, 1]
+(cassia_amd.workloads.polygons). One step = one full frame: line setup,
+scan, pixel-segment emission, the LSD radix sort and the tile-row paint into
+an RGBA8 frame. Lines, layer tables and the frame stay resident in HBM; no
+host<->device copy is inside the timed region.
+
+Prints one JSON line (see the keys in `main`).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "frames/s & pixel-segments/s batched vs plain on 1 MI355X, bit-exact; p99 ms"
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+
+
+def stage_bytes(stats, width, height, n_layers):
+    """Algorithmic HBM bytes per frame and stage.
+    emit: read each line once (16 B points + 2 B id), write each kept segment;
+    sort: read + write every key once per digit pass;
+    paint: read the sorted stream, write the RGBA8 frame, read the props."""
+    L, S, P = stats.lines, stats.segments_painted, stats.sort_passes
+    return {
+        "setup": 18 * L + 8 * L,
+        "emit": 18 * L + 8 * S,
+        "sort": 16 * S * P,
+        "paint": 8 * S + 4 * width * height + 32 * n_layers,
+    }
+
+
+def cpu_baseline(scene, fraction, threads):
+    """The oracle's multi-threaded C restatement on a bounded sample of the
+    same scene: the lines of the first `fraction` of its layers."""
+    from oracle import mold
+    from cassia_amd.workloads import Scene
+    n_layers = max(1, int(scene.n_layers * fraction))
+    keep = scene.layer_ids < n_layers
+    sample = Scene(scene.width, scene.height, scene.lines[keep], scene.layer_ids[keep],
+                   scene.props[:n_layers], scene.background, name=scene.name + "-sample")
+    comp = sample.build(mold)
+    buf = np.zeros((scene.width * scene.height, 4), np.uint8)
+    t0 = time.perf_counter()
+    segs = comp.render_mt(mold.Buffer(buf, scene.width), scene.background, threads)
+    dt = time.perf_counter() - t0
+    return dict(value=segs / dt, unit="pixel-segments/s", cores=threads, kind="port",
+                sample=f"first {n_layers} of {scene.n_layers} layers of the same scene "
+                       f"({int(keep.sum())} lines, {segs} pixel segments, {dt:.2f} s, "
+                       f"{threads} threads, C restatement of the reference CPU path)",
+                seconds=dt, frames_per_s=1.0 / dt)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", choices=["polygons", "circles"], default="polygons")
+    ap.add_argument("--paths", type=int, default=200_000)
+    ap.add_argument("--cpu-fraction", type=float, default=0.02)
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build renders on one GPU only; multi-GPU runs (tile-row bands "
+              "exchanged with one all-to-all) are not implemented", file=sys.stderr)
+        sys.exit(2)
+
+    import cassia_amd
+    from cassia_amd import workloads
+    if cassia_amd.device_count() < 1:
+        raise SystemExit("no gfx950 device")
+    if args.workload == "polygons":
+        scene = workloads.polygons(count=args.paths)
+        workload = (f"single large frame, plain GPU path: {scene.width}x{scene.height}, "
+                    f"{args.paths} star/polygon paths, 5-50 edges, affine, mixed fill rules and blend modes")
+    else:
+        scene = workloads.circles()
+        workload = "plain method: 1024x1024, 10,000 circles x 64 lines, opaque, NonZero, Over"
+    t_build = time.perf_counter()
+    comp = scene.build(cassia_amd)
+    t_build = time.perf_counter() - t_build
+    W, H = scene.width, scene.height
+
+    for _ in range(args.warmup):
+        comp.render_device(W, H, scene.background)
+    lat = []
+    stage = {"setup": 0.0, "emit": 0.0, "sort": 0.0, "paint": 0.0}
+    dev_ms = 0.0
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        a = time.perf_counter()
+        comp.render_device(W, H, scene.background)
+        lat.append((time.perf_counter() - a) * 1e3)
+        st = comp.stats()
+        stage["setup"] += st.setup_ms
+        stage["emit"] += st.emit_ms
+        stage["sort"] += st.sort_ms
+        stage["paint"] += st.paint_ms
+        dev_ms += st.device_ms
+    elapsed = time.perf_counter() - t0
+    st = comp.stats()
+    K = args.steps
+    stage = {k: v / K for k, v in stage.items()}
+    ms_per_step = elapsed * 1e3 / K
+    fps = K / elapsed
+    segs = st.segments
+    nbytes = stage_bytes(st, W, H, scene.n_layers)
+    dominant = max(stage, key=stage.get)
+    achieved = nbytes[dominant] / (stage[dominant] * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
+                "algorithmic_bytes": nbytes[dominant]}
+    out = {
+        "metric": METRIC,
+        "value": segs * fps,
+        "unit": "pixel-segments/s",
+        "n_gpus": 1,
+        "steps": K,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded numpy PCG64 scene generator)",
+        "config": {"workload": workload, "width": W, "height": H, "paths": args.paths if args.workload == "polygons" else 10000,
+                   "lines": int(st.lines), "layers": scene.n_layers},
+        "frames_per_s": fps,
+        "p50_ms": float(np.percentile(lat, 50)),
+        "p99_ms": float(np.percentile(lat, 99)),
+        "device_ms_per_frame": dev_ms / K,
+        "pixel_segments": int(segs),
+        "pixel_segments_painted": int(st.segments_painted),
+        "line_pixels": int(st.line_pixels),
+        "sort_passes": int(st.sort_passes),
+        "radix_bits": int(st.radix_bits),
+        "stage_ms": {k: round(v, 4) for k, v in stage.items()},
+        "stage_gbs": {k: round(nbytes[k] / (stage[k] * 1e-3) / 1e9, 2) if stage[k] > 0 else None for k in stage},
+        "roofline": roofline,
+        "scene_build_s": round(t_build, 2),
+    }
+    if not args.no_cpu:
+        cb = cpu_baseline(scene, args.cpu_fraction, args.cpu_threads)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

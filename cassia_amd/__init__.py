@@ -66,7 +66,8 @@ class _Buffer(C.Structure):
 class FrameStats(C.Structure):
     _fields_ = [("lines", C.c_uint64), ("line_pixels", C.c_uint64), ("segments", C.c_uint64),
                 ("segments_painted", C.c_uint64), ("tiles", C.c_uint64), ("sort_passes", C.c_uint32),
-                ("radix_bits", C.c_uint32), ("device_ms", C.c_double)]
+                ("radix_bits", C.c_uint32), ("device_ms", C.c_double), ("setup_ms", C.c_double),
+                ("emit_ms", C.c_double), ("sort_ms", C.c_double), ("paint_ms", C.c_double)]
 
 
 class CassiaStyling(C.Structure):
@@ -115,6 +116,7 @@ _sig("mg_create_buffer_layer_cache", C.c_int, _vp, C.POINTER(C.c_int32))
 _sig("mg_buffer_layer_cache_clear", C.c_int, _vp, C.c_int32)
 _sig("mg_buffer_layer_cache_release", C.c_int, _vp, C.c_int32)
 _sig("mg_render", C.c_int, _vp, C.POINTER(_Buffer), _f, C.POINTER(_Rect))
+_sig("mg_render_device", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(_vp))
 _sig("mg_pixel_segments", C.c_int, _vp, C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t))
 _sig("mg_last_frame_stats", C.c_int, _vp, C.POINTER(FrameStats))
 _sig("cassia_init", None, C.c_uint32, C.c_uint32)
@@ -548,6 +550,15 @@ class Composition:
             r = _Rect(*crop.px[:2], *crop.px[2:])
         _check(_lib.mg_render(self._h, C.byref(b), _fp(bg), C.byref(r) if r is not None else None))
         return self.stats().segments
+
+    def render_device(self, width, height, background_color, crop: Optional[Rect] = None):
+        """Render into a device-resident frame; returns its device address."""
+        bg = _f32(background_color, 4)
+        r = _Rect(*crop.px[:2], *crop.px[2:]) if crop is not None else None
+        out = _vp()
+        _check(_lib.mg_render_device(self._h, width, height, _fp(bg), C.byref(r) if r is not None else None,
+                                     C.byref(out)))
+        return out.value
 
     def stats(self):
         s = FrameStats()

@@ -118,6 +118,9 @@ struct mg_composition {
     std::unique_ptr<mg::Renderer> rdr;
     std::vector<uint64_t> pixsegs;
     mg_frame_stats stats{};
+    uint64_t version = 1;      // bumped by every change of layers, props or lines
+    uint8_t *device_frame = nullptr;
+    size_t device_frame_bytes = 0;
 
     size_t actual_len() const {
         size_t n = 0;
@@ -140,6 +143,7 @@ struct mg_composition {
                 ++w;
             }
             if (w != ids.size()) {
+                ++version;
                 pts.resize(4 * w);
                 ids.resize(w);
                 uploaded = 0;
@@ -176,6 +180,7 @@ struct mg_composition {
     }
 
     void insert(uint16_t id, const float *lines, size_t n) {
+        ++version;
         pts.insert(pts.end(), lines, lines + 4 * n);
         ids.insert(ids.end(), n, id);
         Layer &l = layers[id];
@@ -319,6 +324,7 @@ MG_EXPORT mg_status mg_insert_lines(mg_composition *c, mg_layer_id id, const flo
 
 MG_EXPORT mg_status mg_insert_lines_multi(mg_composition *c, const mg_layer_id *ids, const float *lines, size_t n) {
     if (check_comp(c) || (n && (!lines || !ids))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    ++c->version;
     c->pts.insert(c->pts.end(), lines, lines + 4 * n);
     c->ids.insert(c->ids.end(), ids, ids + n);
     for (size_t i = 0; i < n; ++i) {
@@ -356,7 +362,8 @@ static Layer *find_layer(const mg_composition *c, mg_layer_id id) {
 #define MG_LAYER(c, id, l)                                                        \
     if (check_comp(c)) return MG_ERR_INVALID_ARG;                                 \
     Layer *l = find_layer(c, id);                                                 \
-    if (!l) return fail(MG_ERR_NO_LAYER, "no layer with this id");
+    if (!l) return fail(MG_ERR_NO_LAYER, "no layer with this id");                \
+    ++const_cast<mg_composition *>(c)->version;
 
 MG_EXPORT mg_status mg_layer_exists(const mg_composition *c, mg_layer_id id, int *out) {
     if (check_comp(c) || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
@@ -485,44 +492,20 @@ MG_EXPORT mg_status mg_buffer_layer_cache_release(mg_composition *c, int32_t id)
     return MG_OK;
 }
 
-MG_EXPORT mg_status mg_render(mg_composition *c, const mg_buffer *buf, const float bg[4], const mg_rect *crop) {
-    if (check_comp(c) || !buf || !bg || !buf->rgba) return fail(MG_ERR_INVALID_ARG, "null argument");
-    mg_status s = c->ensure_renderer();
-    if (s) return s;
+// Composition::render for one frame of `width` x `rows_px` pixels at
+// `stride` into the device image `img` (composition.rs:212-328).
+static mg_status render_frame(mg_composition *c, size_t width, size_t stride, size_t rows_px, const float bg[4],
+                              const mg_rect *crop, int32_t cache_id, int32_t *prev, uint8_t *img,
+                              bool want_tile_ops, mg::FrameResult &r) {
     std::string err;
-    const int32_t cache_id = buf->layer_cache;
-    if (cache_id >= 0 && (cache_id >= 32 || !((c->caches >> cache_id) & 1u)))
-        return fail(MG_ERR_INVALID_ARG, "unknown layer cache");
-    // the layout cache is keyed by (pointer, length) (composition.rs:238-241)
-    auto key = std::make_pair((uintptr_t)buf->rgba, buf->len_px);
-    auto lit = c->layouts.find(key);
-    if (lit == c->layouts.end())
-        lit = c->layouts.emplace(key, std::make_pair(buf->width, buf->width_stride ? buf->width_stride : buf->width)).first;
-    const size_t width = lit->second.first, stride = lit->second.second;
-    const size_t len = buf->len_px;
-    // BufferLayoutBuilder::build checks (buffer_layout.rs:81-92, 103)
-    if (width > len || stride > len || width > stride || width == 0)
-        return fail(MG_ERR_BUFFER, "width / width_stride do not fit the buffer");
-    const size_t rows_px = (len + stride - 1) / stride;
-    if (len - (rows_px - 1) * stride < width) return fail(MG_ERR_BUFFER, "last buffer row is shorter than width");
     const size_t row_len = (width + 7) / 8, tile_rows = (rows_px + 7) / 8;
-    int32_t *prev = nullptr;
-    if (cache_id >= 0) {
-        // Buffer::tiles_len (buffer.rs:37-44) and the resize/clear of render
-        const size_t bw = buf->width_stride ? buf->width_stride : buf->width;
-        const size_t tiles_len = ((buf->width + 7) / 8) * (((len / bw) + 7) / 8);
-        if (tiles_len != row_len * tile_rows) return fail(MG_ERR_BUFFER, "layer cache size mismatch");
-        bool resized = false;
-        prev = c->rdr->cache_array(cache_id, tiles_len, resized, err);
-        if (!prev) return fail(MG_ERR_OOM, err);
-    }
     c->remove_disabled();
     for (auto &kv : c->layers)
         if (kv.second.enabled && kv.second.has_order) c->orders_to_layers[kv.second.order] = kv.first;
-
     mg::FrameSpec f;
     bool has_clips = false;
     c->tables(cache_id, f, has_clips);
+    f.tables_version = c->version * 64 + (uint64_t)(cache_id + 1);
     mg::SceneDesc sd{};
     sd.line_base = 0;
     sd.n_lines = (uint32_t)c->ids.size();
@@ -556,16 +539,76 @@ MG_EXPORT mg_status mg_render(mg_composition *c, const mg_buffer *buf, const flo
     f.scenes.push_back(sd);
     f.scene_line_base.push_back(0);
     f.mode = 0;
-    f.want_tile_ops = buf->flusher != nullptr;
+    f.want_tile_ops = want_tile_ops;
+    mg_status s;
     if ((s = c->sync_lines())) return s;
+    if (!c->rdr->run(f, c->ids.size(), img, prev, r, err))
+        return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
+    if (cache_id >= 0) {
+        bool changed = false;
+        for (auto &kv : c->layers) {
+            uint32_t before = kv.second.is_unchanged;
+            if (kv.second.enabled) kv.second.is_unchanged |= 1u << cache_id;
+            else kv.second.is_unchanged &= ~(1u << cache_id);
+            changed = changed || before != kv.second.is_unchanged;
+        }
+        if (changed) ++c->version;
+    }
+    c->stats.lines = c->ids.size();
+    c->stats.line_pixels = r.line_pixels;
+    c->stats.segments = r.segments;
+    c->stats.segments_painted = r.kept;
+    c->stats.tiles = r.tiles;
+    c->stats.sort_passes = r.passes;
+    c->stats.radix_bits = r.radix_bits;
+    c->stats.device_ms = r.device_ms;
+    c->stats.setup_ms = r.stage_ms[0];
+    c->stats.emit_ms = r.stage_ms[1];
+    c->stats.sort_ms = r.stage_ms[2];
+    c->stats.paint_ms = r.stage_ms[3];
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_render(mg_composition *c, const mg_buffer *buf, const float bg[4], const mg_rect *crop) {
+    if (check_comp(c) || !buf || !bg || !buf->rgba) return fail(MG_ERR_INVALID_ARG, "null argument");
+    mg_status s = c->ensure_renderer();
+    if (s) return s;
+    std::string err;
+    const int32_t cache_id = buf->layer_cache;
+    if (cache_id >= 0 && (cache_id >= 32 || !((c->caches >> cache_id) & 1u)))
+        return fail(MG_ERR_INVALID_ARG, "unknown layer cache");
+    // the layout cache is keyed by (pointer, length) (composition.rs:238-241)
+    auto key = std::make_pair((uintptr_t)buf->rgba, buf->len_px);
+    auto lit = c->layouts.find(key);
+    if (lit == c->layouts.end())
+        lit = c->layouts.emplace(key, std::make_pair(buf->width, buf->width_stride ? buf->width_stride : buf->width)).first;
+    const size_t width = lit->second.first, stride = lit->second.second;
+    const size_t len = buf->len_px;
+    // BufferLayoutBuilder::build checks (buffer_layout.rs:81-92, 103)
+    if (width > len || stride > len || width > stride || width == 0)
+        return fail(MG_ERR_BUFFER, "width / width_stride do not fit the buffer");
+    const size_t rows_px = (len + stride - 1) / stride;
+    if (len - (rows_px - 1) * stride < width) return fail(MG_ERR_BUFFER, "last buffer row is shorter than width");
+    const size_t row_len = (width + 7) / 8, tile_rows = (rows_px + 7) / 8;
+    int32_t *prev = nullptr;
+    if (cache_id >= 0) {
+        // Buffer::tiles_len (buffer.rs:37-44) and the resize/clear of render
+        const size_t bw = buf->width_stride ? buf->width_stride : buf->width;
+        const size_t tiles_len = ((buf->width + 7) / 8) * (((len / bw) + 7) / 8);
+        if (tiles_len != row_len * tile_rows) return fail(MG_ERR_BUFFER, "layer cache size mismatch");
+        bool resized = false;
+        prev = c->rdr->cache_array(cache_id, tiles_len, resized, err);
+        if (!prev) return fail(MG_ERR_OOM, err);
+    }
     const size_t bytes = len * 4;
     uint8_t *img = c->rdr->frame_image(bytes, err);
     if (!img) return fail(MG_ERR_OOM, err);
+    // the caller's pixels go first: tiles the painter skips keep them
     if (hipMemcpyAsync(img, buf->rgba, bytes, hipMemcpyHostToDevice, c->rdr->stream()) != hipSuccess)
         return fail(MG_ERR_DEVICE, "upload of the frame buffer failed");
     mg::FrameResult r;
-    if (!c->rdr->run(f, c->ids.size(), img, prev, r, err))
-        return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
+    if ((s = render_frame(c, width, stride, rows_px, bg, crop, cache_id, prev, img, buf->flusher != nullptr, r)))
+        return s;
     if (hipMemcpyAsync(buf->rgba, img, bytes, hipMemcpyDeviceToHost, c->rdr->stream()) != hipSuccess ||
         hipStreamSynchronize(c->rdr->stream()) != hipSuccess)
         return fail(MG_ERR_DEVICE, "download of the frame failed");
@@ -581,19 +624,22 @@ MG_EXPORT mg_status mg_render(mg_composition *c, const mg_buffer *buf, const flo
             }
         }
     }
-    if (cache_id >= 0)
-        for (auto &kv : c->layers) {
-            if (kv.second.enabled) kv.second.is_unchanged |= 1u << cache_id;
-            else kv.second.is_unchanged &= ~(1u << cache_id);
-        }
-    c->stats.lines = c->ids.size();
-    c->stats.line_pixels = r.line_pixels;
-    c->stats.segments = r.segments;
-    c->stats.segments_painted = r.kept;
-    c->stats.tiles = r.tiles;
-    c->stats.sort_passes = r.passes;
-    c->stats.radix_bits = r.radix_bits;
-    c->stats.device_ms = r.device_ms;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_render_device(mg_composition *c, uint32_t width, uint32_t height, const float bg[4],
+                                     const mg_rect *crop, void **device_rgba) {
+    if (check_comp(c) || !bg || !device_rgba || width == 0 || height == 0)
+        return fail(MG_ERR_INVALID_ARG, "bad argument");
+    mg_status s = c->ensure_renderer();
+    if (s) return s;
+    std::string err;
+    const size_t bytes = (size_t)width * height * 4;
+    uint8_t *img = c->rdr->frame_image(bytes, err);
+    if (!img) return fail(MG_ERR_OOM, err);
+    mg::FrameResult r;
+    if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, false, r))) return s;
+    *device_rgba = img;
     return MG_OK;
 }
 

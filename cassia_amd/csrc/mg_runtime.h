@@ -38,6 +38,7 @@ struct FrameSpec {
     bool want_tile_ops = false;
     bool want_coverage = false;
     uint32_t coverage_cap = 0;
+    uint64_t tables_version = 0;        // layers/props/stops unchanged when equal to the last upload
 };
 
 struct FrameResult {
@@ -45,6 +46,7 @@ struct FrameResult {
     uint32_t passes = 0, radix_bits = 0;
     uint64_t tiles = 0;
     double device_ms = 0.0;
+    double stage_ms[4] = {0, 0, 0, 0};  // setup+scan, emit, sort, paint
     const uint64_t *sorted = nullptr;   // device pointer to the sorted words (mode 1)
 };
 
@@ -85,6 +87,9 @@ private:
     int cus_ = 0;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    hipEvent_t evs_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    uint64_t uploaded_version_ = 0;
+    std::vector<SceneDesc> uploaded_scenes_;
     DevBuf pts_, ids_;
     size_t lines_cap_ = 0;
     DevBuf scenes_, scene_line_base_, layers_, props_, stops_;

@@ -67,6 +67,8 @@ Renderer::~Renderer() {
     for (DevBuf &b : caches_) b.release();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
+    for (hipEvent_t e : evs_)
+        if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -89,6 +91,7 @@ bool Renderer::init(std::string &err) {
     MG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     MG_HIP(hipEventCreate(&ev0_), "hipEventCreate");
     MG_HIP(hipEventCreate(&ev1_), "hipEventCreate");
+    for (hipEvent_t &e : evs_) MG_HIP(hipEventCreate(&e), "hipEventCreate");
     if (!counters_.reserve(sizeof(FrameCounters)) || !err_.reserve(16) || !overflow_.reserve(16) ||
         !part_counters_.reserve(64 * sizeof(uint32_t)) || !cov_count_.reserve(16)) {
         err = "out of device memory";
@@ -236,13 +239,21 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     res.radix_bits = R;
     res.tiles = tiles;
 
-    if (!upload_vec(scenes_, spec.scenes, stream_, err) ||
-        !upload_vec(scene_line_base_, spec.scene_line_base, stream_, err) ||
-        !upload_vec(layers_, spec.layers, stream_, err) || !upload_vec(props_, spec.props, stream_, err) ||
-        !upload_vec(stops_, spec.stops, stream_, err))
-        return false;
+    const bool same_scenes = spec.tables_version != 0 && spec.tables_version == uploaded_version_ &&
+                             uploaded_scenes_.size() == spec.scenes.size() &&
+                             std::memcmp(uploaded_scenes_.data(), spec.scenes.data(),
+                                         spec.scenes.size() * sizeof(SceneDesc)) == 0;
+    if (!same_scenes) {
+        if (!upload_vec(scenes_, spec.scenes, stream_, err) ||
+            !upload_vec(scene_line_base_, spec.scene_line_base, stream_, err) ||
+            !upload_vec(layers_, spec.layers, stream_, err) || !upload_vec(props_, spec.props, stream_, err) ||
+            !upload_vec(stops_, spec.stops, stream_, err))
+            return false;
+        uploaded_version_ = spec.tables_version;
+        uploaded_scenes_ = spec.scenes;
+    }
     // global row -> scene
-    {
+    if (!same_scenes) {
         std::vector<uint32_t> rs(std::max<uint32_t>(global_rows, 1), 0);
         for (uint32_t s = 0; s < n_scenes; ++s)
             for (uint32_t r = 0; r < spec.scenes[s].tile_rows; ++r) rs[spec.scenes[s].row_base + r] = s;
@@ -257,6 +268,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     FrameCounters *ctr = counters_.as<FrameCounters>();
     uint32_t *pc = part_counters_.as<uint32_t>();
     MG_HIP(hipEventRecord(ev0_, stream_), "event");
+    MG_HIP(hipEventRecord(evs_[0], stream_), "event");
     fill64((uint64_t *)ctr, 0, sizeof(FrameCounters) / 8, stream_);
     fill32(err_.as<uint32_t>(), 0, 1, stream_);
     fill32(pc, 0, 64, stream_);
@@ -293,6 +305,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
         return false;
     }
     res.line_pixels = ingest ? 0 : P;
+    MG_HIP(hipEventRecord(evs_[1], stream_), "event");
     const uint64_t cap_segs = std::max<uint64_t>(2 * P, 1);
     const uint64_t nparts_emit = ((ingest ? spec.n_in : P) + kEmitTileHost - 1) / kEmitTileHost;
     const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
@@ -329,6 +342,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
             &ctr->rows_total);
     }
     MG_HIP(hipGetLastError(), "emit launch");
+    MG_HIP(hipEventRecord(evs_[2], stream_), "event");
     uint64_t *in = segs_a_.as<uint64_t>(), *out = segs_b_.as<uint64_t>();
     const uint32_t sort_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nparts_sort, 1), (uint64_t)cus_ * 4);
     for (uint32_t p = 0; p < dims.passes; ++p) {
@@ -347,6 +361,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
         std::swap(in, out);
     }
     MG_HIP(hipGetLastError(), "sort launch");
+    MG_HIP(hipEventRecord(evs_[3], stream_), "event");
     last_sorted_ = in;
     res.sorted = in;
     if (spec.mode != 1) {
@@ -364,11 +379,17 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
         if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, image, prev_layers, tiles, err))
             return false;
     }
+    MG_HIP(hipEventRecord(evs_[4], stream_), "event");
     MG_HIP(hipEventRecord(ev1_, stream_), "event");
     MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
     MG_HIP(hipStreamSynchronize(stream_), "frame");
     float ms = 0.0f;
     MG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+    for (int k = 0; k < 4; ++k) {
+        float sm = 0.0f;
+        MG_HIP(hipEventElapsedTime(&sm, evs_[k], evs_[k + 1]), "elapsed");
+        res.stage_ms[k] = sm;
+    }
     res.segments = hc.segments;
     res.kept = hc.kept;
     res.device_ms = ms;

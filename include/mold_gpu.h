@@ -192,6 +192,15 @@ typedef struct mg_buffer {
 MG_EXPORT mg_status mg_render(mg_composition *comp, const mg_buffer *buffer,
                               const float background[4], const mg_rect *crop);
 
+/* Composition::render into a device-resident RGBA8 frame of width x height
+   (stride = width): no host copies, no layer cache, no flusher. `*device_rgba`
+   receives the device pointer of the frame, valid until the next render on
+   `comp`. For callers that consume frames on the GPU, and for timing the
+   path with its inputs and output resident in HBM. */
+MG_EXPORT mg_status mg_render_device(mg_composition *comp, uint32_t width, uint32_t height,
+                                     const float background[4], const mg_rect *crop,
+                                     void **device_rgba);
+
 /* Composition::pixel_segments (composition.rs:153-188): the canonically
    sorted PixelSegment stream (stable by `seg >> 22` over generation order,
    NONE tail included). The pointer stays valid until the next call on `comp`. */
@@ -203,6 +212,9 @@ typedef struct mg_frame_stats {
     uint64_t lines, line_pixels, segments, segments_painted, tiles;
     uint32_t sort_passes, radix_bits;
     double device_ms;          /* GPU time of the frame (hipEvents)      */
+    /* per stage (hipEvents on the library's stream): line setup + scan,
+       emission, sort (all digit passes), paint */
+    double setup_ms, emit_ms, sort_ms, paint_ms;
 } mg_frame_stats;
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
 

@@ -501,6 +501,30 @@ class Composition:
             if l.is_enabled_:
                 self.orders_to_layers[l.order()] = lid
 
+    def render_mt(self, buffer: Buffer, background_color, n_threads):
+        """Multi-threaded render (no layer cache, crop or flusher): the CPU
+        baseline. Returns the number of non-NONE pixel segments."""
+        self.remove_disabled()
+        for lid in sorted(self.layers_):
+            l = self.layers_[lid]
+            if l.is_enabled_:
+                self.orders_to_layers[l.order()] = lid
+        keep = []
+        ptable, np_ = self._props_table(None, keep)
+        ltable, nl = self._layer_table()
+        cols, ids = self._line_ptrs()
+        data = buffer.buffer
+        flat = data.reshape(-1)
+        stride = buffer.width_stride or buffer.width
+        nseg = C.c_size_t(0)
+        clear = f32(background_color)
+        rc = lib().or_render_mt(int(n_threads), len(ids), *(ptr(c, _f) for c in cols), ptr(ids, _u16),
+                                ltable, nl, ptr(flat, _u8), len(data), buffer.width, stride,
+                                ptr(clear, _f), ptable, np_, C.byref(nseg))
+        if rc != 0:
+            raise RuntimeError("oracle render failed")
+        return nseg.value
+
     def render(self, buffer: Buffer, background_color, crop: Optional[Rect] = None, debug=None):
         """composition.rs:212-328"""
         cache = buffer.layer_cache
