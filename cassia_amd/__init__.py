@@ -117,6 +117,7 @@ _sig("mg_buffer_layer_cache_clear", C.c_int, _vp, C.c_int32)
 _sig("mg_buffer_layer_cache_release", C.c_int, _vp, C.c_int32)
 _sig("mg_render", C.c_int, _vp, C.POINTER(_Buffer), _f, C.POINTER(_Rect))
 _sig("mg_render_device", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(_vp))
+_sig("mg_set_tuning", C.c_int, _vp, C.c_uint32)
 _sig("mg_pixel_segments", C.c_int, _vp, C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t))
 _sig("mg_last_frame_stats", C.c_int, _vp, C.POINTER(FrameStats))
 _sig("cassia_init", None, C.c_uint32, C.c_uint32)
@@ -559,6 +560,10 @@ class Composition:
         _check(_lib.mg_render_device(self._h, width, height, _fp(bg), C.byref(r) if r is not None else None,
                                      C.byref(out)))
         return out.value
+
+    def set_tuning(self, radix_bits=0):
+        """Result-invariant knob: the sort's digit width in bits (4, 6, 8; 0 = automatic)."""
+        _check(_lib.mg_set_tuning(self._h, radix_bits))
 
     def stats(self):
         s = FrameStats()

@@ -119,6 +119,7 @@ struct mg_composition {
     std::vector<uint64_t> pixsegs;
     mg_frame_stats stats{};
     uint64_t version = 1;      // bumped by every change of layers, props or lines
+    uint32_t radix_bits = 0;
     uint8_t *device_frame = nullptr;
     size_t device_frame_bytes = 0;
 
@@ -540,6 +541,7 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.scene_line_base.push_back(0);
     f.mode = 0;
     f.want_tile_ops = want_tile_ops;
+    if (c->radix_bits) f.radix_bits = c->radix_bits;
     mg_status s;
     if ((s = c->sync_lines())) return s;
     if (!c->rdr->run(f, c->ids.size(), img, prev, r, err))
@@ -658,6 +660,7 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
     f.scenes.push_back(sd);
     f.scene_line_base.push_back(0);
     f.mode = 1;
+    if (c->radix_bits) f.radix_bits = c->radix_bits;
     if ((s = c->sync_lines())) return s;
     std::string err;
     mg::FrameResult r;
@@ -666,6 +669,14 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
     if (!c->rdr->download_sorted(c->pixsegs.data(), r.kept, err)) return fail(MG_ERR_DEVICE, err);
     *out = c->pixsegs.data();
     *n = c->pixsegs.size();
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_set_tuning(mg_composition *c, uint32_t radix_bits) {
+    if (check_comp(c)) return MG_ERR_INVALID_ARG;
+    if (radix_bits != 0 && radix_bits != 4 && radix_bits != 6 && radix_bits != 8)
+        return fail(MG_ERR_INVALID_ARG, "radix_bits must be 0, 4, 6 or 8");
+    c->radix_bits = radix_bits;
     return MG_OK;
 }
 

@@ -26,6 +26,14 @@ struct PaintParams {
     uint32_t *cov_count;
     uint32_t cov_cap;
     int8_t *tile_ops;
+    // carry snapshots: per tile (stream position, queue length, offset), and
+    // the queue entries (layer | even_odd << 16, 8 row covers)
+    uint4 *snap_hdr;
+    uint32_t *snap_key;
+    uint64_t *snap_cov;
+    uint32_t *snap_used;
+    uint64_t snap_cap;
+    const uint2 *work;           // (global row, tile column) per paint work group
 };
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n);
@@ -37,16 +45,21 @@ __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64
                            uint32_t *part_counter, uint64_t *total);
 __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
                        FrameDims dims, FrameCounters *counters, uint64_t *out, uint64_t capacity,
-                       uint64_t *status, uint32_t *part_counter, uint32_t *hist, uint32_t *row_count);
+                       uint64_t *status, uint32_t *part_counter);
+__global__ void k_histogram(const uint64_t *words, const FrameCounters *counters, FrameDims dims,
+                            uint32_t *hist);
+__global__ void k_row_starts(const uint64_t *sorted, const FrameCounters *counters, uint32_t rows,
+                             uint32_t row_shift, uint32_t *row_start);
 __global__ void k_ingest(const uint64_t *in, uint64_t n_in, const SceneDesc *scenes, FrameDims dims,
-                         FrameCounters *counters, uint64_t *out, uint64_t *status, uint32_t *part_counter,
-                         uint32_t *hist, uint32_t *row_count);
+                         FrameCounters *counters, uint64_t *out, uint64_t *status, uint32_t *part_counter);
 __global__ void k_digit_bases(const uint32_t *hist, uint32_t *bases, uint32_t radix);
 template <int R>
 __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
                             uint32_t shift, const uint32_t *digit_base, uint32_t *status,
                             uint32_t *part_counter);
+__global__ void k_carry(PaintParams pp);
 __global__ void k_paint(PaintParams pp);
+size_t carry_lds_bytes_host(uint32_t cap, uint32_t rcap);
 
 size_t paint_lds_bytes_host(uint32_t cap, uint32_t rcap);
 

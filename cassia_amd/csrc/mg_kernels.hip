@@ -25,17 +25,20 @@ namespace mg {
 __device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
 __device__ __forceinline__ uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
 
+// Look-back status words carry their whole payload (a count and a flag) in
+// one atomic word and no other memory is published through them, so relaxed
+// device-coherent accesses suffice: no cache invalidation per spin.
 __device__ __forceinline__ uint64_t ld_acquire(const uint64_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_release(uint64_t *p, uint64_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ uint32_t ld_acquire32(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ void st_release32(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Decoupled look-back status words: 2 flag bits above a 62-bit value.
@@ -308,152 +311,148 @@ __device__ __forceinline__ bool render_word(uint64_t w, const SceneDesc &S, cons
     return true;
 }
 
-// Emission: one thread per kEmitIPT consecutive line-pixels (blocked), a
-// persistent grid claiming partitions of kEmitTile line-pixels in order.
-// Output positions follow generation order (line, pixel, slot), so the
+// Emission: one thread per kEmitIPT consecutive line-pixels (blocked), one
+// partition of kEmitTile line-pixels per work group, partitions claimed in
+// order. Output positions follow generation order (line, pixel, slot), so the
 // stable sort that follows yields the canonical order.
 __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ lines,
                                               const uint32_t *__restrict__ offsets,
                                               const SceneDesc *__restrict__ scenes, FrameDims dims,
                                               FrameCounters *__restrict__ counters, uint64_t *__restrict__ out,
-                                              uint64_t capacity, uint64_t *status, uint32_t *part_counter,
-                                              uint32_t *__restrict__ hist /* passes x radix */,
-                                              uint32_t *__restrict__ row_count) {
+                                              uint64_t capacity, uint64_t *status, uint32_t *part_counter) {
     __shared__ uint32_t s_part, wsum[4];
     __shared__ uint64_t s_excl;
     __shared__ uint32_t s_line[2];
-    __shared__ uint32_t s_hist[kMaxPasses * 256];
     __shared__ uint32_t s_off[kEmitTile + 1];
-    const uint32_t radix = 1u << dims.radix_bits;
-    const bool lds_hist = dims.passes * radix <= kMaxPasses * 256;
-    for (uint32_t k = threadIdx.x; k < kMaxPasses * 256; k += blockDim.x) s_hist[k] = 0;
     const uint64_t P = counters->line_pixels;
     const uint32_t nparts = (uint32_t)((P + kEmitTile - 1) / kEmitTile);
     const uint32_t lane = lane_id();
-    uint64_t seg_total = 0;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
-        __syncthreads();
-        const uint32_t part = s_part;
-        if (part >= nparts) break;
-        const uint64_t b0 = (uint64_t)part * kEmitTile;
-        const uint64_t b1 = b0 + kEmitTile < P ? b0 + kEmitTile : P;
-        // lines covering [b0, b1): last line with offset <= x
-        if (threadIdx.x < 2) {
-            uint64_t x = threadIdx.x == 0 ? b0 : b1 - 1;
-            uint32_t lo = 0, hi = dims.n_lines;
+    if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
+    __syncthreads();
+    const uint32_t part = s_part;
+    if (part >= nparts) return;
+    const uint64_t b0 = (uint64_t)part * kEmitTile;
+    const uint64_t b1 = b0 + kEmitTile < P ? b0 + kEmitTile : P;
+    // lines covering [b0, b1): last line with offset <= x
+    if (threadIdx.x < 2) {
+        uint64_t x = threadIdx.x == 0 ? b0 : b1 - 1;
+        uint32_t lo = 0, hi = dims.n_lines;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if ((uint64_t)offsets[mid] <= x) lo = mid; else hi = mid;
+        }
+        s_line[threadIdx.x] = lo;
+    }
+    __syncthreads();
+    const uint32_t l0 = s_line[0], l1 = s_line[1];
+    const bool in_lds = l1 - l0 + 1 <= kEmitTile;
+    if (in_lds)
+        for (uint32_t k = threadIdx.x; k <= l1 - l0; k += blockDim.x) s_off[k] = offsets[l0 + k];
+    __syncthreads();
+    const uint64_t t0 = b0 + (uint64_t)threadIdx.x * kEmitIPT;
+    uint64_t segs[2 * kEmitIPT];
+    uint32_t cnt = 0, produced = 0;
+    if (t0 < b1) {
+        uint32_t li;
+        if (in_lds) {
+            uint32_t lo = 0, hi = l1 - l0 + 1;
             while (hi - lo > 1) {
                 uint32_t mid = (lo + hi) >> 1;
-                if ((uint64_t)offsets[mid] <= x) lo = mid; else hi = mid;
+                if ((uint64_t)s_off[mid] <= t0) lo = mid; else hi = mid;
             }
-            s_line[threadIdx.x] = lo;
+            li = l0 + lo;
+        } else {
+            uint32_t lo = l0, hi = l1 + 1;
+            while (hi - lo > 1) {
+                uint32_t mid = (lo + hi) >> 1;
+                if ((uint64_t)offsets[mid] <= t0) lo = mid; else hi = mid;
+            }
+            li = lo;
         }
-        __syncthreads();
-        const uint32_t l0 = s_line[0], l1 = s_line[1];
-        const bool in_lds = l1 - l0 + 1 <= kEmitTile;
-        if (in_lds)
-            for (uint32_t k = threadIdx.x; k <= l1 - l0; k += blockDim.x) s_off[k] = offsets[l0 + k];
-        __syncthreads();
-        // this thread's line-pixels
-        const uint64_t t0 = b0 + (uint64_t)threadIdx.x * kEmitIPT;
-        uint64_t segs[2 * kEmitIPT];
-        uint32_t rows[2 * kEmitIPT];
-        uint32_t cnt = 0;
-        if (t0 < b1) {
-            uint32_t li;
-            if (in_lds) {
-                uint32_t lo = 0, hi = l1 - l0 + 1;
-                while (hi - lo > 1) {
-                    uint32_t mid = (lo + hi) >> 1;
-                    if ((uint64_t)s_off[mid] <= t0) lo = mid; else hi = mid;
-                }
-                li = l0 + lo;
-            } else {
-                uint32_t lo = l0, hi = l1 + 1;
-                while (hi - lo > 1) {
-                    uint32_t mid = (lo + hi) >> 1;
-                    if ((uint64_t)offsets[mid] <= t0) lo = mid; else hi = mid;
-                }
-                li = lo;
-            }
-            LineSetup L = lines[li];
-            uint64_t next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
-            // lengths of 0 share offsets: skip to the line that owns t0
-            while (next <= t0) {
+        LineSetup L = lines[li];
+        uint64_t first = offsets[li];
+        uint64_t next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
+        while (next <= t0) {  // zero-length lines share offsets
+            ++li;
+            L = lines[li];
+            first = next;
+            next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
+        }
+        uint32_t sc = L.packed >> 19;
+        const SceneDesc *S = scenes + sc;
+        for (uint32_t k = 0; k < kEmitIPT; ++k) {
+            uint64_t p = t0 + k;
+            if (p >= b1) break;
+            while (p >= next) {
                 ++li;
                 L = lines[li];
+                first = next;
                 next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
+                S = scenes + (L.packed >> 19);
             }
-            uint32_t sc = L.packed >> 19;
-            SceneDesc S = scenes[sc];
-            for (uint32_t k = 0; k < kEmitIPT; ++k) {
-                uint64_t p = t0 + k;
-                if (p >= b1) break;
-                while (p >= next) {
-                    ++li;
-                    L = lines[li];
-                    next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
-                    uint32_t s2 = L.packed >> 19;
-                    if (s2 != sc) { sc = s2; S = scenes[sc]; }
-                }
-                uint64_t w[2];
-                int n = rasterize_pixel(L, (int32_t)(p - offsets[li]), w);
-                seg_total += (uint64_t)n;
-                for (int q = 0; q < n; ++q) {
-                    if (dims.mode == 0) {
-                        uint64_t rw;
-                        uint32_t grow;
-                        if (render_word(w[q], S, dims, &rw, &grow)) {
-                            segs[cnt] = rw;
-                            rows[cnt] = grow;
-                            ++cnt;
-                        }
-                    } else {
-                        segs[cnt] = w[q];
-                        rows[cnt] = 0;
-                        ++cnt;
-                    }
+            uint64_t w[2];
+            int n = rasterize_pixel(L, (int32_t)(p - first), w);
+            produced += (uint32_t)n;
+            for (int q = 0; q < n; ++q) {
+                if (dims.mode == 0) {
+                    uint64_t rw;
+                    uint32_t grow;
+                    if (render_word(w[q], *S, dims, &rw, &grow)) segs[cnt++] = rw;
+                } else {
+                    segs[cnt++] = w[q];
                 }
             }
         }
-        uint32_t agg;
-        uint32_t ex = block_excl_scan_256(cnt, wsum, &agg);
-        if (threadIdx.x == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
-        if (threadIdx.x < 64) {
-            uint64_t e = part ? lookback_wave(status, part) : 0;
-            if (threadIdx.x == 0) {
-                s_excl = e;
-                if (part) st_release(status + part, kStInc | (e + agg));
-                if (part == nparts - 1) counters->kept = e + agg;
-            }
+    }
+    uint32_t agg;
+    uint32_t ex = block_excl_scan_256(cnt, wsum, &agg);
+    if (threadIdx.x == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
+    if (threadIdx.x < 64) {
+        uint64_t e = part ? lookback_wave(status, part) : 0;
+        if (threadIdx.x == 0) {
+            s_excl = e;
+            if (part) st_release(status + part, kStInc | (e + agg));
+            if (part == nparts - 1) counters->kept = e + agg;
         }
-        __syncthreads();
-        const uint64_t dst = s_excl + ex;
-        for (uint32_t q = 0; q < cnt; ++q) {
-            uint64_t w = segs[q];
-            if (dst + q < capacity) out[dst + q] = w;
+    }
+    __syncthreads();
+    const uint64_t dst = s_excl + ex;
+    for (uint32_t q = 0; q < cnt; ++q)
+        if (dst + q < capacity) out[dst + q] = segs[q];
+    uint32_t prod = produced;
+    for (int off = 32; off > 0; off >>= 1) prod += __shfl_xor(prod, off);
+    if (lane == 0 && prod) atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)prod);
+}
+
+// Digit histograms of every sort pass in one read of the stream (persistent
+// grid, 8 words per thread in flight, LDS histograms flushed once per work
+// group).
+__global__ __launch_bounds__(256) void k_histogram(const uint64_t *__restrict__ words,
+                                                   const FrameCounters *__restrict__ counters, FrameDims dims,
+                                                   uint32_t *__restrict__ hist) {
+    __shared__ uint32_t s_hist[kMaxPasses * 256];
+    const uint32_t radix = 1u << dims.radix_bits;
+    const bool lds_hist = dims.passes * radix <= kMaxPasses * 256;
+    for (uint32_t k = threadIdx.x; k < kMaxPasses * 256; k += blockDim.x) s_hist[k] = 0;
+    __syncthreads();
+    const uint64_t n = counters->kept;
+    constexpr uint32_t IPT = 8;
+    const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * IPT;
+    for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * IPT; base < n; base += stride) {
+        uint64_t w[IPT];
+#pragma unroll
+        for (uint32_t k = 0; k < IPT; ++k) {
+            const uint64_t i = base + k * blockDim.x + threadIdx.x;
+            w[k] = i < n ? words[i] : ~0ull;
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < IPT; ++k) {
+            if (w[k] == ~0ull) continue;
             uint32_t shift = kPayloadBits;
             for (uint32_t ps = 0; ps < dims.passes; ++ps, shift += dims.radix_bits) {
-                uint32_t dg = (uint32_t)(w >> shift) & (radix - 1);
+                const uint32_t dg = (uint32_t)(w[k] >> shift) & (radix - 1);
                 if (lds_hist) atomicAdd(&s_hist[ps * radix + dg], 1u);
                 else atomicAdd(&hist[ps * radix + dg], 1u);
-            }
-        }
-        // per-tile-row counts, aggregated across the wave
-        if (dims.mode == 0) {
-            for (uint32_t q = 0; q < 2 * kEmitIPT; ++q) {
-                bool has = q < cnt;
-                uint32_t r = has ? rows[q] : 0;
-                uint64_t pending = __ballot(has);
-                while (pending) {
-                    uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1;
-                    uint32_t lr = __shfl(r, leader);
-                    uint64_t same = __ballot(has && r == lr);
-                    if (lane == leader) atomicAdd(&row_count[lr], (uint32_t)__popcll(same));
-                    if (has && r == lr) has = false;
-                    pending &= ~same;
-                }
             }
         }
     }
@@ -461,95 +460,67 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     if (lds_hist)
         for (uint32_t k = threadIdx.x; k < dims.passes * radix; k += blockDim.x)
             if (s_hist[k]) atomicAdd(&hist[k], s_hist[k]);
-    for (int off = 32; off > 0; off >>= 1) seg_total += __shfl_xor(seg_total, off);
-    if (lane == 0 && seg_total) atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)seg_total);
+}
+
+// First sorted word of every global tile row (row_start[rows] = kept): one
+// binary search per row over the sorted render words.
+__global__ void k_row_starts(const uint64_t *__restrict__ sorted, const FrameCounters *__restrict__ counters,
+                             uint32_t rows, uint32_t row_shift, uint32_t *__restrict__ row_start) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r > rows) return;
+    const uint64_t n = counters->kept;
+    uint64_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if ((sorted[mid] >> row_shift) < r) lo = mid + 1; else hi = mid;
+    }
+    row_start[r] = (uint32_t)lo;
 }
 
 // Ingest of an external PixelSegment stream (the cassia_render entry,
 // cassia/src/Cassia.h:34-39): NONE entries and segments of layers without
-// styling are dropped, the rest become render words in input order, with the
-// same digit histograms and row counts as k_emit. Scene 0 only.
+// styling are dropped, the rest become render words in input order. Scene 0.
 __global__ __launch_bounds__(256) void k_ingest(const uint64_t *__restrict__ in, uint64_t n_in,
                                                 const SceneDesc *__restrict__ scenes, FrameDims dims,
                                                 FrameCounters *__restrict__ counters, uint64_t *__restrict__ out,
-                                                uint64_t *status, uint32_t *part_counter,
-                                                uint32_t *__restrict__ hist, uint32_t *__restrict__ row_count) {
+                                                uint64_t *status, uint32_t *part_counter) {
     __shared__ uint32_t s_part, wsum[4];
     __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_hist[kMaxPasses * 256];
-    const uint32_t radix = 1u << dims.radix_bits;
-    const bool lds_hist = dims.passes * radix <= kMaxPasses * 256;
-    for (uint32_t k = threadIdx.x; k < kMaxPasses * 256; k += blockDim.x) s_hist[k] = 0;
     const SceneDesc S = scenes[0];
     const uint32_t nparts = (uint32_t)((n_in + kEmitTile - 1) / kEmitTile);
     const uint32_t lane = lane_id();
-    uint64_t live = 0;
-    for (;;) {
-        __syncthreads();
-        if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
-        __syncthreads();
-        const uint32_t part = s_part;
-        if (part >= nparts) break;
-        const uint64_t t0 = (uint64_t)part * kEmitTile + (uint64_t)threadIdx.x * kEmitIPT;
-        uint64_t segs[kEmitIPT];
-        uint32_t rows[kEmitIPT];
-        uint32_t cnt = 0;
-        for (uint32_t k = 0; k < kEmitIPT; ++k) {
-            uint64_t p = t0 + k;
-            if (p >= n_in) break;
-            uint64_t w = in[p];
-            if (w >> 63) continue;
-            ++live;
-            if (((uint32_t)(w >> 22) & 0xFFFFu) >= S.n_props) continue;
-            uint64_t rw;
-            uint32_t grow;
-            if (render_word(w, S, dims, &rw, &grow)) {
-                segs[cnt] = rw;
-                rows[cnt] = grow;
-                ++cnt;
-            }
-        }
-        uint32_t agg;
-        uint32_t ex = block_excl_scan_256(cnt, wsum, &agg);
-        if (threadIdx.x == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
-        if (threadIdx.x < 64) {
-            uint64_t e = part ? lookback_wave(status, part) : 0;
-            if (threadIdx.x == 0) {
-                s_excl = e;
-                if (part) st_release(status + part, kStInc | (e + agg));
-                if (part == nparts - 1) counters->kept = e + agg;
-            }
-        }
-        __syncthreads();
-        const uint64_t dst = s_excl + ex;
-        for (uint32_t q = 0; q < cnt; ++q) {
-            uint64_t w = segs[q];
-            out[dst + q] = w;
-            uint32_t shift = kPayloadBits;
-            for (uint32_t ps = 0; ps < dims.passes; ++ps, shift += dims.radix_bits) {
-                uint32_t dg = (uint32_t)(w >> shift) & (radix - 1);
-                if (lds_hist) atomicAdd(&s_hist[ps * radix + dg], 1u);
-                else atomicAdd(&hist[ps * radix + dg], 1u);
-            }
-        }
-        for (uint32_t q = 0; q < kEmitIPT; ++q) {
-            bool has = q < cnt;
-            uint32_t r = has ? rows[q] : 0;
-            uint64_t pending = __ballot(has);
-            while (pending) {
-                uint32_t leader = (uint32_t)__ffsll((long long)pending) - 1;
-                uint32_t lr = __shfl(r, leader);
-                uint64_t same = __ballot(has && r == lr);
-                if (lane == leader) atomicAdd(&row_count[lr], (uint32_t)__popcll(same));
-                if (has && r == lr) has = false;
-                pending &= ~same;
-            }
+    if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
+    __syncthreads();
+    const uint32_t part = s_part;
+    if (part >= nparts) return;
+    const uint64_t t0 = (uint64_t)part * kEmitTile + (uint64_t)threadIdx.x * kEmitIPT;
+    uint64_t segs[kEmitIPT];
+    uint32_t cnt = 0, live = 0;
+    for (uint32_t k = 0; k < kEmitIPT; ++k) {
+        const uint64_t p = t0 + k;
+        if (p >= n_in) break;
+        const uint64_t w = in[p];
+        if (w >> 63) continue;
+        ++live;
+        if (((uint32_t)(w >> 22) & 0xFFFFu) >= S.n_props) continue;
+        uint64_t rw;
+        uint32_t grow;
+        if (render_word(w, S, dims, &rw, &grow)) segs[cnt++] = rw;
+    }
+    uint32_t agg;
+    const uint32_t ex = block_excl_scan_256(cnt, wsum, &agg);
+    if (threadIdx.x == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
+    if (threadIdx.x < 64) {
+        const uint64_t e = part ? lookback_wave(status, part) : 0;
+        if (threadIdx.x == 0) {
+            s_excl = e;
+            if (part) st_release(status + part, kStInc | (e + agg));
+            if (part == nparts - 1) counters->kept = e + agg;
         }
     }
     __syncthreads();
-    if (lds_hist)
-        for (uint32_t k = threadIdx.x; k < dims.passes * radix; k += blockDim.x)
-            if (s_hist[k]) atomicAdd(&hist[k], s_hist[k]);
+    const uint64_t dst = s_excl + ex;
+    for (uint32_t q = 0; q < cnt; ++q) out[dst + q] = segs[q];
     for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
     if (lane == 0 && live) atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)live);
 }
@@ -604,13 +575,12 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
     const uint64_t n = counters->kept;
     const uint32_t nparts = (uint32_t)((n + kSortTile - 1) / kSortTile);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    for (;;) {
-        __syncthreads();
+    {
         if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
         for (uint32_t k = threadIdx.x; k < 4 * RADIX; k += 256) (&s_wcnt[0][0])[k] = 0;
         __syncthreads();
         const uint32_t part = s_part;
-        if (part >= nparts) break;
+        if (part >= nparts) return;
         const uint64_t base = (uint64_t)part * kSortTile;
         const uint32_t valid_n = (uint32_t)(n - base < kSortTile ? n - base : kSortTile);
         uint64_t keys[kSortKPT];
@@ -701,50 +671,82 @@ template __global__ void k_sort_pass<8>(const uint64_t *, uint64_t *, const Fram
                                         const uint32_t *, uint32_t *, uint32_t *);
 
 // ------------------------------------------------------------------ paint
+//
+// Two passes over the sorted render words of every tile row.
+//
+// k_carry (one wave per tile row) walks the row left to right and keeps only
+// the cover queue of LayerWorkbench (layer_workbench.rs:134-148, 177-198,
+// 449-485): per tile, the layers arriving with a non-empty carried cover and
+// their 8 row covers. It stores the queue entering every tile (a
+// "snapshot") together with the stream position of the tile's segments.
+//
+// k_paint (one wave per tile, lane = pixel x*8+y) starts from its tile's
+// snapshot: it rebuilds the workbench id list, runs the optimisation passes,
+// accumulates coverage and blends every layer, and writes RGBA8. Tiles are
+// independent, so the frame's tiles fill the whole GPU.
 
-// LDS layout of the paint kernel (one wave per work group), `cap` ids /
-// carries and `rcap` runs:
-//   carries (double-buffered): layer u32, cover u64 (8 x i8, byte y = row y)
-//   ids: layer u32, run i32, carry i32, props flags u32, mask u32
-//   runs: layer u32, begin u32, end u32, cover sums 8 x i32
-//   cells: area i32[64], cover i32[64] (pixel = x*8+y)
-struct PaintLds {
-    uint32_t *c_layer[2];
+// Carry-pass LDS: carries (double-buffered) key = layer | even_odd << 16,
+// cover u64 (byte y = row y); ids: key, run, carry; runs: layer, begin, end,
+// cover sums 8 x i32.
+struct CarryLds {
+    uint32_t *c_key[2];
     uint64_t *c_cov[2];
-    uint32_t *i_layer;
-    int32_t *i_run;
-    int32_t *i_carry;
-    uint32_t *i_flags;
-    uint32_t *i_mask;  // bit0 mask, bit1 skip_clipping
+    uint32_t *i_key;
+    int32_t *i_run, *i_carry;
     uint32_t *r_layer, *r_beg, *r_end;
     int32_t *r_cov;
+};
+
+size_t carry_lds_bytes_host(uint32_t cap, uint32_t rcap) {
+    return (size_t)cap * (2 * 4 + 2 * 8 + 3 * 4) + (size_t)rcap * (3 * 4 + 8 * 4);
+}
+
+__device__ __forceinline__ CarryLds carry_lds(unsigned char *p, uint32_t cap, uint32_t rcap) {
+    CarryLds L;
+    L.c_cov[0] = (uint64_t *)p; p += cap * 8;
+    L.c_cov[1] = (uint64_t *)p; p += cap * 8;
+    L.c_key[0] = (uint32_t *)p; p += cap * 4;
+    L.c_key[1] = (uint32_t *)p; p += cap * 4;
+    L.i_key = (uint32_t *)p; p += cap * 4;
+    L.i_run = (int32_t *)p; p += cap * 4;
+    L.i_carry = (int32_t *)p; p += cap * 4;
+    L.r_layer = (uint32_t *)p; p += rcap * 4;
+    L.r_beg = (uint32_t *)p; p += rcap * 4;
+    L.r_end = (uint32_t *)p; p += rcap * 4;
+    L.r_cov = (int32_t *)p;
+    return L;
+}
+
+// Paint-pass LDS: carry keys of the snapshot; ids: key, run | carry << 16,
+// props flags (| mask bits 30, 31), colour; runs: layer, begin, end; cells.
+struct PaintLds {
+    uint32_t *c_key;
+    uint32_t *i_key, *i_rc, *i_flags;
+    float4 *i_color;
+    uint32_t *r_layer, *r_beg, *r_end;
     int32_t *cell_area, *cell_cov;
 };
 
 size_t paint_lds_bytes_host(uint32_t cap, uint32_t rcap) {
-    return (size_t)cap * (2 * 4 + 2 * 8 + 5 * 4) + (size_t)rcap * (3 * 4 + 8 * 4) + 2 * 64 * 4;
+    return (size_t)cap * (4 + 3 * 4 + 16) + (size_t)rcap * (3 * 4) + 2 * 64 * 4;
 }
 
-__device__ __forceinline__ PaintLds paint_lds(unsigned char *lds, uint32_t cap, uint32_t rcap) {
+__device__ __forceinline__ PaintLds paint_lds(unsigned char *p, uint32_t cap, uint32_t rcap) {
     PaintLds L;
-    unsigned char *p = lds;
-    L.c_cov[0] = (uint64_t *)p; p += cap * 8;
-    L.c_cov[1] = (uint64_t *)p; p += cap * 8;
-    L.c_layer[0] = (uint32_t *)p; p += cap * 4;
-    L.c_layer[1] = (uint32_t *)p; p += cap * 4;
-    L.i_layer = (uint32_t *)p; p += cap * 4;
-    L.i_run = (int32_t *)p; p += cap * 4;
-    L.i_carry = (int32_t *)p; p += cap * 4;
+    L.i_color = (float4 *)p; p += cap * 16;
+    L.c_key = (uint32_t *)p; p += cap * 4;
+    L.i_key = (uint32_t *)p; p += cap * 4;
+    L.i_rc = (uint32_t *)p; p += cap * 4;
     L.i_flags = (uint32_t *)p; p += cap * 4;
-    L.i_mask = (uint32_t *)p; p += cap * 4;
     L.r_layer = (uint32_t *)p; p += rcap * 4;
     L.r_beg = (uint32_t *)p; p += rcap * 4;
     L.r_end = (uint32_t *)p; p += rcap * 4;
-    L.r_cov = (int32_t *)p; p += rcap * 32;
     L.cell_area = (int32_t *)p; p += 256;
     L.cell_cov = (int32_t *)p;
     return L;
 }
+
+constexpr uint32_t kMaskBit = 1u << 30, kSkipClipBit = 1u << 31, kNone16 = 0xFFFFu;
 
 __device__ __forceinline__ int32_t cov_byte(uint64_t c, uint32_t y) { return as_i8((int32_t)(c >> (8 * y))); }
 
@@ -752,76 +754,74 @@ __device__ __forceinline__ int32_t cov_byte(uint64_t c, uint32_t y) { return as_
 __device__ __forceinline__ bool cover_empty(uint64_t c, bool eo) {
     if (!eo) return c == 0;
     for (uint32_t y = 0; y < 8; ++y) {
-        int32_t v = cov_byte(c, y);
-        int32_t a = as_i8(v < 0 ? -v : v);
+        const int32_t v = cov_byte(c, y);
+        const int32_t a = as_i8(v < 0 ? -v : v);
         if ((a & 31) != 0) return false;
     }
     return true;
 }
 __device__ __forceinline__ bool cover_full(uint64_t c, bool eo) {
     for (uint32_t y = 0; y < 8; ++y) {
-        int32_t v = cov_byte(c, y);
-        int32_t a = as_i8(v < 0 ? -v : v);
+        const int32_t v = cov_byte(c, y);
+        const int32_t a = as_i8(v < 0 ? -v : v);
         if (eo ? ((a & 31) != 16) : (a != 16)) return false;
     }
     return true;
 }
-__device__ __forceinline__ uint64_t cover_add(uint64_t a, const int32_t *sums /* 8 */) {
-    uint64_t r = 0;
-    for (uint32_t y = 0; y < 8; ++y) {
-        uint32_t v = (uint32_t)(cov_byte(a, y) + sums[y]) & 0xFFu;
-        r |= (uint64_t)v << (8 * y);
-    }
-    return r;
-}
 
-// Gradient::color_at for one pixel row `row` of the column at (gx, gy)
+// Gradient::color_at for pixel row `row` of the column at (gx, gy)
 // (style.rs:96-175).
 __device__ void gradient_color(const PropsEntry &p, const float *stops, float gx, float gy, uint32_t row,
                                float out[4]) {
-    float dx = p.grad_end[0] - p.grad_start[0];
-    float dy = p.grad_end[1] - p.grad_start[1];
-    float dot = dx * dx + dy * dy;
-    float dot_recip = 1.0f / dot;
+    const float dx = p.grad_end[0] - p.grad_start[0];
+    const float dy = p.grad_end[1] - p.grad_start[1];
+    const float dot = dx * dx + dy * dy;
+    const float dot_recip = 1.0f / dot;
     float t;
     if (!(p.flags & PF_RADIAL)) {
-        float tx = (gx - p.grad_start[0]) * dx * dot_recip;
-        float ty = gy - p.grad_start[1];
+        const float tx = (gx - p.grad_start[0]) * dx * dot_recip;
+        const float ty = gy - p.grad_start[1];
         t = __builtin_fmaf(((float)row + ty) * dy, dot_recip, tx);
     } else {
-        float px = gx - p.grad_start[0];
-        float px2 = px * px;
-        float py = (float)row + (gy - p.grad_start[1]);
+        const float px = gx - p.grad_start[0];
+        const float px2 = px * px;
+        const float py = (float)row + (gy - p.grad_start[1]);
         t = __builtin_sqrtf(__builtin_fmaf(py, py, px2) * dot_recip);
     }
     const float *st = stops + p.stops_offset;
-    uint32_t ch[4] = {0, 0, 0, 0};
+    uint32_t ch0 = 0, ch1 = 0, ch2 = 0, ch3 = 0;
     bool acc = t <= st[4];
-    if (acc)
-        for (int c = 0; c < 4; ++c) ch[c] |= __float_as_uint(st[c]);
+    if (acc) {
+        ch0 |= __float_as_uint(st[0]); ch1 |= __float_as_uint(st[1]);
+        ch2 |= __float_as_uint(st[2]); ch3 |= __float_as_uint(st[3]);
+    }
     float start_stop = 0.0f;
-    const float *start_color = st;
+    const float *sc = st;
     for (uint32_t k = 1; k < p.n_stops; ++k) {
         const float *color = st + 5 * k;
-        float end_stop = color[4];
-        bool m = acc ^ (t < end_stop);
+        const float end_stop = color[4];
+        const bool m = acc ^ (t < end_stop);
         if (m) {
-            float d = end_stop - start_stop;
-            float lt = (t - start_stop) * (1.0f / d);
-            for (int c = 0; c < 4; ++c) {
-                float v = __builtin_fmaf(lt, color[c], __builtin_fmaf(-lt, start_color[c], start_color[c]));
-                ch[c] |= __float_as_uint(v);
-            }
+            const float d = end_stop - start_stop;
+            const float lt = (t - start_stop) * (1.0f / d);
+            ch0 |= __float_as_uint(__builtin_fmaf(lt, color[0], __builtin_fmaf(-lt, sc[0], sc[0])));
+            ch1 |= __float_as_uint(__builtin_fmaf(lt, color[1], __builtin_fmaf(-lt, sc[1], sc[1])));
+            ch2 |= __float_as_uint(__builtin_fmaf(lt, color[2], __builtin_fmaf(-lt, sc[2], sc[2])));
+            ch3 |= __float_as_uint(__builtin_fmaf(lt, color[3], __builtin_fmaf(-lt, sc[3], sc[3])));
             acc = true;
         }
         start_stop = end_stop;
-        start_color = color;
+        sc = color;
     }
     if (!acc) {
         const float *last = st + 5 * (p.n_stops - 1);
-        for (int c = 0; c < 4; ++c) ch[c] |= __float_as_uint(last[c]);
+        ch0 |= __float_as_uint(last[0]); ch1 |= __float_as_uint(last[1]);
+        ch2 |= __float_as_uint(last[2]); ch3 |= __float_as_uint(last[3]);
     }
-    for (int c = 0; c < 4; ++c) out[c] = __uint_as_float(ch[c]);
+    out[0] = __uint_as_float(ch0);
+    out[1] = __uint_as_float(ch1);
+    out[2] = __uint_as_float(ch2);
+    out[3] = __uint_as_float(ch3);
 }
 
 __device__ __forceinline__ uint32_t seg_colp(uint64_t w, uint32_t lb, uint32_t cmask) {
@@ -829,9 +829,11 @@ __device__ __forceinline__ uint32_t seg_colp(uint64_t w, uint32_t lb, uint32_t c
 }
 
 // Reads the runs (maximal same-layer groups) of tile column `colp` starting
-// at *pos; returns the number of runs or -1 on overflow.
-__device__ int32_t build_runs(const PaintParams &pp, const PaintLds &L, uint64_t *pos, uint64_t end,
-                              uint32_t colp, uint32_t rcap) {
+// at *pos; with SUMS also their per-row cover sums. Returns the run count or
+// -1 on overflow.
+template <bool SUMS>
+__device__ int32_t build_runs(const PaintParams &pp, uint32_t *r_layer, uint32_t *r_beg, uint32_t *r_end,
+                              int32_t *r_cov, uint64_t *pos, uint64_t end, uint32_t colp, uint32_t rcap) {
     const uint32_t lane = lane_id();
     const uint32_t lb = pp.layer_bits, lmask = (1u << lb) - 1u;
     const uint32_t cmask = (1u << pp.col_bits) - 1u;
@@ -839,97 +841,99 @@ __device__ int32_t build_runs(const PaintParams &pp, const PaintLds &L, uint64_t
     int32_t nruns = 0;
     uint32_t prev_layer = 0xFFFFFFFFu;
     for (;;) {
-        uint64_t idx = p + lane;
-        uint64_t w = idx < end ? pp.segs[idx] : ~0ull;
-        bool in = idx < end && seg_colp(w, lb, cmask) == colp;
-        uint64_t inmask = __ballot(in);
-        uint32_t n_in = (uint32_t)__popcll(inmask);  // contiguous prefix (sorted)
-        uint32_t layer = (uint32_t)(w >> kPayloadBits) & lmask;
+        const uint64_t idx = p + lane;
+        const uint64_t w = idx < end ? pp.segs[idx] : ~0ull;
+        const bool in = idx < end && seg_colp(w, lb, cmask) == colp;
+        const uint64_t inmask = __ballot(in);
+        const uint32_t n_in = (uint32_t)__popcll(inmask);  // a prefix: the stream is sorted
+        const uint32_t layer = (uint32_t)(w >> kPayloadBits) & lmask;
         uint32_t prevl = __shfl_up(layer, 1);
         if (lane == 0) prevl = prev_layer;
-        bool head = in && layer != prevl;
-        uint64_t heads = __ballot(head);
-        uint32_t nh = (uint32_t)__popcll(heads);
+        const bool head = in && layer != prevl;
+        const uint64_t heads = __ballot(head);
+        const uint32_t nh = (uint32_t)__popcll(heads);
         if ((uint32_t)nruns + nh > rcap) return -1;
-        uint32_t my_run = (uint32_t)nruns + (uint32_t)__popcll(heads & ((2ull << lane) - 1ull)) - 1u;
+        const uint32_t my_run = (uint32_t)nruns + (uint32_t)__popcll(heads & ((2ull << lane) - 1ull)) - 1u;
         if (head) {
-            L.r_layer[my_run] = layer;
-            L.r_beg[my_run] = (uint32_t)(idx - *pos);
-            if (my_run > 0) L.r_end[my_run - 1] = (uint32_t)(idx - *pos);
-            for (uint32_t y = 0; y < 8; ++y) L.r_cov[my_run * 8 + y] = 0;
+            r_layer[my_run] = layer;
+            r_beg[my_run] = (uint32_t)(idx - *pos);
+            if (my_run > 0) r_end[my_run - 1] = (uint32_t)(idx - *pos);
+            if (SUMS)
+                for (uint32_t y = 0; y < 8; ++y) r_cov[my_run * 8 + y] = 0;
         }
         __syncthreads();
-        if (in) {
-            int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
-            uint32_t ty = (uint32_t)(w >> 19) & 7u;
-            atomicAdd(&L.r_cov[my_run * 8 + ty], cover);
+        if (SUMS && in) {
+            const int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
+            const uint32_t ty = (uint32_t)(w >> 19) & 7u;
+            atomicAdd(&r_cov[my_run * 8 + ty], cover);
         }
         nruns += (int32_t)nh;
         prev_layer = __shfl(layer, n_in ? n_in - 1 : 0);
-        if (n_in == 0) prev_layer = 0xFFFFFFFFu;
         p += n_in;
         if (n_in < 64) break;
     }
-    if (nruns > 0 && lane == 0) L.r_end[nruns - 1] = (uint32_t)(p - *pos);
+    if (nruns > 0 && lane == 0) r_end[nruns - 1] = (uint32_t)(p - *pos);
     __syncthreads();
     *pos = p;
     return nruns;
 }
 
-// ids = sorted union of run layers and carry layers; returns the count.
-__device__ uint32_t merge_ids(const PaintLds &L, uint32_t cur, uint32_t nr, uint32_t nc) {
+// ids = sorted union of run layers and carry layers (populate_layers,
+// layer_workbench.rs:401-428). c_key holds layer | extra bits << 16.
+// Outputs key = layer, run (or -1), carry (or -1). Returns the count.
+__device__ uint32_t merge_ids(const uint32_t *c_key, const uint32_t *r_layer, uint32_t nr, uint32_t nc,
+                              uint32_t *i_key, int32_t *i_run, int32_t *i_carry) {
     const uint32_t lane = lane_id();
     const uint32_t n = nr + nc;
-    const uint32_t *cl = L.c_layer[cur];
     for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-        uint32_t k = k0 + lane;
+        const uint32_t k = k0 + lane;
         if (k < nr) {
-            uint32_t v = L.r_layer[k];
+            const uint32_t v = r_layer[k];
             uint32_t lo = 0, hi = nc;  // #carries < v
             while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (cl[mid] < v) lo = mid + 1; else hi = mid;
+                const uint32_t mid = (lo + hi) >> 1;
+                if ((c_key[mid] & 0xFFFFu) < v) lo = mid + 1; else hi = mid;
             }
-            uint32_t s = k + lo;
-            L.i_layer[s] = v;
-            L.i_run[s] = (int32_t)k;
-            L.i_carry[s] = -1;
+            const uint32_t s = k + lo;
+            i_key[s] = v;
+            i_run[s] = (int32_t)k;
+            i_carry[s] = -1;
         } else if (k < n) {
-            uint32_t m = k - nr;
-            uint32_t v = cl[m];
+            const uint32_t m = k - nr;
+            const uint32_t v = c_key[m] & 0xFFFFu;
             uint32_t lo = 0, hi = nr;  // #runs <= v
             while (lo < hi) {
-                uint32_t mid = (lo + hi) >> 1;
-                if (L.r_layer[mid] <= v) lo = mid + 1; else hi = mid;
+                const uint32_t mid = (lo + hi) >> 1;
+                if (r_layer[mid] <= v) lo = mid + 1; else hi = mid;
             }
-            uint32_t s = m + lo;
-            L.i_layer[s] = v;
-            L.i_run[s] = -1;
-            L.i_carry[s] = (int32_t)m;
+            const uint32_t s = m + lo;
+            i_key[s] = v;
+            i_run[s] = -1;
+            i_carry[s] = (int32_t)m;
         }
     }
     __syncthreads();
     // drop the carry entry that follows its run, moving its carry index over
     uint32_t out = 0;
     for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-        uint32_t k = k0 + lane;
+        const uint32_t k = k0 + lane;
         uint32_t lay = 0;
         int32_t run = -1, car = -1;
         bool keep = false;
         if (k < n) {
-            lay = L.i_layer[k];
-            run = L.i_run[k];
-            car = L.i_carry[k];
-            keep = !(k > 0 && L.i_layer[k - 1] == lay);
-            if (keep && k + 1 < n && L.i_layer[k + 1] == lay) car = L.i_carry[k + 1];
+            lay = i_key[k];
+            run = i_run[k];
+            car = i_carry[k];
+            keep = !(k > 0 && i_key[k - 1] == lay);
+            if (keep && k + 1 < n && i_key[k + 1] == lay) car = i_carry[k + 1];
         }
-        uint64_t km = __ballot(keep);
+        const uint64_t km = __ballot(keep);
         __syncthreads();
         if (keep) {
-            uint32_t d = out + (uint32_t)__popcll(km & lanemask_lt());
-            L.i_layer[d] = lay;
-            L.i_run[d] = run;
-            L.i_carry[d] = car;
+            const uint32_t d = out + (uint32_t)__popcll(km & lanemask_lt());
+            i_key[d] = lay;
+            i_run[d] = run;
+            i_carry[d] = car;
         }
         out += (uint32_t)__popcll(km);
         __syncthreads();
@@ -937,65 +941,122 @@ __device__ uint32_t merge_ids(const PaintLds &L, uint32_t cur, uint32_t nr, uint
     return out;
 }
 
-// Cover of id k after the tile: carry in + run sums (i8 wrapping).
-__device__ __forceinline__ uint64_t id_cover_out(const PaintLds &L, uint32_t cur, uint32_t k) {
-    int32_t sums[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    int32_t r = L.i_run[k];
-    if (r >= 0)
-        for (uint32_t y = 0; y < 8; ++y) sums[y] = L.r_cov[r * 8 + y];
-    int32_t c = L.i_carry[k];
-    uint64_t base = c >= 0 ? L.c_cov[cur][c] : 0ull;
-    return cover_add(base, sums);
-}
-
-// New carry list from the ids (filtered: Cover::is_empty drops, as
-// layer_workbench.rs:194,475 do; unfiltered while folding the left of the
-// row, painter/mod.rs:515-550). Returns the count.
-__device__ uint32_t next_carries(const PaintLds &L, uint32_t cur, uint32_t n, bool filter) {
+// The cover queue of one tile row, left to right; stores the queue entering
+// each tile of [h0, h1).
+__global__ __launch_bounds__(64) void k_carry(PaintParams pp) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t lane = lane_id();
-    const uint32_t nxt = cur ^ 1u;
-    uint32_t out = 0;
-    for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-        uint32_t k = k0 + lane;
-        bool keep = false;
-        uint64_t c = 0;
-        if (k < n) {
-            c = id_cover_out(L, cur, k);
-            keep = !filter || !cover_empty(c, L.i_flags[k] & PF_EVENODD);
+    const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs;
+    CarryLds L = carry_lds(lds_raw, cap, rcap);
+    const uint32_t grow = blockIdx.x;
+    const uint32_t sc = pp.row_scene[grow];
+    const SceneDesc S = pp.scenes[sc];
+    const uint32_t j = grow - S.row_base;
+    if (j < S.crop_v0 || j >= S.crop_v1) return;
+    const PropsEntry *props = pp.props + S.props_base;
+    uint64_t pos = pp.row_start[grow];
+    const uint64_t end = pp.row_start[grow + 1];
+    const uint32_t lb = pp.layer_bits, cmask = (1u << pp.col_bits) - 1u;
+    const uint32_t h0 = S.crop_h0, h1 = S.crop_h1 < S.row_len ? S.crop_h1 : S.row_len;
+    uint32_t cur = 0, nc = 0;
+    bool fold = true;  // tiles left of h0 (and tile -1): unfiltered (painter/mod.rs:515-550)
+    uint32_t c = 0;
+    for (;;) {
+        uint32_t colp;
+        if (fold) {
+            if (pos < end && seg_colp(pp.segs[pos], lb, cmask) <= h0) {
+                colp = seg_colp(pp.segs[pos], lb, cmask);
+            } else {
+                fold = false;
+                c = h0;
+                continue;
+            }
+        } else {
+            if (c >= h1) break;
+            // snapshot of the queue entering tile c
+            uint32_t base = 0;
+            if (lane == 0 && nc) base = atomicAdd(pp.snap_used, nc);
+            base = __shfl(base, 0);
+            const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + c;
+            if ((uint64_t)base + nc > pp.snap_cap) {
+                if (lane == 0) atomicOr(pp.overflow, 2u);
+                return;
+            }
+            for (uint32_t k = lane; k < nc; k += 64) {
+                pp.snap_key[base + k] = L.c_key[cur][k];
+                pp.snap_cov[base + k] = L.c_cov[cur][k];
+            }
+            if (lane == 0) pp.snap_hdr[tile] = make_uint4((uint32_t)pos, nc, base, 0u);
+            colp = c + 1;
         }
-        uint64_t km = __ballot(keep);
-        if (keep) {
-            uint32_t d = out + (uint32_t)__popcll(km & lanemask_lt());
-            L.c_layer[nxt][d] = L.i_layer[k];
-            L.c_cov[nxt][d] = c;
+        const int32_t nr = build_runs<true>(pp, L.r_layer, L.r_beg, L.r_end, L.r_cov, &pos, end, colp, rcap);
+        if (nr < 0 || (uint32_t)nr + nc > cap) {
+            if (lane == 0) atomicOr(pp.overflow, 1u);
+            return;
         }
-        out += (uint32_t)__popcll(km);
+        const uint32_t n = merge_ids(L.c_key[cur], L.r_layer, (uint32_t)nr, nc, L.i_key, L.i_run, L.i_carry);
+        // next queue: cover in + run sums (i8 wrapping), Cover::is_empty
+        // drops outside the fold (layer_workbench.rs:194,475)
+        const uint32_t nxt = cur ^ 1u;
+        uint32_t out = 0;
+        for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+            const uint32_t k = k0 + lane;
+            bool keep = false;
+            uint64_t cv = 0;
+            uint32_t key = 0;
+            if (k < n) {
+                const int32_t car = L.i_carry[k], run = L.i_run[k];
+                const uint32_t lay = L.i_key[k];
+                uint32_t eo;
+                uint64_t base_cov = 0;
+                if (car >= 0) {
+                    eo = L.c_key[cur][car] >> 16;
+                    base_cov = L.c_cov[cur][car];
+                } else {
+                    eo = (props[lay].flags & PF_EVENODD) ? 1u : 0u;
+                }
+                uint64_t r = 0;
+                for (uint32_t y = 0; y < 8; ++y) {
+                    const int32_t sum = cov_byte(base_cov, y) + (run >= 0 ? L.r_cov[run * 8 + y] : 0);
+                    r |= (uint64_t)((uint32_t)sum & 0xFFu) << (8 * y);
+                }
+                cv = r;
+                key = lay | (eo << 16);
+                keep = fold || !cover_empty(cv, eo != 0);
+            }
+            const uint64_t km = __ballot(keep);
+            if (keep) {
+                const uint32_t d = out + (uint32_t)__popcll(km & lanemask_lt());
+                L.c_key[nxt][d] = key;
+                L.c_cov[nxt][d] = cv;
+            }
+            out += (uint32_t)__popcll(km);
+        }
+        __syncthreads();
+        nc = out;
+        cur = nxt;
+        if (!fold) ++c;
     }
-    __syncthreads();
-    return out;
 }
 
 // A = 16 * P + area for this lane's pixel and id k (painter/mod.rs:229-243,
 // 347-362): cells from the id's run (segments at tbase + [beg, end)) and its
 // carried cover in column 0; P is the i8 running cover prefix.
-__device__ __forceinline__ int32_t pixel_area(const PaintParams &pp, const PaintLds &L, uint32_t cur,
-                                              uint64_t tbase, uint32_t k, uint32_t px, uint32_t py) {
+__device__ __forceinline__ int32_t pixel_area(const PaintParams &pp, const PaintLds &L, uint64_t tbase,
+                                              int32_t run, int32_t cy, uint32_t px, uint32_t py) {
     const uint32_t lane = lane_id();
-    const int32_t run = L.i_run[k];
-    const int32_t car = L.i_carry[k];
-    const int32_t cy = car >= 0 ? cov_byte(L.c_cov[cur][car], py) : 0;
     if (run < 0) return 16 * cy;
     L.cell_area[lane] = 0;
     L.cell_cov[lane] = 0;
     __syncthreads();
     const uint32_t beg = L.r_beg[run], en = L.r_end[run];
     for (uint32_t s0 = beg; s0 < en; s0 += 64) {
-        uint32_t s = s0 + lane;
+        const uint32_t s = s0 + lane;
         if (s < en) {
-            uint64_t w = pp.segs[tbase + s];
-            int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
-            int32_t area = (int32_t)((int64_t)(w << 48) >> 54);
-            uint32_t cell = ((uint32_t)(w >> 16) & 7u) * 8u + ((uint32_t)(w >> 19) & 7u);
+            const uint64_t w = pp.segs[tbase + s];
+            const int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
+            const int32_t area = (int32_t)((int64_t)(w << 48) >> 54);
+            const uint32_t cell = ((uint32_t)(w >> 16) & 7u) * 8u + ((uint32_t)(w >> 19) & 7u);
             atomicAdd(&L.cell_area[cell], area);
             atomicAdd(&L.cell_cov[cell], cover);
         }
@@ -1009,262 +1070,259 @@ __device__ __forceinline__ int32_t pixel_area(const PaintParams &pp, const Paint
     return A;
 }
 
-// One wave walks the tiles of one tile row.
+// One wave paints one tile (LayerWorkbench::drive_tile_painting,
+// layer_workbench.rs:430-486, and Painter, painter/mod.rs:260-493).
 __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t lane = lane_id();
     const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs;
     PaintLds L = paint_lds(lds_raw, cap, rcap);
-    const uint32_t grow = blockIdx.x;
+    // the carry pass stopped early: its snapshots are incomplete
+    if (__hip_atomic_load(pp.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const uint2 wk = pp.work[blockIdx.x];
+    const uint32_t grow = wk.x, c = wk.y;
     const uint32_t sc = pp.row_scene[grow];
     const SceneDesc S = pp.scenes[sc];
     const uint32_t j = grow - S.row_base;
-    if (j < S.crop_v0 || j >= S.crop_v1) return;
     const PropsEntry *props = pp.props + S.props_base;
-    uint64_t pos = pp.row_start[grow];
     const uint64_t end = pp.row_start[grow + 1];
-    const uint32_t lb = pp.layer_bits, cmask = (1u << pp.col_bits) - 1u;
     const uint32_t px = lane >> 3, py = lane & 7u;  // pixel of this lane (x*8+y)
-    const uint32_t h0 = S.crop_h0, h1 = S.crop_h1 < S.row_len ? S.crop_h1 : S.row_len;
     const bool clips = S.flags & SF_CLIPS;
-    uint32_t cur = 0, nc = 0;
+    const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + c;
+    const uint4 hdr = pp.snap_hdr[tile];
+    uint64_t pos = hdr.x;
+    const uint32_t nc = hdr.y;
+    if (nc > cap) {
+        if (lane == 0) atomicOr(pp.overflow, 1u);
+        return;
+    }
+    const uint64_t *c_cov = pp.snap_cov + hdr.z;
+    for (uint32_t k = lane; k < nc; k += 64) L.c_key[k] = pp.snap_key[hdr.z + k];
+    __syncthreads();
 
-    // ---- fold every tile left of the first painted one, unfiltered
-    //      (populate_covers, painter/mod.rs:515-550)
-    while (pos < end) {
-        const uint32_t c0 = seg_colp(pp.segs[pos], lb, cmask);
-        if (c0 > h0) break;
-        int32_t nr = build_runs(pp, L, &pos, end, c0, rcap);
-        if (nr < 0 || (uint32_t)nr + nc > cap) {
-            if (lane == 0) atomicOr(pp.overflow, 1u);
-            return;
+    // ---- runs and ids of the tile
+    const uint64_t tbase = pos;
+    const int32_t nr = build_runs<false>(pp, L.r_layer, L.r_beg, L.r_end, nullptr, &pos, end, c + 1, rcap);
+    if (nr < 0 || (uint32_t)nr + nc > cap) {
+        if (lane == 0) atomicOr(pp.overflow, 1u);
+        return;
+    }
+    int32_t *i_run = (int32_t *)L.i_flags;  // scratch for the merge, replaced below
+    int32_t *i_car = (int32_t *)L.i_rc;
+    const uint32_t n = merge_ids(L.c_key, L.r_layer, (uint32_t)nr, nc, L.i_key, i_run, i_car);
+    for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+        const uint32_t k = k0 + lane;
+        if (k < n) {
+            const int32_t run = i_run[k], car = i_car[k];
+            const PropsEntry &p = props[L.i_key[k]];
+            L.i_rc[k] = (run < 0 ? kNone16 : (uint32_t)run) | ((car < 0 ? kNone16 : (uint32_t)car) << 16);
+            L.i_flags[k] = (p.flags & ~(kMaskBit | kSkipClipBit)) | kMaskBit;
+            L.i_color[k] = make_float4(p.color[0], p.color[1], p.color[2], p.color[3]);
         }
-        const uint32_t n = merge_ids(L, cur, (uint32_t)nr, nc);
-        for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-            uint32_t k = k0 + lane;
-            if (k < n) L.i_flags[k] = props[L.i_layer[k]].flags;
+    }
+    __syncthreads();
+
+    // ---- coverage dump (debug): A of every id, independent of the passes
+    if (pp.cov_a) {
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t rc = L.i_rc[k];
+            const int32_t run = (rc & 0xFFFFu) == kNone16 ? -1 : (int32_t)(rc & 0xFFFFu);
+            const int32_t car = (rc >> 16) == kNone16 ? -1 : (int32_t)(rc >> 16);
+            const int32_t cy = car >= 0 ? cov_byte(c_cov[car], py) : 0;
+            const int32_t A = pixel_area(pp, L, tbase, run, cy, px, py);
+            uint32_t rec = 0;
+            if (lane == 0) rec = atomicAdd(pp.cov_count, 1u);
+            rec = __shfl(rec, 0);
+            if (rec < pp.cov_cap) {
+                pp.cov_a[(uint64_t)rec * 64 + lane] = A;
+                if (lane == 0) {
+                    pp.cov_key[rec * 4 + 0] = sc;
+                    pp.cov_key[rec * 4 + 1] = j;
+                    pp.cov_key[rec * 4 + 2] = c;
+                    pp.cov_key[rec * 4 + 3] = L.i_key[k];
+                }
+            }
         }
-        __syncthreads();
-        nc = next_carries(L, cur, n, false);
-        cur ^= 1u;
     }
 
-    for (uint32_t c = h0; c < h1; ++c) {
-        // ---- segments, runs and ids of tile c (populate_layers, layer_workbench.rs:401-428)
-        const uint64_t tbase = pos;
-        int32_t nr = build_runs(pp, L, &pos, end, c + 1, rcap);
-        if (nr < 0 || (uint32_t)nr + nc > cap) {
-            if (lane == 0) atomicOr(pp.overflow, 1u);
-            return;
-        }
-        const uint32_t n = merge_ids(L, cur, (uint32_t)nr, nc);
+    // ---- workbench passes (layer_workbench.rs:390-399)
+    int op = 2;  // 0 None, 1 Solid, 2 ColorBuffer
+    float solid0 = 0.0f, solid1 = 0.0f, solid2 = 0.0f, solid3 = 0.0f;
+    uint32_t skipped = 0;
+    if (S.flags & SF_CACHE) {
+        // tile_unchanged_pass (layer_workbench.rs:200-222)
+        bool changed = false;
         for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-            uint32_t k = k0 + lane;
-            if (k < n) {
-                L.i_flags[k] = props[L.i_layer[k]].flags;
-                L.i_mask[k] = 1u;
-            }
+            const uint32_t k = k0 + lane;
+            if (__ballot(k < n && !(L.i_flags[k] & PF_UNCHANGED))) changed = true;
         }
+        int32_t *prev = pp.prev_layers + tile;
+        const int32_t old = *prev;
+        const int32_t now = (int32_t)(uint16_t)n;
         __syncthreads();
-
-        // ---- coverage dump (debug): A of every id, independent of the passes
-        if (pp.cov_a) {
-            for (uint32_t k = 0; k < n; ++k) {
-                const int32_t A = pixel_area(pp, L, cur, tbase, k, px, py);
-                uint32_t rec = 0;
-                if (lane == 0) rec = atomicAdd(pp.cov_count, 1u);
-                rec = __shfl(rec, 0);
-                if (rec < pp.cov_cap) {
-                    pp.cov_a[(uint64_t)rec * 64 + lane] = A;
-                    if (lane == 0) {
-                        pp.cov_key[rec * 4 + 0] = sc;
-                        pp.cov_key[rec * 4 + 1] = j;
-                        pp.cov_key[rec * 4 + 2] = c;
-                        pp.cov_key[rec * 4 + 3] = L.i_layer[k];
-                    }
+        if (lane == 0) *prev = now;
+        if (old >= 0 && old == now && !changed) op = 0;
+    }
+    auto is_full = [&](uint32_t k, uint32_t f) {
+        const uint32_t rc = L.i_rc[k];
+        if ((rc & 0xFFFFu) != kNone16 || (rc >> 16) == kNone16) return false;
+        return cover_full(c_cov[rc >> 16], f & PF_EVENODD);
+    };
+    if (op != 0 && clips) {
+        // skip_trivial_clips_pass (layer_workbench.rs:224-308): serial, uniform
+        bool has = false, full = false, used = false;
+        uint32_t last = 0, ci = 0;
+        for (uint32_t k = 0; k < n; ++k) {
+            uint32_t f = L.i_flags[k];
+            const uint32_t lay = L.i_key[k];
+            if (!(f & kMaskBit)) continue;
+            if (f & PF_CLIP) {
+                const bool isfull = is_full(k, f);
+                has = true;
+                full = isfull;
+                last = (lay + props[lay].clip_layers) & 0xFFFFu;
+                ci = k;
+                used = false;
+                if (isfull) f &= ~kMaskBit;
+            } else if (f & PF_CLIPPED) {
+                if (has && lay <= last) {
+                    if (full) f |= kSkipClipBit; else used = true;
+                } else {
+                    f &= ~kMaskBit;
                 }
             }
-        }
-
-        // ---- workbench passes (layer_workbench.rs:390-399)
-        int op = 2;  // 0 None, 1 Solid, 2 ColorBuffer
-        float solid[4] = {0.0f, 0.0f, 0.0f, 0.0f};
-        uint32_t skipped = 0;
-        if (S.flags & SF_CACHE) {
-            // tile_unchanged_pass (layer_workbench.rs:200-222)
-            bool changed = false;
-            for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-                uint32_t k = k0 + lane;
-                if (__ballot(k < n && !(L.i_flags[k] & PF_UNCHANGED))) changed = true;
-            }
-            int32_t *prev = pp.prev_layers + S.cache_base + j * S.row_len + c;
-            const int32_t old = *prev;
-            const int32_t now = (int32_t)(uint16_t)n;
             __syncthreads();
-            if (lane == 0) *prev = now;
-            if (old >= 0 && old == now && !changed) op = 0;
+            if (lane == 0) L.i_flags[k] = f;
+            if (has && lay > last) {
+                has = false;
+                if (!used && lane == 0) L.i_flags[ci] &= ~kMaskBit;
+            }
+            __syncthreads();
         }
-        if (op != 0 && clips) {
-            // skip_trivial_clips_pass (layer_workbench.rs:224-308): serial, uniform
-            bool has = false, full = false, used = false;
-            uint32_t last = 0, ci = 0;
-            for (uint32_t k = 0; k < n; ++k) {
+        if (has && !used && lane == 0) L.i_flags[ci] &= ~kMaskBit;
+        __syncthreads();
+    }
+    if (op != 0) {
+        // skip_fully_covered_layers (layer_workbench.rs:310-388)
+        int first = 0;  // 0 unset, 1 opaque, 2 incomplete
+        int32_t k_opaque = -1;
+        for (int32_t top = (int32_t)n - 1; top >= 0; top -= 64) {
+            const int32_t k = top - (int32_t)lane;
+            bool pI = false, pO = false;
+            if (k >= 0) {
                 const uint32_t f = L.i_flags[k];
-                const uint32_t lay = L.i_layer[k];
-                uint32_t m = L.i_mask[k];
-                if (!(m & 1u)) continue;
-                if (f & PF_CLIP) {
-                    bool isfull = false;
-                    if (L.i_run[k] < 0 && L.i_carry[k] >= 0)
-                        isfull = cover_full(L.c_cov[cur][L.i_carry[k]], f & PF_EVENODD);
-                    has = true;
-                    full = isfull;
-                    last = (lay + props[lay].clip_layers) & 0xFFFFu;
-                    ci = k;
-                    used = false;
-                    if (isfull) m &= ~1u;
-                } else if (f & PF_CLIPPED) {
-                    if (has && lay <= last) {
-                        if (full) m |= 2u; else used = true;
-                    } else {
-                        m &= ~1u;
-                    }
+                if (f & kMaskBit) {
+                    const bool clipped = !(f & PF_CLIP) && (f & PF_CLIPPED) && !(f & kSkipClipBit);
+                    pI = clipped || !is_full((uint32_t)k, f);
+                    pO = !pI && (f & PF_OPAQUE);
                 }
-                __syncthreads();
-                if (lane == 0) L.i_mask[k] = m;
-                if (has && lay > last) {
-                    has = false;
-                    if (!used && lane == 0) L.i_mask[ci] &= ~1u;
-                }
-                __syncthreads();
             }
-            if (has && !used && lane == 0) L.i_mask[ci] &= ~1u;
-            __syncthreads();
+            const uint64_t bI = __ballot(pI), bO = __ballot(pO);
+            const uint64_t before = bO ? ((1ull << (__ffsll((long long)bO) - 1)) - 1ull) : ~0ull;
+            if (first == 0 && (bI & before)) first = 2;
+            if (bO) {
+                const uint32_t l = (uint32_t)__ffsll((long long)bO) - 1;
+                k_opaque = top - (int32_t)l;
+                if (first == 0) first = 1;
+                break;
+            }
         }
-        if (op != 0) {
-            // skip_fully_covered_layers (layer_workbench.rs:310-388)
-            int first = 0;  // 0 unset, 1 opaque, 2 incomplete
-            int32_t k_opaque = -1;
-            for (int32_t top = (int32_t)n - 1; top >= 0; top -= 64) {
-                const int32_t k = top - (int32_t)lane;
-                bool pI = false, pO = false;
-                if (k >= 0) {
+        if (k_opaque >= 0) skipped = (uint32_t)k_opaque;
+        if (first != 2) {
+            uint32_t from = 0;
+            float col[4];
+            if (first == 1) {
+                const float4 oc = L.i_color[k_opaque];
+                col[0] = oc.x; col[1] = oc.y; col[2] = oc.z; col[3] = oc.w;
+                from = (uint32_t)k_opaque + 1;
+            } else {
+                col[0] = S.clear[0]; col[1] = S.clear[1]; col[2] = S.clear[2]; col[3] = S.clear[3];
+            }
+            bool ok = true;
+            for (uint32_t k0 = from; k0 < n; k0 += 64) {
+                const uint32_t k = k0 + lane;
+                const bool bad = k < n && (L.i_flags[k] & kMaskBit) &&
+                                 ((L.i_flags[k] & PF_CLIP) || (L.i_flags[k] & PF_GRADIENT));
+                if (__ballot(bad)) ok = false;
+            }
+            if (ok) {
+                for (uint32_t k = from; k < n; ++k) {
                     const uint32_t f = L.i_flags[k];
-                    const uint32_t m = L.i_mask[k];
-                    if (m & 1u) {
-                        const bool clipped = !(f & PF_CLIP) && (f & PF_CLIPPED) && !(m & 2u);
-                        bool isfull = false;
-                        if (L.i_run[k] < 0 && L.i_carry[k] >= 0)
-                            isfull = cover_full(L.c_cov[cur][L.i_carry[k]], f & PF_EVENODD);
-                        pI = clipped || !isfull;
-                        pO = !pI && (f & PF_OPAQUE);
-                    }
+                    if (!(f & kMaskBit)) continue;
+                    const float4 sc4 = L.i_color[k];
+                    const float src[4] = {sc4.x, sc4.y, sc4.z, sc4.w};
+                    blend_color((f >> PF_BLEND_SHIFT) & 15u, col, src);
                 }
-                const uint64_t bI = __ballot(pI), bO = __ballot(pO);
-                const uint64_t before = bO ? ((1ull << (__ffsll((long long)bO) - 1)) - 1ull) : ~0ull;
-                if (first == 0 && (bI & before)) first = 2;
-                if (bO) {
-                    const uint32_t l = (uint32_t)__ffsll((long long)bO) - 1;
-                    k_opaque = top - (int32_t)l;
-                    if (first == 0) first = 1;
-                    break;
-                }
-            }
-            if (k_opaque >= 0) skipped = (uint32_t)k_opaque;
-            if (first != 2) {
-                uint32_t from = 0;
-                float col[4];
-                if (first == 1) {
-                    const PropsEntry &po = props[L.i_layer[k_opaque]];
-                    for (int q = 0; q < 4; ++q) col[q] = po.color[q];
-                    from = (uint32_t)k_opaque + 1;
-                } else {
-                    for (int q = 0; q < 4; ++q) col[q] = S.clear[q];
-                }
-                bool ok = true;
-                for (uint32_t k0 = from; k0 < n; k0 += 64) {
-                    const uint32_t k = k0 + lane;
-                    const bool bad = k < n && (L.i_mask[k] & 1u) &&
-                                     ((L.i_flags[k] & PF_CLIP) || (L.i_flags[k] & PF_GRADIENT));
-                    if (__ballot(bad)) ok = false;
-                }
-                if (ok) {
-                    for (uint32_t k = from; k < n; ++k) {
-                        if (!(L.i_mask[k] & 1u)) continue;
-                        const PropsEntry &pk = props[L.i_layer[k]];
-                        blend_color((pk.flags >> PF_BLEND_SHIFT) & 15u, col, pk.color);
-                    }
-                    op = 1;
-                    for (int q = 0; q < 4; ++q) solid[q] = col[q];
-                }
+                op = 1;
+                solid0 = col[0]; solid1 = col[1]; solid2 = col[2]; solid3 = col[3];
             }
         }
-
-        // ---- paint (TileWriteOp::ColorBuffer, layer_workbench.rs:449-485) and write
-        uint32_t pix = 0;
-        if (op == 2) {
-            float c0 = S.clear[0], c1 = S.clear[1], c2 = S.clear[2], ca = S.clear[3];
-            bool has_clip = false;
-            uint32_t clip_last = 0;
-            float clip_mask = 0.0f;
-            for (uint32_t k = skipped; k < n; ++k) {
-                const uint32_t m = L.i_mask[k];
-                if (!(m & 1u)) continue;
-                const uint32_t f = L.i_flags[k];
-                const uint32_t lay = L.i_layer[k];
-                if (has_clip && clip_last < lay) has_clip = false;
-                const int32_t A = pixel_area(pp, L, cur, tbase, k, px, py);
-                const float cov = coverage_from_area(A, f & PF_EVENODD);
-                if (f & PF_CLIP) {
-                    // clip_at (painter/mod.rs:402-416)
-                    if (!has_clip) {
-                        has_clip = true;
-                        clip_last = (lay + props[lay].clip_layers) & 0xFFFFu;
-                    }
-                    clip_mask = cov;
-                    continue;
-                }
-                const bool apply_clip = (f & PF_CLIPPED) && !(m & 2u);
-                const uint64_t nz = __ballot(cov != 0.0f);
-                const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
-                if (col_zero || (apply_clip && !has_clip)) continue;
-                const PropsEntry &pk = props[lay];
-                float fill[4];
-                if (f & PF_GRADIENT) {
-                    gradient_color(pk, pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, fill);
-                } else {
-                    fill[0] = pk.color[0]; fill[1] = pk.color[1]; fill[2] = pk.color[2]; fill[3] = pk.color[3];
-                }
-                // blend_at (painter/mod.rs:364-400)
-                float a = fill[3] * cov;
-                if (apply_clip) a *= clip_mask;
-                const float inv = 1.0f - a;
-                const uint32_t mode = (f >> PF_BLEND_SHIFT) & 15u;
-                const float b0 = blend_lane(mode, c0, fill[0]) * a;
-                const float b1 = blend_lane(mode, c1, fill[1]) * a;
-                const float b2 = blend_lane(mode, c2, fill[2]) * a;
-                c0 = __builtin_fmaf(c0, inv, b0);
-                c1 = __builtin_fmaf(c1, inv, b1);
-                c2 = __builtin_fmaf(c2, inv, b2);
-                ca = __builtin_fmaf(ca, inv, a);
-            }
-            pix = pixel_bytes(c0, c1, c2, ca);
-        } else if (op == 1) {
-            pix = solid_bytes(solid);
-        }
-        if (op != 0) {
-            const uint32_t gx = c * 8 + px, gy = j * 8 + py;
-            if (gx < S.width && gy < S.rows_px) {
-                uint32_t *o = (uint32_t *)(pp.out + S.out_offset) + (uint64_t)gy * S.stride + gx;
-                *o = pix;
-            }
-        }
-        if (pp.tile_ops && lane == 0) pp.tile_ops[S.cache_base + j * S.row_len + c] = (int8_t)op;
-
-        // ---- carries for the next tile
-        nc = next_carries(L, cur, n, true);
-        cur ^= 1u;
     }
+
+    // ---- paint (TileWriteOp::ColorBuffer, layer_workbench.rs:449-485) and write
+    uint32_t pix = 0;
+    if (op == 2) {
+        float c0 = S.clear[0], c1 = S.clear[1], c2 = S.clear[2], ca = S.clear[3];
+        bool has_clip = false;
+        uint32_t clip_last = 0;
+        float clip_mask = 0.0f;
+        for (uint32_t k = skipped; k < n; ++k) {
+            const uint32_t f = L.i_flags[k];
+            if (!(f & kMaskBit)) continue;
+            const uint32_t rc = L.i_rc[k];
+            const float4 fc = L.i_color[k];
+            const uint32_t lay = L.i_key[k];
+            if (has_clip && clip_last < lay) has_clip = false;
+            const int32_t run = (rc & 0xFFFFu) == kNone16 ? -1 : (int32_t)(rc & 0xFFFFu);
+            const int32_t car = (rc >> 16) == kNone16 ? -1 : (int32_t)(rc >> 16);
+            const int32_t cy = car >= 0 ? cov_byte(c_cov[car], py) : 0;
+            const int32_t A = pixel_area(pp, L, tbase, run, cy, px, py);
+            const float cov = coverage_from_area(A, f & PF_EVENODD);
+            if (f & PF_CLIP) {
+                // clip_at (painter/mod.rs:402-416)
+                if (!has_clip) {
+                    has_clip = true;
+                    clip_last = (lay + props[lay].clip_layers) & 0xFFFFu;
+                }
+                clip_mask = cov;
+                continue;
+            }
+            const bool apply_clip = (f & PF_CLIPPED) && !(f & kSkipClipBit);
+            const uint64_t nz = __ballot(cov != 0.0f);
+            const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
+            if (col_zero || (apply_clip && !has_clip)) continue;
+            float f0 = fc.x, f1 = fc.y, f2 = fc.z, f3 = fc.w;
+            if (f & PF_GRADIENT) {
+                float g[4];
+                gradient_color(props[lay], pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, g);
+                f0 = g[0]; f1 = g[1]; f2 = g[2]; f3 = g[3];
+            }
+            // blend_at (painter/mod.rs:364-400)
+            float a = f3 * cov;
+            if (apply_clip) a *= clip_mask;
+            const float inv = 1.0f - a;
+            const uint32_t mode = (f >> PF_BLEND_SHIFT) & 15u;
+            const float b0 = blend_lane(mode, c0, f0) * a;
+            const float b1 = blend_lane(mode, c1, f1) * a;
+            const float b2 = blend_lane(mode, c2, f2) * a;
+            c0 = __builtin_fmaf(c0, inv, b0);
+            c1 = __builtin_fmaf(c1, inv, b1);
+            c2 = __builtin_fmaf(c2, inv, b2);
+            ca = __builtin_fmaf(ca, inv, a);
+        }
+        pix = pixel_bytes(c0, c1, c2, ca);
+    } else if (op == 1) {
+        const float sc4[4] = {solid0, solid1, solid2, solid3};
+        pix = solid_bytes(sc4);
+    }
+    if (op != 0) {
+        const uint32_t gx = c * 8 + px, gy = j * 8 + py;
+        if (gx < S.width && gy < S.rows_px) {
+            uint32_t *o = (uint32_t *)(pp.out + S.out_offset) + (uint64_t)gy * S.stride + gx;
+            *o = pix;
+        }
+    }
+    if (pp.tile_ops && lane == 0) pp.tile_ops[tile] = (int8_t)op;
 }
 
 }  // namespace mg

@@ -97,10 +97,12 @@ private:
     DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
     DevBuf row_count_, row_start_, row_scene_, row_status_;
     DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
+    DevBuf work_, snap_hdr_, snap_key_, snap_cov_;
+    size_t snap_cap_ = 0;
     std::vector<DevBuf> caches_;
     std::vector<size_t> cache_tiles_;
     const uint64_t *last_sorted_ = nullptr;
-    uint32_t cap_ids_ = 256, cap_runs_ = 256;
+    uint32_t cap_ids_ = 128, cap_runs_ = 128;  // painter LDS capacities of the last frame
 };
 
 // Number of usable gfx950 devices.

@@ -62,7 +62,8 @@ Renderer::~Renderer() {
                      &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &image_, &overflow_,
-                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_};
+                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &snap_hdr_, &snap_key_,
+                     &snap_cov_};
     for (DevBuf *b : all) b->release();
     for (DevBuf &b : caches_) b.release();
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -92,7 +93,7 @@ bool Renderer::init(std::string &err) {
     MG_HIP(hipEventCreate(&ev0_), "hipEventCreate");
     MG_HIP(hipEventCreate(&ev1_), "hipEventCreate");
     for (hipEvent_t &e : evs_) MG_HIP(hipEventCreate(&e), "hipEventCreate");
-    if (!counters_.reserve(sizeof(FrameCounters)) || !err_.reserve(16) || !overflow_.reserve(16) ||
+    if (!counters_.reserve(sizeof(FrameCounters)) || !err_.reserve(16) || !overflow_.reserve(64) ||
         !part_counters_.reserve(64 * sizeof(uint32_t)) || !cov_count_.reserve(16)) {
         err = "out of device memory";
         return false;
@@ -230,7 +231,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     dims.radix_bits = R;
     dims.passes = (dims.key_bits + R - 1) / R;
     dims.mode = spec.mode;
-    if (kPayloadBits + dims.key_bits > 64 || dims.passes > 8) {
+    if (kPayloadBits + dims.key_bits > 64 || dims.passes > 16) {
         err = "frame too large for the 64-bit render key";
         return false;
     }
@@ -310,8 +311,8 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     const uint64_t nparts_emit = ((ingest ? spec.n_in : P) + kEmitTileHost - 1) / kEmitTileHost;
     const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
     if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) ||
-        !emit_status_.reserve(std::max<uint64_t>(nparts_emit, 1) * 8) || !hist_.reserve(8 * radix * 4) ||
-        !bases_.reserve(8 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4) ||
+        !emit_status_.reserve(std::max<uint64_t>(nparts_emit, 1) * 8) || !hist_.reserve(16 * radix * 4) ||
+        !bases_.reserve(16 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4) ||
         !row_count_.reserve(((size_t)global_rows + 1) * 4) || !row_start_.reserve(((size_t)global_rows + 1) * 4) ||
         !row_status_.reserve(((global_rows + 1 + kScanTileHost - 1) / kScanTileHost) * 8)) {
         err = "out of device memory (segments)";
@@ -321,30 +322,22 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     fill32(row_count_.as<uint32_t>(), 0, global_rows + 1, stream_);
     if (ingest && spec.n_in) {
         fill64(emit_status_.as<uint64_t>(), 0, nparts_emit, stream_);
-        uint32_t grid = (uint32_t)std::min<uint64_t>(nparts_emit, (uint64_t)cus_ * 8);
-        k_ingest<<<grid, 256, 0, stream_>>>(spec.in_segs, spec.n_in, scenes_.as<SceneDesc>(), dims, ctr,
-                                            segs_a_.as<uint64_t>(), emit_status_.as<uint64_t>(), pc + 1,
-                                            hist_.as<uint32_t>(), row_count_.as<uint32_t>());
+        k_ingest<<<(uint32_t)nparts_emit, 256, 0, stream_>>>(spec.in_segs, spec.n_in, scenes_.as<SceneDesc>(), dims,
+                                                           ctr, segs_a_.as<uint64_t>(), emit_status_.as<uint64_t>(),
+                                                           pc + 1);
     } else if (P) {
         fill64(emit_status_.as<uint64_t>(), 0, nparts_emit, stream_);
-        uint32_t grid = (uint32_t)std::min<uint64_t>(nparts_emit, (uint64_t)cus_ * 8);
-        k_emit<<<grid, 256, 0, stream_>>>(setup_.as<LineSetup>(), offsets_.as<uint32_t>(), scenes_.as<SceneDesc>(),
-                                          dims, ctr, segs_a_.as<uint64_t>(), cap_segs,
-                                          emit_status_.as<uint64_t>(), pc + 1, hist_.as<uint32_t>(),
-                                          row_count_.as<uint32_t>());
+        k_emit<<<(uint32_t)nparts_emit, 256, 0, stream_>>>(setup_.as<LineSetup>(), offsets_.as<uint32_t>(),
+                                                         scenes_.as<SceneDesc>(), dims, ctr, segs_a_.as<uint64_t>(),
+                                                         cap_segs, emit_status_.as<uint64_t>(), pc + 1);
     }
+    k_histogram<<<(uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((cap_segs + 2047) / 2048, (uint64_t)cus_ * 4)),
+                  256, 0, stream_>>>(segs_a_.as<uint64_t>(), ctr, dims, hist_.as<uint32_t>());
     k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
-    if (spec.mode != 1) {
-        const uint64_t nr = (uint64_t)global_rows + 1;
-        fill64(row_status_.as<uint64_t>(), 0, (nr + kScanTileHost - 1) / kScanTileHost, stream_);
-        k_scan_u32<<<(uint32_t)((nr + kScanTileHost - 1) / kScanTileHost), 256, 0, stream_>>>(
-            row_count_.as<uint32_t>(), row_start_.as<uint32_t>(), nr, row_status_.as<uint64_t>(), pc + 2,
-            &ctr->rows_total);
-    }
     MG_HIP(hipGetLastError(), "emit launch");
     MG_HIP(hipEventRecord(evs_[2], stream_), "event");
     uint64_t *in = segs_a_.as<uint64_t>(), *out = segs_b_.as<uint64_t>();
-    const uint32_t sort_grid = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(nparts_sort, 1), (uint64_t)cus_ * 4);
+    const uint32_t sort_grid = (uint32_t)std::max<uint64_t>(nparts_sort, 1);
     for (uint32_t p = 0; p < dims.passes; ++p) {
         fill32(sort_status_.as<uint32_t>(), 0, nparts_sort * radix, stream_);
         const uint32_t shift = kPayloadBits + p * R;
@@ -364,6 +357,11 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     MG_HIP(hipEventRecord(evs_[3], stream_), "event");
     last_sorted_ = in;
     res.sorted = in;
+    if (spec.mode != 1) {
+        const uint32_t row_shift = kPayloadBits + dims.col_bits + dims.layer_bits;
+        k_row_starts<<<(global_rows + 1 + 255) / 256, 256, 0, stream_>>>(in, ctr, global_rows, row_shift,
+                                                                         row_start_.as<uint32_t>());
+    }
     if (spec.mode != 1) {
         if (spec.want_tile_ops) {
             if (!tile_ops_.reserve(std::max<uint64_t>(tiles, 1))) { err = "out of device memory"; return false; }
@@ -400,6 +398,25 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
                      const uint64_t *sorted, uint8_t *image, int32_t *prev_layers, size_t tiles,
                      std::string &err) {
     if (global_rows == 0) return true;
+    // one paint work group per tile inside the crop
+    std::vector<uint32_t> work;
+    for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
+        const SceneDesc &s = spec.scenes[sc];
+        const uint32_t h1 = std::min(s.crop_h1, s.row_len);
+        for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j)
+            for (uint32_t c = s.crop_h0; c < h1; ++c) {
+                work.push_back(s.row_base + j);
+                work.push_back(c);
+            }
+    }
+    const uint32_t n_work = (uint32_t)(work.size() / 2);
+    if (n_work == 0) return true;
+    if (!upload_vec(work_, work, stream_, err)) return false;
+    if (!snap_hdr_.reserve(std::max<size_t>(tiles, 1) * 16)) {
+        err = "out of device memory (carry snapshots)";
+        return false;
+    }
+
     PaintParams pp{};
     pp.segs = sorted;
     pp.row_start = row_start_.as<uint32_t>();
@@ -413,48 +430,68 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.layer_bits = layer_bits;
     pp.overflow = overflow_.as<uint32_t>();
     pp.tile_ops = spec.want_tile_ops ? tile_ops_.as<int8_t>() : nullptr;
+    pp.snap_hdr = snap_hdr_.as<uint4>();
+    pp.snap_used = overflow_.as<uint32_t>() + 1;
+    pp.work = work_.as<uint2>();
     if (spec.want_coverage) {
         pp.cov_a = cov_a_.as<int32_t>();
         pp.cov_key = cov_key_.as<uint32_t>();
         pp.cov_count = cov_count_.as<uint32_t>();
         pp.cov_cap = spec.coverage_cap;
     }
-    // BufferLayerCache state is updated by the painter: keep a copy so an
-    // overflow retry starts from the same state.
+    // BufferLayerCache state is updated by the painter: keep a copy so a
+    // retry starts from the same state.
     DevBuf saved;
     if (prev_layers) {
         if (!saved.reserve(std::max<size_t>(tiles, 1) * 4)) { err = "out of device memory"; return false; }
         MG_HIP(hipMemcpyAsync(saved.ptr, prev_layers, tiles * 4, hipMemcpyDeviceToDevice, stream_), "save cache");
     }
-    static const uint32_t caps[][2] = {{256, 256}, {1024, 384}, {2048, 512}, {3072, 640}};
+    static const uint32_t caps[][2] = {{128, 128}, {512, 256}, {1024, 512}, {2048, 768}, {3072, 1024},
+                                       {4096, 1024}};
     size_t lds_max = 65536;
     {
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, device_) == hipSuccess) lds_max = p.sharedMemPerBlock;
     }
-    for (int attempt = 0; attempt < 4; ++attempt) {
-        uint32_t cap = std::max(caps[attempt][0], cap_ids_), rcap = std::max(caps[attempt][1], cap_runs_);
-        size_t lds = paint_lds_bytes_host(cap, rcap);
-        if (lds > lds_max) break;
+    int level = 0;
+    while (level < 5 && caps[level][0] < cap_ids_) ++level;
+    if (snap_cap_ < tiles * 16) snap_cap_ = tiles * 16;
+    for (int attempt = 0; attempt < 12; ++attempt) {
+        const uint32_t cap = caps[level][0], rcap = caps[level][1];
+        const size_t lds_c = carry_lds_bytes_host(cap, rcap), lds_p = paint_lds_bytes_host(cap, rcap);
+        if (std::max(lds_c, lds_p) > lds_max) break;
+        if (!snap_key_.reserve(snap_cap_ * 4) || !snap_cov_.reserve(snap_cap_ * 8)) {
+            err = "out of device memory (carry snapshots)";
+            return false;
+        }
+        pp.snap_key = snap_key_.as<uint32_t>();
+        pp.snap_cov = snap_cov_.as<uint64_t>();
+        pp.snap_cap = snap_cap_;
         if (attempt > 0 && prev_layers)
             MG_HIP(hipMemcpyAsync(prev_layers, saved.ptr, tiles * 4, hipMemcpyDeviceToDevice, stream_), "restore cache");
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
-        fill32(overflow_.as<uint32_t>(), 0, 1, stream_);
+        fill32(overflow_.as<uint32_t>(), 0, 2, stream_);
         pp.cap_ids = cap;
         pp.cap_runs = rcap;
-        MG_HIP(hipFuncSetAttribute((const void *)k_paint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds),
+        MG_HIP(hipFuncSetAttribute((const void *)k_carry, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c),
+               "carry lds");
+        MG_HIP(hipFuncSetAttribute((const void *)k_paint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p),
                "paint lds");
-        k_paint<<<global_rows, 64, lds, stream_>>>(pp);
+        k_carry<<<global_rows, 64, lds_c, stream_>>>(pp);
+        k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
         MG_HIP(hipGetLastError(), "paint launch");
-        uint32_t ov = 0;
-        MG_HIP(hipMemcpyAsync(&ov, overflow_.ptr, 4, hipMemcpyDeviceToHost, stream_), "read overflow");
+        uint32_t ov[2] = {0, 0};
+        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 8, hipMemcpyDeviceToHost, stream_), "read overflow");
         MG_HIP(hipStreamSynchronize(stream_), "paint");
-        if (!ov) {
+        if (!ov[0]) {
             cap_ids_ = cap;
             cap_runs_ = rcap;
             saved.release();
             return true;
         }
+        if (ov[0] & 2u) snap_cap_ *= 2;
+        if ((ov[0] & 1u) && level < 5) ++level;
+        else if (ov[0] & 1u) break;
     }
     saved.release();
     err = "a tile holds more layers than the painter's LDS capacity";

@@ -206,6 +206,11 @@ MG_EXPORT mg_status mg_render_device(mg_composition *comp, uint32_t width, uint3
    NONE tail included). The pointer stays valid until the next call on `comp`. */
 MG_EXPORT mg_status mg_pixel_segments(mg_composition *comp, const uint64_t **out, size_t *n);
 
+/* Result-invariant tuning knob of the renderer (0 = automatic): the sort's
+   digit width in bits (4, 6 or 8). Every setting produces the same bytes;
+   only the speed changes. */
+MG_EXPORT mg_status mg_set_tuning(mg_composition *comp, uint32_t radix_bits);
+
 /* Statistics of the last render: lines, line-pixels, non-NONE segments,
    segments kept for painting, tiles, digit passes, radix bits. */
 typedef struct mg_frame_stats {

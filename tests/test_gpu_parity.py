@@ -93,6 +93,37 @@ def test_pixel_segments_canonical_stream(gpu, cpu):
         assert np.array_equal(s_cpu, s_gpu)
 
 
+@pytest.mark.parametrize("radix", [4, 6, 8])
+def test_tuning_is_result_invariant(gpu, cpu, radix):
+    w, h = 150, 70
+    comp = scenegen.build_scene(cpu, 600, w, h, 60, transforms=True, clips=True)
+    ref = np.zeros((w * h, 4), np.uint8)
+    comp.render(cpu.Buffer(ref, w), [1.0, 1.0, 1.0, 1.0], None)
+    g = scenegen.build_scene(gpu, 600, w, h, 60, transforms=True, clips=True)
+    g.set_tuning(radix)
+    out = np.zeros((w * h, 4), np.uint8)
+    g.render(gpu.Buffer(out, w), [1.0, 1.0, 1.0, 1.0], None)
+    assert_same_image(ref, out, w)
+    s_cpu = scenegen.build_scene(cpu, 601, 90, 50, 20).pixel_segments()
+    g2 = scenegen.build_scene(gpu, 601, 90, 50, 20)
+    g2.set_tuning(radix)
+    assert np.array_equal(s_cpu, g2.pixel_segments())
+
+
+def test_full_size_circles_frame(gpu, cpu):
+    """The CPU-runnable configuration at full size: 1024x1024, 10,000 circles."""
+    from cassia_amd import workloads
+    scene = workloads.circles()
+    bufs = []
+    for m in (cpu, gpu):
+        comp = scene.build(m)
+        b = np.zeros((1024 * 1024, 4), np.uint8)
+        comp.render(m.Buffer(b, 1024), scene.background, None)
+        bufs.append(b)
+    assert_same_image(bufs[0], bufs[1], 1024)
+    assert len(np.unique(bufs[1].view(np.uint32))) > 5000
+
+
 def test_empty_composition(gpu, cpu):
     for m in (cpu, gpu):
         buf = np.zeros((16 * 9, 4), np.uint8)
