@@ -18,21 +18,30 @@ struct PaintParams {
     int32_t *prev_layers;        // BufferLayerCache::layers_per_tile (-1 = None)
     uint8_t *out;                // RGBA8 frame(s)
     uint32_t col_bits, layer_bits;
-    uint32_t cap_ids, cap_runs;  // LDS capacities
-    uint32_t *overflow;
+    uint32_t cap_ids, cap_runs;  // row-walk LDS capacities: queue + runs, runs of one tile
+    uint32_t *overflow;          // bit 0: LDS capacity, bit 1: record capacity
     // debug: coverage records (A values) and per-tile write ops
     int32_t *cov_a;
     uint32_t *cov_key;
     uint32_t *cov_count;
     uint32_t cov_cap;
     int8_t *tile_ops;
-    // carry snapshots: per tile (stream position, queue length, offset), and
-    // the queue entries (layer | even_odd << 16, 8 row covers)
-    uint4 *snap_hdr;
-    uint32_t *snap_key;
-    uint64_t *snap_cov;
-    uint32_t *snap_used;
-    uint64_t snap_cap;
+    // tile columns: first word per (row, colp) (stride tstride), run counts
+    uint32_t *tile_start;
+    uint32_t *tile_nr;
+    uint32_t tstride;
+    // runs, indexed by their column's first word + run index:
+    // layer | even_odd << 16, first word, 8 row cover sums (i8 each)
+    uint32_t *run_key;
+    uint32_t *run_beg;
+    uint64_t *run_cov;
+    // per-tile id records (layer | has_run << 16 | has_carry << 17, run first
+    // word, carried cover lo, hi) and run lengths; per-tile (offset, count)
+    uint4 *rec;
+    uint32_t *rec_len;
+    uint2 *rec_hdr;
+    uint32_t *rec_used;
+    uint64_t rec_cap;
     const uint2 *work;           // (global row, tile column) per paint work group
 };
 
@@ -57,11 +66,12 @@ template <int R>
 __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
                             uint32_t shift, const uint32_t *digit_base, uint32_t *status,
                             uint32_t *part_counter);
-__global__ void k_carry(PaintParams pp);
+__global__ void k_tile_starts(PaintParams pp, uint32_t global_rows);
+__global__ void k_tile_runs(PaintParams pp);
+__global__ void k_row_walk(PaintParams pp);
 __global__ void k_paint(PaintParams pp);
-size_t carry_lds_bytes_host(uint32_t cap, uint32_t rcap);
-
-size_t paint_lds_bytes_host(uint32_t cap, uint32_t rcap);
+size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride);
+size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 
 constexpr uint32_t kEmitTileHost = 1024;
 constexpr uint32_t kSortTileHost = 4096;

@@ -8,9 +8,11 @@
 //                  histograms and per-tile-row counts fused in
 //   k_digit_bases  histogram -> digit base offsets
 //   k_sort_pass    one stable LSD digit pass (onesweep: block rank + decoupled look-back)
-//   k_paint        one wave per tile row walks its tiles:  (painter/mod.rs:495-596,
-//                  carries, workbench decisions, coverage,  layer_workbench.rs:134-486)
-//                  blending and RGBA8 stores
+//   k_row_starts   first word of every tile row (binary search)
+//   k_tile_starts, k_tile_runs, k_row_walk, k_paint       (painter/mod.rs:229-596,
+//                  tile ranges, runs with cover sums, the   layer_workbench.rs:134-486)
+//                  row-serial cover carry producing per-tile id records, and
+//                  the tile-parallel workbench + painter with RGBA8 stores
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -447,12 +449,23 @@ __global__ __launch_bounds__(256) void k_histogram(const uint64_t *__restrict__ 
         }
 #pragma unroll
         for (uint32_t k = 0; k < IPT; ++k) {
-            if (w[k] == ~0ull) continue;
+            const bool valid = w[k] != ~0ull;
+            const uint64_t vmask = __ballot(valid);
             uint32_t shift = kPayloadBits;
             for (uint32_t ps = 0; ps < dims.passes; ++ps, shift += dims.radix_bits) {
                 const uint32_t dg = (uint32_t)(w[k] >> shift) & (radix - 1);
-                if (lds_hist) atomicAdd(&s_hist[ps * radix + dg], 1u);
-                else atomicAdd(&hist[ps * radix + dg], 1u);
+                // high digits are mostly equal across a wave: one add for all
+                const uint32_t d0 = __shfl(dg, (int)(__ffsll((long long)vmask) - 1));
+                const bool uniform = __ballot(valid && dg == d0) == vmask;
+                if (uniform) {
+                    if (lane_id() == 0 && vmask) {
+                        if (lds_hist) atomicAdd(&s_hist[ps * radix + d0], (uint32_t)__popcll(vmask));
+                        else atomicAdd(&hist[ps * radix + d0], (uint32_t)__popcll(vmask));
+                    }
+                } else if (valid) {
+                    if (lds_hist) atomicAdd(&s_hist[ps * radix + dg], 1u);
+                    else atomicAdd(&hist[ps * radix + dg], 1u);
+                }
             }
         }
     }
@@ -672,83 +685,56 @@ template __global__ void k_sort_pass<8>(const uint64_t *, uint64_t *, const Fram
 
 // ------------------------------------------------------------------ paint
 //
-// Two passes over the sorted render words of every tile row.
+// Four kernels over the sorted render words. Tile rows are independent, and a
+// tile's words are contiguous and sorted by layer.
 //
-// k_carry (one wave per tile row) walks the row left to right and keeps only
-// the cover queue of LayerWorkbench (layer_workbench.rs:134-148, 177-198,
-// 449-485): per tile, the layers arriving with a non-empty carried cover and
-// their 8 row covers. It stores the queue entering every tile (a
-// "snapshot") together with the stream position of the tile's segments.
-//
-// k_paint (one wave per tile, lane = pixel x*8+y) starts from its tile's
-// snapshot: it rebuilds the workbench id list, runs the optimisation passes,
-// accumulates coverage and blends every layer, and writes RGBA8. Tiles are
-// independent, so the frame's tiles fill the whole GPU.
+// k_tile_starts  per (tile row, tile column): index of the column's first
+//                word (binary search in the row).
+// k_tile_runs    per tile column, one wave: the runs (maximal same-layer
+//                groups) with their 8 per-row cover sums, from a segmented
+//                byte-wise scan in registers. Run r of the column is stored at
+//                index (column's first word + r), so no allocation is needed.
+// k_row_walk     per tile row, one 256-thread work group: the cover queue of
+//                LayerWorkbench (layer_workbench.rs:134-198, 449-485) carried
+//                left to right. Per tile it merges the queue with the tile's
+//                runs (merge path) and stores the tile's id list
+//                (populate_layers, layer_workbench.rs:401-428) as records
+//                (layer, run, carried cover) in layer order.
+// k_paint        per tile, one wave (lane = pixel x*8+y): the workbench passes
+//                and the painter (painter/mod.rs:229-493) over the tile's
+//                records in batches of 64, with the batch's segments staged
+//                in LDS; no barrier per layer.
 
-// Carry-pass LDS: carries (double-buffered) key = layer | even_odd << 16,
-// cover u64 (byte y = row y); ids: key, run, carry; runs: layer, begin, end,
-// cover sums 8 x i32.
-struct CarryLds {
-    uint32_t *c_key[2];
-    uint64_t *c_cov[2];
-    uint32_t *i_key;
-    int32_t *i_run, *i_carry;
-    uint32_t *r_layer, *r_beg, *r_end;
-    int32_t *r_cov;
-};
+constexpr uint32_t kMaskBit = 1u << 30, kSkipClipBit = 1u << 31;
+constexpr uint32_t kRunEvenOdd = 1u << 16;                    // run key: layer | even_odd << 16
+constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flags
+constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
+constexpr uint32_t kRecChunk = 16384;                         // records reserved per row at a time
 
-size_t carry_lds_bytes_host(uint32_t cap, uint32_t rcap) {
-    return (size_t)cap * (2 * 4 + 2 * 8 + 3 * 4) + (size_t)rcap * (3 * 4 + 8 * 4);
+size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride) {
+    return (size_t)cap * 24 + (size_t)rcap * 16 + (size_t)tstride * 8 + 64;
 }
-
-__device__ __forceinline__ CarryLds carry_lds(unsigned char *p, uint32_t cap, uint32_t rcap) {
-    CarryLds L;
-    L.c_cov[0] = (uint64_t *)p; p += cap * 8;
-    L.c_cov[1] = (uint64_t *)p; p += cap * 8;
-    L.c_key[0] = (uint32_t *)p; p += cap * 4;
-    L.c_key[1] = (uint32_t *)p; p += cap * 4;
-    L.i_key = (uint32_t *)p; p += cap * 4;
-    L.i_run = (int32_t *)p; p += cap * 4;
-    L.i_carry = (int32_t *)p; p += cap * 4;
-    L.r_layer = (uint32_t *)p; p += rcap * 4;
-    L.r_beg = (uint32_t *)p; p += rcap * 4;
-    L.r_end = (uint32_t *)p; p += rcap * 4;
-    L.r_cov = (int32_t *)p;
-    return L;
+size_t paint_lds_bytes_host(uint32_t cap, bool clips) {
+    return (size_t)kStageWin * 8 + (clips ? (size_t)cap * 4 : 0);
 }
-
-// Paint-pass LDS: carry keys of the snapshot; ids: key, run | carry << 16,
-// props flags (| mask bits 30, 31), colour; runs: layer, begin, end; cells.
-struct PaintLds {
-    uint32_t *c_key;
-    uint32_t *i_key, *i_rc, *i_flags;
-    float4 *i_color;
-    uint32_t *r_layer, *r_beg, *r_end;
-    int32_t *cell_area, *cell_cov;
-};
-
-size_t paint_lds_bytes_host(uint32_t cap, uint32_t rcap) {
-    return (size_t)cap * (4 + 3 * 4 + 16) + (size_t)rcap * (3 * 4) + 2 * 64 * 4;
-}
-
-__device__ __forceinline__ PaintLds paint_lds(unsigned char *p, uint32_t cap, uint32_t rcap) {
-    PaintLds L;
-    L.i_color = (float4 *)p; p += cap * 16;
-    L.c_key = (uint32_t *)p; p += cap * 4;
-    L.i_key = (uint32_t *)p; p += cap * 4;
-    L.i_rc = (uint32_t *)p; p += cap * 4;
-    L.i_flags = (uint32_t *)p; p += cap * 4;
-    L.r_layer = (uint32_t *)p; p += rcap * 4;
-    L.r_beg = (uint32_t *)p; p += rcap * 4;
-    L.r_end = (uint32_t *)p; p += rcap * 4;
-    L.cell_area = (int32_t *)p; p += 256;
-    L.cell_cov = (int32_t *)p;
-    return L;
-}
-
-constexpr uint32_t kMaskBit = 1u << 30, kSkipClipBit = 1u << 31, kNone16 = 0xFFFFu;
 
 __device__ __forceinline__ int32_t cov_byte(uint64_t c, uint32_t y) { return as_i8((int32_t)(c >> (8 * y))); }
+
+// Byte-wise wrapping add of 8 packed i8 row covers.
+__device__ __forceinline__ uint64_t add_bytes(uint64_t a, uint64_t b) {
+    constexpr uint64_t H = 0x8080808080808080ull;
+    return ((a & ~H) + (b & ~H)) ^ ((a ^ b) & H);
+}
+
+__device__ __forceinline__ uint32_t rlane(uint32_t v, uint32_t l) {
+    return (uint32_t)__builtin_amdgcn_readlane((int)v, (int)l);
+}
+__device__ __forceinline__ float rlane_f(float v, uint32_t l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), (int)l));
+}
+__device__ __forceinline__ uint64_t rlane64(uint64_t v, uint32_t l) {
+    return (uint64_t)rlane((uint32_t)v, l) | ((uint64_t)rlane((uint32_t)(v >> 32), l) << 32);
+}
 
 // Cover::is_empty / is_full over 8 rows (painter/mod.rs:164-189).
 __device__ __forceinline__ bool cover_empty(uint64_t c, bool eo) {
@@ -828,246 +814,349 @@ __device__ __forceinline__ uint32_t seg_colp(uint64_t w, uint32_t lb, uint32_t c
     return (uint32_t)(w >> (kPayloadBits + lb)) & cmask;
 }
 
-// Reads the runs (maximal same-layer groups) of tile column `colp` starting
-// at *pos; with SUMS also their per-row cover sums. Returns the run count or
-// -1 on overflow.
-template <bool SUMS>
-__device__ int32_t build_runs(const PaintParams &pp, uint32_t *r_layer, uint32_t *r_beg, uint32_t *r_end,
-                              int32_t *r_cov, uint64_t *pos, uint64_t end, uint32_t colp, uint32_t rcap) {
-    const uint32_t lane = lane_id();
-    const uint32_t lb = pp.layer_bits, lmask = (1u << lb) - 1u;
-    const uint32_t cmask = (1u << pp.col_bits) - 1u;
-    uint64_t p = *pos;
-    int32_t nruns = 0;
-    uint32_t prev_layer = 0xFFFFFFFFu;
-    for (;;) {
-        const uint64_t idx = p + lane;
-        const uint64_t w = idx < end ? pp.segs[idx] : ~0ull;
-        const bool in = idx < end && seg_colp(w, lb, cmask) == colp;
-        const uint64_t inmask = __ballot(in);
-        const uint32_t n_in = (uint32_t)__popcll(inmask);  // a prefix: the stream is sorted
-        const uint32_t layer = (uint32_t)(w >> kPayloadBits) & lmask;
-        uint32_t prevl = __shfl_up(layer, 1);
-        if (lane == 0) prevl = prev_layer;
-        const bool head = in && layer != prevl;
-        const uint64_t heads = __ballot(head);
-        const uint32_t nh = (uint32_t)__popcll(heads);
-        if ((uint32_t)nruns + nh > rcap) return -1;
-        const uint32_t my_run = (uint32_t)nruns + (uint32_t)__popcll(heads & ((2ull << lane) - 1ull)) - 1u;
-        if (head) {
-            r_layer[my_run] = layer;
-            r_beg[my_run] = (uint32_t)(idx - *pos);
-            if (my_run > 0) r_end[my_run - 1] = (uint32_t)(idx - *pos);
-            if (SUMS)
-                for (uint32_t y = 0; y < 8; ++y) r_cov[my_run * 8 + y] = 0;
-        }
-        __syncthreads();
-        if (SUMS && in) {
-            const int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
-            const uint32_t ty = (uint32_t)(w >> 19) & 7u;
-            atomicAdd(&r_cov[my_run * 8 + ty], cover);
-        }
-        nruns += (int32_t)nh;
-        prev_layer = __shfl(layer, n_in ? n_in - 1 : 0);
-        p += n_in;
-        if (n_in < 64) break;
-    }
-    if (nruns > 0 && lane == 0) r_end[nruns - 1] = (uint32_t)(p - *pos);
-    __syncthreads();
-    *pos = p;
-    return nruns;
+// Painted column range [h0, h1) of a scene's rows (crop in tiles).
+__device__ __forceinline__ uint32_t scene_h1(const SceneDesc &S) {
+    return S.crop_h1 < S.row_len ? S.crop_h1 : S.row_len;
 }
 
-// ids = sorted union of run layers and carry layers (populate_layers,
-// layer_workbench.rs:401-428). c_key holds layer | extra bits << 16.
-// Outputs key = layer, run (or -1), carry (or -1). Returns the count.
-__device__ uint32_t merge_ids(const uint32_t *c_key, const uint32_t *r_layer, uint32_t nr, uint32_t nc,
-                              uint32_t *i_key, int32_t *i_run, int32_t *i_carry) {
-    const uint32_t lane = lane_id();
-    const uint32_t n = nr + nc;
-    for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-        const uint32_t k = k0 + lane;
-        if (k < nr) {
-            const uint32_t v = r_layer[k];
-            uint32_t lo = 0, hi = nc;  // #carries < v
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if ((c_key[mid] & 0xFFFFu) < v) lo = mid + 1; else hi = mid;
-            }
-            const uint32_t s = k + lo;
-            i_key[s] = v;
-            i_run[s] = (int32_t)k;
-            i_carry[s] = -1;
-        } else if (k < n) {
-            const uint32_t m = k - nr;
-            const uint32_t v = c_key[m] & 0xFFFFu;
-            uint32_t lo = 0, hi = nr;  // #runs <= v
-            while (lo < hi) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (r_layer[mid] <= v) lo = mid + 1; else hi = mid;
-            }
-            const uint32_t s = m + lo;
-            i_key[s] = v;
-            i_run[s] = -1;
-            i_carry[s] = (int32_t)m;
-        }
+__global__ __launch_bounds__(256) void k_tile_starts(PaintParams pp, uint32_t global_rows) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= global_rows * pp.tstride) return;
+    const uint32_t grow = i / pp.tstride, k = i - grow * pp.tstride;
+    const uint32_t lb = pp.layer_bits, cmask = (1u << pp.col_bits) - 1u;
+    uint32_t lo = pp.row_start[grow], hi = pp.row_start[grow + 1];
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg_colp(pp.segs[mid], lb, cmask) < k) lo = mid + 1; else hi = mid;
     }
-    __syncthreads();
-    // drop the carry entry that follows its run, moving its carry index over
-    uint32_t out = 0;
-    for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-        const uint32_t k = k0 + lane;
-        uint32_t lay = 0;
-        int32_t run = -1, car = -1;
-        bool keep = false;
-        if (k < n) {
-            lay = i_key[k];
-            run = i_run[k];
-            car = i_carry[k];
-            keep = !(k > 0 && i_key[k - 1] == lay);
-            if (keep && k + 1 < n && i_key[k + 1] == lay) car = i_carry[k + 1];
-        }
-        const uint64_t km = __ballot(keep);
-        __syncthreads();
-        if (keep) {
-            const uint32_t d = out + (uint32_t)__popcll(km & lanemask_lt());
-            i_key[d] = lay;
-            i_run[d] = run;
-            i_carry[d] = car;
-        }
-        out += (uint32_t)__popcll(km);
-        __syncthreads();
-    }
-    return out;
+    pp.tile_start[i] = lo;
 }
 
-// The cover queue of one tile row, left to right; stores the queue entering
-// each tile of [h0, h1).
-__global__ __launch_bounds__(64) void k_carry(PaintParams pp) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
+// Runs of tile column colp (col + 1; 0 gathers every col < 0) of row grow.
+// Four columns per work group, one wave each; waves never synchronise.
+__global__ __launch_bounds__(256) void k_tile_runs(PaintParams pp) {
     const uint32_t lane = lane_id();
-    const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs;
-    CarryLds L = carry_lds(lds_raw, cap, rcap);
     const uint32_t grow = blockIdx.x;
-    const uint32_t sc = pp.row_scene[grow];
-    const SceneDesc S = pp.scenes[sc];
+    const uint32_t colp = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const SceneDesc &S = pp.scenes[pp.row_scene[grow]];
+    if (colp > scene_h1(S)) return;
+    const uint32_t *ts_row = pp.tile_start + (uint64_t)grow * pp.tstride;
+    const uint32_t ts = ts_row[colp], te = ts_row[colp + 1];
+    const PropsEntry *props = pp.props + S.props_base;
+    const uint32_t lmask = (1u << pp.layer_bits) - 1u;
+    uint32_t nr = 0, prev_layer = 0xFFFFFFFFu;
+    uint64_t open = 0;  // cover sums of the run still open at the previous chunk's end
+    for (uint32_t p = ts; p < te; p += 256) {
+        uint64_t w[4];
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t idx = p + b * 64 + lane;
+            w[b] = idx < te ? pp.segs[idx] : 0ull;
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < 4; ++b) {
+            const uint32_t base = p + b * 64;
+            if (base >= te) break;
+            const uint32_t nvalid = te - base < 64 ? te - base : 64;
+            const bool valid = lane < nvalid;
+            const uint32_t layer = valid ? (uint32_t)(w[b] >> kPayloadBits) & lmask : 0xFFFFFFFFu;
+            uint32_t prevl = __shfl_up(layer, 1);
+            if (lane == 0) prevl = prev_layer;
+            const bool head = valid && layer != prevl;
+            const uint64_t heads = __ballot(head);
+            // the open run ended exactly at the previous chunk's end
+            if (lane == 0 && head && nr) pp.run_cov[ts + nr - 1] = open;
+            uint64_t s = 0;
+            if (valid) {
+                const int32_t cover = (int32_t)((int64_t)(w[b] << 58) >> 58);
+                const uint32_t ty = (uint32_t)(w[b] >> 19) & 7u;
+                s = (uint64_t)((uint32_t)cover & 0xFFu) << (8 * ty);
+            }
+            // segmented inclusive scan: (f, s) (+) (f', s') = (f | f', f' ? s' : s + s')
+            uint32_t f = head ? 1u : 0u;
+#pragma unroll
+            for (uint32_t d = 1; d < 64; d <<= 1) {
+                const uint64_t sp = __shfl_up(s, d);
+                const uint32_t fp = __shfl_up(f, d);
+                if (lane >= d) {
+                    if (!f) s = add_bytes(sp, s);
+                    f |= fp;
+                }
+            }
+            if (!f) s = add_bytes(open, s);
+            const uint32_t r = nr - 1u + (uint32_t)__popcll(heads & ((2ull << lane) - 1ull));
+            if (head) {
+                const bool eo = props[layer].flags & PF_EVENODD;
+                pp.run_key[ts + r] = layer | (eo ? kRunEvenOdd : 0u);
+                pp.run_beg[ts + r] = base + lane;
+            }
+            // a run ending inside this chunk is complete; the last one stays open
+            if (valid && lane + 1 < nvalid && ((heads >> lane) & 2ull)) pp.run_cov[ts + r] = s;
+            nr += (uint32_t)__popcll(heads);
+            open = rlane64(s, nvalid - 1);
+            prev_layer = rlane(layer, nvalid - 1);
+        }
+    }
+    if (lane == 0) {
+        pp.tile_nr[(uint64_t)grow * pp.tstride + colp] = nr;
+        if (nr) pp.run_cov[ts + nr - 1] = open;
+    }
+}
+
+// Merge-path split of two layer-sorted lists at diagonal d: q elements first
+// on equal layers. Returns the number of q elements before the split.
+__device__ __forceinline__ uint32_t merge_split(const uint32_t *qk, uint32_t nq, const uint32_t *rk, uint32_t nr,
+                                                uint32_t d) {
+    uint32_t lo = d > nr ? d - nr : 0, hi = d < nq ? d : nq;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if ((qk[mid] & 0xFFFFu) <= (rk[d - 1 - mid] & 0xFFFFu)) lo = mid + 1; else hi = mid;
+    }
+    return lo;
+}
+
+// The cover queue of one tile row, left to right (painter/mod.rs:495-596:
+// tiles left of the crop and tile -1 are folded without dropping empty
+// covers; inside the crop, covers that become empty are dropped,
+// layer_workbench.rs:194,475).
+__global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_rec[3];
+    const uint32_t tid = threadIdx.x;
+    const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs, tstride = pp.tstride;
+    unsigned char *p = lds_raw;
+    uint64_t *qc[2];
+    qc[0] = (uint64_t *)p; p += (size_t)cap * 8;
+    qc[1] = (uint64_t *)p; p += (size_t)cap * 8;
+    uint64_t *rc = (uint64_t *)p; p += (size_t)rcap * 8;
+    uint32_t *qk[2];
+    qk[0] = (uint32_t *)p; p += (size_t)cap * 4;
+    qk[1] = (uint32_t *)p; p += (size_t)cap * 4;
+    uint32_t *rk = (uint32_t *)p; p += (size_t)rcap * 4;
+    uint32_t *rb = (uint32_t *)p; p += (size_t)rcap * 4;
+    uint32_t *ts_l = (uint32_t *)p; p += (size_t)tstride * 4;
+    uint32_t *nr_l = (uint32_t *)p;
+
+    const uint32_t grow = blockIdx.x;
+    const SceneDesc S = pp.scenes[pp.row_scene[grow]];
     const uint32_t j = grow - S.row_base;
     if (j < S.crop_v0 || j >= S.crop_v1) return;
-    const PropsEntry *props = pp.props + S.props_base;
-    uint64_t pos = pp.row_start[grow];
-    const uint64_t end = pp.row_start[grow + 1];
-    const uint32_t lb = pp.layer_bits, cmask = (1u << pp.col_bits) - 1u;
-    const uint32_t h0 = S.crop_h0, h1 = S.crop_h1 < S.row_len ? S.crop_h1 : S.row_len;
-    uint32_t cur = 0, nc = 0;
-    bool fold = true;  // tiles left of h0 (and tile -1): unfiltered (painter/mod.rs:515-550)
-    uint32_t c = 0;
-    for (;;) {
-        uint32_t colp;
-        if (fold) {
-            if (pos < end && seg_colp(pp.segs[pos], lb, cmask) <= h0) {
-                colp = seg_colp(pp.segs[pos], lb, cmask);
-            } else {
-                fold = false;
-                c = h0;
-                continue;
-            }
-        } else {
-            if (c >= h1) break;
-            // snapshot of the queue entering tile c
-            uint32_t base = 0;
-            if (lane == 0 && nc) base = atomicAdd(pp.snap_used, nc);
-            base = __shfl(base, 0);
-            const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + c;
-            if ((uint64_t)base + nc > pp.snap_cap) {
-                if (lane == 0) atomicOr(pp.overflow, 2u);
-                return;
-            }
-            for (uint32_t k = lane; k < nc; k += 64) {
-                pp.snap_key[base + k] = L.c_key[cur][k];
-                pp.snap_cov[base + k] = L.c_cov[cur][k];
-            }
-            if (lane == 0) pp.snap_hdr[tile] = make_uint4((uint32_t)pos, nc, base, 0u);
-            colp = c + 1;
-        }
-        const int32_t nr = build_runs<true>(pp, L.r_layer, L.r_beg, L.r_end, L.r_cov, &pos, end, colp, rcap);
-        if (nr < 0 || (uint32_t)nr + nc > cap) {
-            if (lane == 0) atomicOr(pp.overflow, 1u);
+    const uint32_t h0 = S.crop_h0, h1 = scene_h1(S);
+    if (h0 >= h1) return;
+    for (uint32_t k = tid; k < tstride; k += 256) {
+        ts_l[k] = pp.tile_start[(uint64_t)grow * tstride + k];
+        nr_l[k] = k + 1 < tstride ? pp.tile_nr[(uint64_t)grow * tstride + k] : 0u;
+    }
+    __syncthreads();
+    uint32_t nq = 0, cur = 0;
+    uint32_t rec_next = 0, rec_end = 0;
+    for (uint32_t colp = 0; colp <= h1; ++colp) {
+        const bool fold = colp <= h0;
+        const uint32_t nr = nr_l[colp];
+        if (fold && nr == 0) continue;
+        if (nr > rcap || nq + nr > cap) {
+            if (tid == 0) atomicOr(pp.overflow, 1u);
             return;
         }
-        const uint32_t n = merge_ids(L.c_key[cur], L.r_layer, (uint32_t)nr, nc, L.i_key, L.i_run, L.i_carry);
-        // next queue: cover in + run sums (i8 wrapping), Cover::is_empty
-        // drops outside the fold (layer_workbench.rs:194,475)
-        const uint32_t nxt = cur ^ 1u;
-        uint32_t out = 0;
-        for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-            const uint32_t k = k0 + lane;
-            bool keep = false;
-            uint64_t cv = 0;
-            uint32_t key = 0;
-            if (k < n) {
-                const int32_t car = L.i_carry[k], run = L.i_run[k];
-                const uint32_t lay = L.i_key[k];
-                uint32_t eo;
-                uint64_t base_cov = 0;
-                if (car >= 0) {
-                    eo = L.c_key[cur][car] >> 16;
-                    base_cov = L.c_cov[cur][car];
-                } else {
-                    eo = (props[lay].flags & PF_EVENODD) ? 1u : 0u;
-                }
-                uint64_t r = 0;
-                for (uint32_t y = 0; y < 8; ++y) {
-                    const int32_t sum = cov_byte(base_cov, y) + (run >= 0 ? L.r_cov[run * 8 + y] : 0);
-                    r |= (uint64_t)((uint32_t)sum & 0xFFu) << (8 * y);
-                }
-                cv = r;
-                key = lay | (eo << 16);
-                keep = fold || !cover_empty(cv, eo != 0);
-            }
-            const uint64_t km = __ballot(keep);
-            if (keep) {
-                const uint32_t d = out + (uint32_t)__popcll(km & lanemask_lt());
-                L.c_key[nxt][d] = key;
-                L.c_cov[nxt][d] = cv;
-            }
-            out += (uint32_t)__popcll(km);
+        const uint32_t ts = ts_l[colp], te = ts_l[colp + 1];
+        for (uint32_t k = tid; k < nr; k += 256) {
+            rk[k] = pp.run_key[ts + k];
+            rb[k] = pp.run_beg[ts + k];
+            rc[k] = pp.run_cov[ts + k];
         }
         __syncthreads();
-        nc = out;
+        const uint32_t *qkc = qk[cur];
+        const uint64_t *qcc = qc[cur];
+        const uint32_t total = nq + nr;
+        const uint32_t per = (total + 255) / 256;
+        const uint32_t d0 = tid * per < total ? tid * per : total;
+        const uint32_t d1 = d0 + per < total ? d0 + per : total;
+        const uint32_t i0 = merge_split(qkc, nq, rk, nr, d0);
+        // union element -> (q index or -1, r index or -1); second halves of
+        // equal-layer pairs are skipped
+        auto walk = [&](auto &&emit) {
+            uint32_t i = i0, r = d0 - i0;
+            for (uint32_t m = d0; m < d1; ++m) {
+                const bool take_q = i < nq && (r >= nr || (qkc[i] & 0xFFFFu) <= (rk[r] & 0xFFFFu));
+                if (take_q) {
+                    const uint32_t lay = qkc[i] & 0xFFFFu;
+                    const int32_t ri = (r < nr && (rk[r] & 0xFFFFu) == lay) ? (int32_t)r : -1;
+                    emit(lay, (int32_t)i, ri);
+                    ++i;
+                } else {
+                    const uint32_t lay = rk[r] & 0xFFFFu;
+                    if (!(i > 0 && (qkc[i - 1] & 0xFFFFu) == lay)) emit(lay, -1, (int32_t)r);
+                    ++r;
+                }
+            }
+        };
+        auto next_cover = [&](int32_t qi, int32_t ri, bool *keep) {
+            const uint64_t a = qi >= 0 ? qcc[qi] : 0ull, b = ri >= 0 ? rc[ri] : 0ull;
+            const uint64_t v = add_bytes(a, b);
+            const bool eo = qi >= 0 ? (qkc[qi] & kRunEvenOdd) != 0 : (rk[ri] & kRunEvenOdd) != 0;
+            *keep = fold || !cover_empty(v, eo);
+            return v;
+        };
+        uint32_t cu = 0, cq = 0;
+        walk([&](uint32_t, int32_t qi, int32_t ri) {
+            bool keep;
+            next_cover(qi, ri, &keep);
+            ++cu;
+            cq += keep ? 1u : 0u;
+        });
+        uint32_t all;
+        const uint32_t ex = block_excl_scan_256(cu | (cq << 16), s_wsum, &all);
+        const uint32_t n_union = all & 0xFFFFu, nq_new = all >> 16;
+        uint32_t rec_off = 0;
+        if (!fold) {
+            if (rec_next + n_union > rec_end) {
+                if (tid == 0) {
+                    const uint32_t sz = n_union > kRecChunk ? n_union : kRecChunk;
+                    const uint32_t base = atomicAdd(pp.rec_used, sz);
+                    s_rec[0] = base;
+                    s_rec[1] = base + sz;
+                    s_rec[2] = (uint64_t)base + sz > pp.rec_cap ? 1u : 0u;
+                }
+                __syncthreads();
+                if (s_rec[2]) {
+                    if (tid == 0) atomicOr(pp.overflow, 2u);
+                    return;
+                }
+                rec_next = s_rec[0];
+                rec_end = s_rec[1];
+            }
+            rec_off = rec_next;
+            rec_next += n_union;
+        }
+        const uint32_t nxt = cur ^ 1u;
+        uint32_t ou = rec_off + (ex & 0xFFFFu), oq = ex >> 16;
+        walk([&](uint32_t lay, int32_t qi, int32_t ri) {
+            bool keep;
+            const uint64_t v = next_cover(qi, ri, &keep);
+            if (!fold) {
+                uint32_t key = lay, beg = 0, len = 0;
+                uint64_t cv = 0;
+                if (ri >= 0) {
+                    key |= kRecRun;
+                    beg = rb[ri];
+                    len = ((uint32_t)ri + 1 < nr ? rb[ri + 1] : te) - beg;
+                }
+                if (qi >= 0) {
+                    key |= kRecCarry;
+                    cv = qcc[qi];
+                }
+                pp.rec[ou] = make_uint4(key, beg, (uint32_t)cv, (uint32_t)(cv >> 32));
+                pp.rec_len[ou] = len;
+                ++ou;
+            }
+            if (keep) {
+                qk[nxt][oq] = lay | ((qi >= 0 ? qkc[qi] : rk[ri]) & kRunEvenOdd);
+                qc[nxt][oq] = v;
+                ++oq;
+            }
+        });
+        if (!fold && tid == 0) {
+            const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + (colp - 1);
+            pp.rec_hdr[tile] = make_uint2(rec_off, n_union);
+        }
+        __syncthreads();
+        nq = nq_new;
         cur = nxt;
-        if (!fold) ++c;
     }
 }
 
-// A = 16 * P + area for this lane's pixel and id k (painter/mod.rs:229-243,
-// 347-362): cells from the id's run (segments at tbase + [beg, end)) and its
-// carried cover in column 0; P is the i8 running cover prefix.
-__device__ __forceinline__ int32_t pixel_area(const PaintParams &pp, const PaintLds &L, uint64_t tbase,
-                                              int32_t run, int32_t cy, uint32_t px, uint32_t py) {
+// Staged window of sorted words for the paint wave.
+struct Stage {
+    uint64_t *w;
+    uint32_t base, end;
+};
+
+__device__ __forceinline__ void stage_load(Stage &st, const uint64_t *segs, uint32_t from, uint32_t to) {
     const uint32_t lane = lane_id();
-    if (run < 0) return 16 * cy;
-    L.cell_area[lane] = 0;
-    L.cell_cov[lane] = 0;
     __syncthreads();
-    const uint32_t beg = L.r_beg[run], en = L.r_end[run];
-    for (uint32_t s0 = beg; s0 < en; s0 += 64) {
-        const uint32_t s = s0 + lane;
-        if (s < en) {
-            const uint64_t w = pp.segs[tbase + s];
+    for (uint32_t k = lane; k < to - from; k += 64) st.w[k] = segs[from + k];
+    __syncthreads();
+    st.base = from;
+    st.end = to;
+}
+
+// A = 16 * P + area for this lane's pixel (painter/mod.rs:229-243, 347-362):
+// P is the i8 running cover prefix of row py over columns < px starting from
+// the carried cover cy, area the i16 area sum of cell (px, py); the record's
+// run is words [beg, beg + len) of the sorted stream, all inside [.., lim).
+__device__ __forceinline__ int32_t run_area(Stage &st, const uint64_t *segs, uint32_t beg, uint32_t len,
+                                            uint32_t lim, int32_t cy, uint32_t px, uint32_t py) {
+    int32_t P = cy, ar = 0;
+    uint32_t s = beg;
+    const uint32_t e = beg + len;
+    const uint32_t hi = lim > e ? lim : e;
+    while (s < e) {
+        if (s < st.base || s >= st.end) stage_load(st, segs, s, hi - s < kStageWin ? hi : s + kStageWin);
+        const uint32_t stop = e < st.end ? e : st.end;
+        for (; s < stop; ++s) {
+            const uint64_t w = st.w[s - st.base];
+            const uint32_t x = (uint32_t)(w >> 16) & 7u, y = (uint32_t)(w >> 19) & 7u;
             const int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
             const int32_t area = (int32_t)((int64_t)(w << 48) >> 54);
-            const uint32_t cell = ((uint32_t)(w >> 16) & 7u) * 8u + ((uint32_t)(w >> 19) & 7u);
-            atomicAdd(&L.cell_area[cell], area);
-            atomicAdd(&L.cell_cov[cell], cover);
+            if (y == py) {
+                if (x < px) P += cover;
+                else if (x == px) ar += area;
+            }
         }
     }
-    __syncthreads();
-    int32_t P = cy;
-    for (uint32_t x = 0; x < 7; ++x)
-        if (x < px) P += L.cell_cov[x * 8 + py];
-    const int32_t A = 16 * as_i8(P) + as_i16(L.cell_area[lane]);
-    __syncthreads();
-    return A;
+    return 16 * as_i8(P) + as_i16(ar);
+}
+
+// One batch of up to 64 records, lane k holding record k0 + k.
+struct Batch {
+    uint32_t key, beg, len, flags;
+    uint64_t cov;
+    float4 color;
+    uint32_t lim;  // end of the batch's last run
+};
+
+__device__ __forceinline__ void load_batch(const PaintParams &pp, const PropsEntry *props, const uint32_t *fl,
+                                           const uint4 *rec, const uint32_t *rlen, uint32_t n, uint32_t k0,
+                                           Batch &b) {
+    const uint32_t k = k0 + lane_id();
+    const bool in = k < n;
+    const uint4 rv = in ? rec[k] : make_uint4(0, 0, 0, 0);
+    b.key = rv.x;
+    b.beg = rv.y;
+    b.cov = (uint64_t)rv.z | ((uint64_t)rv.w << 32);
+    b.len = in ? rlen[k] : 0u;
+    const uint32_t lay = rv.x & 0xFFFFu;
+    if (in) {
+        const PropsEntry &pe = props[lay];
+        b.flags = fl ? fl[k] : ((pe.flags & ~(kMaskBit | kSkipClipBit)) | kMaskBit);
+        b.color = make_float4(pe.color[0], pe.color[1], pe.color[2], pe.color[3]);
+    } else {
+        b.flags = 0;
+        b.color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+    const uint64_t rm = __ballot(in && (rv.x & kRecRun));
+    b.lim = 0;
+    if (rm) {
+        const uint32_t last = 63u - (uint32_t)__clzll((long long)rm);
+        b.lim = rlane(b.beg + b.len, last);
+    }
+}
+
+// Stages the batch's runs when they fit the window.
+__device__ __forceinline__ void stage_batch(Stage &st, const uint64_t *segs, const Batch &b) {
+    const uint64_t rm = __ballot(b.key & kRecRun);
+    if (!rm) return;
+    const uint32_t first = rlane(b.beg, (uint32_t)__ffsll((long long)rm) - 1u);
+    if (b.lim - first <= kStageWin && (first < st.base || b.lim > st.end)) stage_load(st, segs, first, b.lim);
+}
+
+__device__ __forceinline__ int32_t record_area(Stage &st, const uint64_t *segs, const Batch &b, uint32_t q,
+                                               uint32_t px, uint32_t py) {
+    const uint32_t key = rlane(b.key, q);
+    const int32_t cy = (key & kRecCarry) ? cov_byte(rlane64(b.cov, q), py) : 0;
+    if (!(key & kRecRun)) return 16 * cy;
+    return run_area(st, segs, rlane(b.beg, q), rlane(b.len, q), b.lim, cy, px, py);
 }
 
 // One wave paints one tile (LayerWorkbench::drive_tile_painting,
@@ -1075,9 +1164,7 @@ __device__ __forceinline__ int32_t pixel_area(const PaintParams &pp, const Paint
 __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     const uint32_t lane = lane_id();
-    const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs;
-    PaintLds L = paint_lds(lds_raw, cap, rcap);
-    // the carry pass stopped early: its snapshots are incomplete
+    // the row walk stopped early: its records are incomplete
     if (__hip_atomic_load(pp.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     const uint2 wk = pp.work[blockIdx.x];
     const uint32_t grow = wk.x, c = wk.y;
@@ -1085,65 +1172,57 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     const SceneDesc S = pp.scenes[sc];
     const uint32_t j = grow - S.row_base;
     const PropsEntry *props = pp.props + S.props_base;
-    const uint64_t end = pp.row_start[grow + 1];
     const uint32_t px = lane >> 3, py = lane & 7u;  // pixel of this lane (x*8+y)
     const bool clips = S.flags & SF_CLIPS;
     const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + c;
-    const uint4 hdr = pp.snap_hdr[tile];
-    uint64_t pos = hdr.x;
-    const uint32_t nc = hdr.y;
-    if (nc > cap) {
-        if (lane == 0) atomicOr(pp.overflow, 1u);
-        return;
+    const uint2 hdr = pp.rec_hdr[tile];
+    const uint32_t n = hdr.y;
+    const uint4 *rec = pp.rec + hdr.x;
+    const uint32_t *rlen = pp.rec_len + hdr.x;
+    Stage st;
+    st.w = (uint64_t *)lds_raw;
+    st.base = st.end = 0;
+    uint32_t *fl = clips ? (uint32_t *)(lds_raw + kStageWin * 8) : nullptr;
+    const uint64_t *segs = pp.segs;
+    if (fl) {
+        for (uint32_t k = lane; k < n; k += 64)
+            fl[k] = (props[rec[k].x & 0xFFFFu].flags & ~(kMaskBit | kSkipClipBit)) | kMaskBit;
+        __syncthreads();
     }
-    const uint64_t *c_cov = pp.snap_cov + hdr.z;
-    for (uint32_t k = lane; k < nc; k += 64) L.c_key[k] = pp.snap_key[hdr.z + k];
-    __syncthreads();
-
-    // ---- runs and ids of the tile
-    const uint64_t tbase = pos;
-    const int32_t nr = build_runs<false>(pp, L.r_layer, L.r_beg, L.r_end, nullptr, &pos, end, c + 1, rcap);
-    if (nr < 0 || (uint32_t)nr + nc > cap) {
-        if (lane == 0) atomicOr(pp.overflow, 1u);
-        return;
-    }
-    int32_t *i_run = (int32_t *)L.i_flags;  // scratch for the merge, replaced below
-    int32_t *i_car = (int32_t *)L.i_rc;
-    const uint32_t n = merge_ids(L.c_key, L.r_layer, (uint32_t)nr, nc, L.i_key, i_run, i_car);
-    for (uint32_t k0 = 0; k0 < n; k0 += 64) {
-        const uint32_t k = k0 + lane;
-        if (k < n) {
-            const int32_t run = i_run[k], car = i_car[k];
-            const PropsEntry &p = props[L.i_key[k]];
-            L.i_rc[k] = (run < 0 ? kNone16 : (uint32_t)run) | ((car < 0 ? kNone16 : (uint32_t)car) << 16);
-            L.i_flags[k] = (p.flags & ~(kMaskBit | kSkipClipBit)) | kMaskBit;
-            L.i_color[k] = make_float4(p.color[0], p.color[1], p.color[2], p.color[3]);
-        }
-    }
-    __syncthreads();
 
     // ---- coverage dump (debug): A of every id, independent of the passes
     if (pp.cov_a) {
-        for (uint32_t k = 0; k < n; ++k) {
-            const uint32_t rc = L.i_rc[k];
-            const int32_t run = (rc & 0xFFFFu) == kNone16 ? -1 : (int32_t)(rc & 0xFFFFu);
-            const int32_t car = (rc >> 16) == kNone16 ? -1 : (int32_t)(rc >> 16);
-            const int32_t cy = car >= 0 ? cov_byte(c_cov[car], py) : 0;
-            const int32_t A = pixel_area(pp, L, tbase, run, cy, px, py);
-            uint32_t rec = 0;
-            if (lane == 0) rec = atomicAdd(pp.cov_count, 1u);
-            rec = __shfl(rec, 0);
-            if (rec < pp.cov_cap) {
-                pp.cov_a[(uint64_t)rec * 64 + lane] = A;
-                if (lane == 0) {
-                    pp.cov_key[rec * 4 + 0] = sc;
-                    pp.cov_key[rec * 4 + 1] = j;
-                    pp.cov_key[rec * 4 + 2] = c;
-                    pp.cov_key[rec * 4 + 3] = L.i_key[k];
+        for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+            Batch b;
+            load_batch(pp, props, fl, rec, rlen, n, k0, b);
+            stage_batch(st, segs, b);
+            const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const int32_t A = record_area(st, segs, b, q, px, py);
+                uint32_t r = 0;
+                if (lane == 0) r = atomicAdd(pp.cov_count, 1u);
+                r = __shfl(r, 0);
+                if (r < pp.cov_cap) {
+                    pp.cov_a[(uint64_t)r * 64 + lane] = A;
+                    if (lane == 0) {
+                        pp.cov_key[r * 4 + 0] = sc;
+                        pp.cov_key[r * 4 + 1] = j;
+                        pp.cov_key[r * 4 + 2] = c;
+                        pp.cov_key[r * 4 + 3] = rlane(b.key, q) & 0xFFFFu;
+                    }
                 }
             }
         }
     }
+
+    auto id_flags = [&](uint32_t k, uint32_t lay) {
+        return fl ? fl[k] : ((props[lay].flags & ~(kMaskBit | kSkipClipBit)) | kMaskBit);
+    };
+    // a carried-only id whose cover is full (Cover::is_full)
+    auto is_full = [&](const uint4 &rv, uint32_t f) {
+        if ((rv.x & kRecRun) || !(rv.x & kRecCarry)) return false;
+        return cover_full((uint64_t)rv.z | ((uint64_t)rv.w << 32), f & PF_EVENODD);
+    };
 
     // ---- workbench passes (layer_workbench.rs:390-399)
     int op = 2;  // 0 None, 1 Solid, 2 ColorBuffer
@@ -1154,7 +1233,9 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
         bool changed = false;
         for (uint32_t k0 = 0; k0 < n; k0 += 64) {
             const uint32_t k = k0 + lane;
-            if (__ballot(k < n && !(L.i_flags[k] & PF_UNCHANGED))) changed = true;
+            bool ch = false;
+            if (k < n) ch = !(props[rec[k].x & 0xFFFFu].flags & PF_UNCHANGED);
+            if (__ballot(ch)) changed = true;
         }
         int32_t *prev = pp.prev_layers + tile;
         const int32_t old = *prev;
@@ -1163,21 +1244,17 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
         if (lane == 0) *prev = now;
         if (old >= 0 && old == now && !changed) op = 0;
     }
-    auto is_full = [&](uint32_t k, uint32_t f) {
-        const uint32_t rc = L.i_rc[k];
-        if ((rc & 0xFFFFu) != kNone16 || (rc >> 16) == kNone16) return false;
-        return cover_full(c_cov[rc >> 16], f & PF_EVENODD);
-    };
     if (op != 0 && clips) {
         // skip_trivial_clips_pass (layer_workbench.rs:224-308): serial, uniform
         bool has = false, full = false, used = false;
         uint32_t last = 0, ci = 0;
         for (uint32_t k = 0; k < n; ++k) {
-            uint32_t f = L.i_flags[k];
-            const uint32_t lay = L.i_key[k];
+            uint32_t f = fl[k];
+            const uint4 rv = rec[k];
+            const uint32_t lay = rv.x & 0xFFFFu;
             if (!(f & kMaskBit)) continue;
             if (f & PF_CLIP) {
-                const bool isfull = is_full(k, f);
+                const bool isfull = is_full(rv, f);
                 has = true;
                 full = isfull;
                 last = (lay + props[lay].clip_layers) & 0xFFFFu;
@@ -1192,14 +1269,14 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
                 }
             }
             __syncthreads();
-            if (lane == 0) L.i_flags[k] = f;
+            if (lane == 0) fl[k] = f;
             if (has && lay > last) {
                 has = false;
-                if (!used && lane == 0) L.i_flags[ci] &= ~kMaskBit;
+                if (!used && lane == 0) fl[ci] &= ~kMaskBit;
             }
             __syncthreads();
         }
-        if (has && !used && lane == 0) L.i_flags[ci] &= ~kMaskBit;
+        if (has && !used && lane == 0) fl[ci] &= ~kMaskBit;
         __syncthreads();
     }
     if (op != 0) {
@@ -1210,10 +1287,11 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
             const int32_t k = top - (int32_t)lane;
             bool pI = false, pO = false;
             if (k >= 0) {
-                const uint32_t f = L.i_flags[k];
+                const uint4 rv = rec[k];
+                const uint32_t f = id_flags((uint32_t)k, rv.x & 0xFFFFu);
                 if (f & kMaskBit) {
                     const bool clipped = !(f & PF_CLIP) && (f & PF_CLIPPED) && !(f & kSkipClipBit);
-                    pI = clipped || !is_full((uint32_t)k, f);
+                    pI = clipped || !is_full(rv, f);
                     pO = !pI && (f & PF_OPAQUE);
                 }
             }
@@ -1232,8 +1310,8 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
             uint32_t from = 0;
             float col[4];
             if (first == 1) {
-                const float4 oc = L.i_color[k_opaque];
-                col[0] = oc.x; col[1] = oc.y; col[2] = oc.z; col[3] = oc.w;
+                const PropsEntry &pe = props[rec[k_opaque].x & 0xFFFFu];
+                col[0] = pe.color[0]; col[1] = pe.color[1]; col[2] = pe.color[2]; col[3] = pe.color[3];
                 from = (uint32_t)k_opaque + 1;
             } else {
                 col[0] = S.clear[0]; col[1] = S.clear[1]; col[2] = S.clear[2]; col[3] = S.clear[3];
@@ -1241,17 +1319,25 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
             bool ok = true;
             for (uint32_t k0 = from; k0 < n; k0 += 64) {
                 const uint32_t k = k0 + lane;
-                const bool bad = k < n && (L.i_flags[k] & kMaskBit) &&
-                                 ((L.i_flags[k] & PF_CLIP) || (L.i_flags[k] & PF_GRADIENT));
+                bool bad = false;
+                if (k < n) {
+                    const uint32_t f = id_flags(k, rec[k].x & 0xFFFFu);
+                    bad = (f & kMaskBit) && ((f & PF_CLIP) || (f & PF_GRADIENT));
+                }
                 if (__ballot(bad)) ok = false;
             }
             if (ok) {
-                for (uint32_t k = from; k < n; ++k) {
-                    const uint32_t f = L.i_flags[k];
-                    if (!(f & kMaskBit)) continue;
-                    const float4 sc4 = L.i_color[k];
-                    const float src[4] = {sc4.x, sc4.y, sc4.z, sc4.w};
-                    blend_color((f >> PF_BLEND_SHIFT) & 15u, col, src);
+                for (uint32_t k0 = from; k0 < n; k0 += 64) {
+                    Batch b;
+                    load_batch(pp, props, fl, rec, rlen, n, k0, b);
+                    const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
+                    for (uint32_t q = 0; q < cnt; ++q) {
+                        const uint32_t f = rlane(b.flags, q);
+                        if (!(f & kMaskBit)) continue;
+                        const float src[4] = {rlane_f(b.color.x, q), rlane_f(b.color.y, q), rlane_f(b.color.z, q),
+                                              rlane_f(b.color.w, q)};
+                        blend_color((f >> PF_BLEND_SHIFT) & 15u, col, src);
+                    }
                 }
                 op = 1;
                 solid0 = col[0]; solid1 = col[1]; solid2 = col[2]; solid3 = col[3];
@@ -1266,49 +1352,51 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
         bool has_clip = false;
         uint32_t clip_last = 0;
         float clip_mask = 0.0f;
-        for (uint32_t k = skipped; k < n; ++k) {
-            const uint32_t f = L.i_flags[k];
-            if (!(f & kMaskBit)) continue;
-            const uint32_t rc = L.i_rc[k];
-            const float4 fc = L.i_color[k];
-            const uint32_t lay = L.i_key[k];
-            if (has_clip && clip_last < lay) has_clip = false;
-            const int32_t run = (rc & 0xFFFFu) == kNone16 ? -1 : (int32_t)(rc & 0xFFFFu);
-            const int32_t car = (rc >> 16) == kNone16 ? -1 : (int32_t)(rc >> 16);
-            const int32_t cy = car >= 0 ? cov_byte(c_cov[car], py) : 0;
-            const int32_t A = pixel_area(pp, L, tbase, run, cy, px, py);
-            const float cov = coverage_from_area(A, f & PF_EVENODD);
-            if (f & PF_CLIP) {
-                // clip_at (painter/mod.rs:402-416)
-                if (!has_clip) {
-                    has_clip = true;
-                    clip_last = (lay + props[lay].clip_layers) & 0xFFFFu;
+        for (uint32_t k0 = skipped; k0 < n; k0 += 64) {
+            Batch b;
+            load_batch(pp, props, fl, rec, rlen, n, k0, b);
+            stage_batch(st, segs, b);
+            const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const uint32_t f = rlane(b.flags, q);
+                if (!(f & kMaskBit)) continue;
+                const uint32_t lay = rlane(b.key, q) & 0xFFFFu;
+                if (has_clip && clip_last < lay) has_clip = false;
+                const int32_t A = record_area(st, segs, b, q, px, py);
+                const float cov = coverage_from_area(A, f & PF_EVENODD);
+                if (f & PF_CLIP) {
+                    // clip_at (painter/mod.rs:402-416)
+                    if (!has_clip) {
+                        has_clip = true;
+                        clip_last = (lay + props[lay].clip_layers) & 0xFFFFu;
+                    }
+                    clip_mask = cov;
+                    continue;
                 }
-                clip_mask = cov;
-                continue;
+                const bool apply_clip = (f & PF_CLIPPED) && !(f & kSkipClipBit);
+                const uint64_t nz = __ballot(cov != 0.0f);
+                const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
+                if (col_zero || (apply_clip && !has_clip)) continue;
+                float f0 = rlane_f(b.color.x, q), f1 = rlane_f(b.color.y, q), f2 = rlane_f(b.color.z, q),
+                      f3 = rlane_f(b.color.w, q);
+                if (f & PF_GRADIENT) {
+                    float g[4];
+                    gradient_color(props[lay], pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, g);
+                    f0 = g[0]; f1 = g[1]; f2 = g[2]; f3 = g[3];
+                }
+                // blend_at (painter/mod.rs:364-400)
+                float a = f3 * cov;
+                if (apply_clip) a *= clip_mask;
+                const float inv = 1.0f - a;
+                const uint32_t mode = (f >> PF_BLEND_SHIFT) & 15u;
+                const float b0 = blend_lane(mode, c0, f0) * a;
+                const float b1 = blend_lane(mode, c1, f1) * a;
+                const float b2 = blend_lane(mode, c2, f2) * a;
+                c0 = __builtin_fmaf(c0, inv, b0);
+                c1 = __builtin_fmaf(c1, inv, b1);
+                c2 = __builtin_fmaf(c2, inv, b2);
+                ca = __builtin_fmaf(ca, inv, a);
             }
-            const bool apply_clip = (f & PF_CLIPPED) && !(f & kSkipClipBit);
-            const uint64_t nz = __ballot(cov != 0.0f);
-            const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
-            if (col_zero || (apply_clip && !has_clip)) continue;
-            float f0 = fc.x, f1 = fc.y, f2 = fc.z, f3 = fc.w;
-            if (f & PF_GRADIENT) {
-                float g[4];
-                gradient_color(props[lay], pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, g);
-                f0 = g[0]; f1 = g[1]; f2 = g[2]; f3 = g[3];
-            }
-            // blend_at (painter/mod.rs:364-400)
-            float a = f3 * cov;
-            if (apply_clip) a *= clip_mask;
-            const float inv = 1.0f - a;
-            const uint32_t mode = (f >> PF_BLEND_SHIFT) & 15u;
-            const float b0 = blend_lane(mode, c0, f0) * a;
-            const float b1 = blend_lane(mode, c1, f1) * a;
-            const float b2 = blend_lane(mode, c2, f2) * a;
-            c0 = __builtin_fmaf(c0, inv, b0);
-            c1 = __builtin_fmaf(c1, inv, b1);
-            c2 = __builtin_fmaf(c2, inv, b2);
-            ca = __builtin_fmaf(ca, inv, a);
         }
         pix = pixel_bytes(c0, c1, c2, ca);
     } else if (op == 1) {

@@ -80,7 +80,7 @@ public:
 
 private:
     bool paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
-               const uint64_t *sorted, uint8_t *image, int32_t *prev_layers, size_t tiles,
+               const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
                std::string &err);
 
     int device_ = -1;
@@ -97,12 +97,11 @@ private:
     DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
     DevBuf row_count_, row_start_, row_scene_, row_status_;
     DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
-    DevBuf work_, snap_hdr_, snap_key_, snap_cov_;
-    size_t snap_cap_ = 0;
+    DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
+    uint64_t rec_cap_ = 0;  // id records of the last frame's size
     std::vector<DevBuf> caches_;
     std::vector<size_t> cache_tiles_;
     const uint64_t *last_sorted_ = nullptr;
-    uint32_t cap_ids_ = 128, cap_runs_ = 128;  // painter LDS capacities of the last frame
 };
 
 // Number of usable gfx950 devices.

@@ -62,8 +62,8 @@ Renderer::~Renderer() {
                      &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &image_, &overflow_,
-                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &snap_hdr_, &snap_key_,
-                     &snap_cov_};
+                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &tile_start_, &tile_nr_,
+                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_};
     for (DevBuf *b : all) b->release();
     for (DevBuf &b : caches_) b.release();
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -374,7 +374,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
             }
             fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         }
-        if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, image, prev_layers, tiles, err))
+        if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, cap_segs, image, prev_layers, tiles, err))
             return false;
     }
     MG_HIP(hipEventRecord(evs_[4], stream_), "event");
@@ -395,14 +395,18 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
 }
 
 bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
-                     const uint64_t *sorted, uint8_t *image, int32_t *prev_layers, size_t tiles,
+                     const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
                      std::string &err) {
     if (global_rows == 0) return true;
     // one paint work group per tile inside the crop
     std::vector<uint32_t> work;
+    uint32_t tstride = 2;
+    bool clips = false;
     for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
         const SceneDesc &s = spec.scenes[sc];
         const uint32_t h1 = std::min(s.crop_h1, s.row_len);
+        tstride = std::max(tstride, h1 + 2);
+        clips |= (s.flags & SF_CLIPS) != 0;
         for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j)
             for (uint32_t c = s.crop_h0; c < h1; ++c) {
                 work.push_back(s.row_base + j);
@@ -412,8 +416,11 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     const uint32_t n_work = (uint32_t)(work.size() / 2);
     if (n_work == 0) return true;
     if (!upload_vec(work_, work, stream_, err)) return false;
-    if (!snap_hdr_.reserve(std::max<size_t>(tiles, 1) * 16)) {
-        err = "out of device memory (carry snapshots)";
+    const size_t n_cols = (size_t)global_rows * tstride;
+    if (!rec_hdr_.reserve(std::max<size_t>(tiles, 1) * 8) || !tile_start_.reserve(n_cols * 4) ||
+        !tile_nr_.reserve(n_cols * 4) || !run_key_.reserve(words_cap * 4) || !run_beg_.reserve(words_cap * 4) ||
+        !run_cov_.reserve(words_cap * 8)) {
+        err = "out of device memory (tile runs)";
         return false;
     }
 
@@ -430,8 +437,14 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.layer_bits = layer_bits;
     pp.overflow = overflow_.as<uint32_t>();
     pp.tile_ops = spec.want_tile_ops ? tile_ops_.as<int8_t>() : nullptr;
-    pp.snap_hdr = snap_hdr_.as<uint4>();
-    pp.snap_used = overflow_.as<uint32_t>() + 1;
+    pp.tile_start = tile_start_.as<uint32_t>();
+    pp.tile_nr = tile_nr_.as<uint32_t>();
+    pp.tstride = tstride;
+    pp.run_key = run_key_.as<uint32_t>();
+    pp.run_beg = run_beg_.as<uint32_t>();
+    pp.run_cov = run_cov_.as<uint64_t>();
+    pp.rec_hdr = rec_hdr_.as<uint2>();
+    pp.rec_used = overflow_.as<uint32_t>() + 1;
     pp.work = work_.as<uint2>();
     if (spec.want_coverage) {
         pp.cov_a = cov_a_.as<int32_t>();
@@ -439,62 +452,55 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.cov_count = cov_count_.as<uint32_t>();
         pp.cov_cap = spec.coverage_cap;
     }
-    // BufferLayerCache state is updated by the painter: keep a copy so a
-    // retry starts from the same state.
-    DevBuf saved;
-    if (prev_layers) {
-        if (!saved.reserve(std::max<size_t>(tiles, 1) * 4)) { err = "out of device memory"; return false; }
-        MG_HIP(hipMemcpyAsync(saved.ptr, prev_layers, tiles * 4, hipMemcpyDeviceToDevice, stream_), "save cache");
-    }
-    static const uint32_t caps[][2] = {{128, 128}, {512, 256}, {1024, 512}, {2048, 768}, {3072, 1024},
-                                       {4096, 1024}};
+    k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows);
+    k_tile_runs<<<dim3(global_rows, (tstride - 1 + 3) / 4), 256, 0, stream_>>>(pp);
+    MG_HIP(hipGetLastError(), "tile runs launch");
+
+    // Row-walk LDS: the whole work group's share (one work group per CU is
+    // enough for the tile rows); runs of one tile up to rcap.
     size_t lds_max = 65536;
     {
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, device_) == hipSuccess) lds_max = p.sharedMemPerBlock;
     }
-    int level = 0;
-    while (level < 5 && caps[level][0] < cap_ids_) ++level;
-    if (snap_cap_ < tiles * 16) snap_cap_ = tiles * 16;
-    for (int attempt = 0; attempt < 12; ++attempt) {
-        const uint32_t cap = caps[level][0], rcap = caps[level][1];
-        const size_t lds_c = carry_lds_bytes_host(cap, rcap), lds_p = paint_lds_bytes_host(cap, rcap);
-        if (std::max(lds_c, lds_p) > lds_max) break;
-        if (!snap_key_.reserve(snap_cap_ * 4) || !snap_cov_.reserve(snap_cap_ * 8)) {
-            err = "out of device memory (carry snapshots)";
+    const uint32_t rcap = 2048;
+    const size_t fixed = walk_lds_bytes_host(0, rcap, tstride) + 256;
+    if (fixed + 64 * 24 > lds_max) {
+        err = "canvas too wide for the row walk";
+        return false;
+    }
+    const uint32_t cap = (uint32_t)std::min<size_t>(((lds_max - fixed) / 24) & ~size_t(63), 0xFFFFu - rcap);
+    const size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride), lds_p = paint_lds_bytes_host(cap, clips);
+    pp.cap_ids = cap;
+    pp.cap_runs = rcap;
+    MG_HIP(hipFuncSetAttribute((const void *)k_row_walk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_w),
+           "walk lds");
+    MG_HIP(hipFuncSetAttribute((const void *)k_paint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p),
+           "paint lds");
+    if (rec_cap_ < tiles * 64) rec_cap_ = tiles * 64;
+    for (int attempt = 0; attempt < 8; ++attempt) {
+        if (rec_cap_ > 0xFFFF0000ull) rec_cap_ = 0xFFFF0000ull;
+        if (!rec_.reserve(rec_cap_ * 16) || !rec_len_.reserve(rec_cap_ * 4)) {
+            err = "out of device memory (tile id records)";
             return false;
         }
-        pp.snap_key = snap_key_.as<uint32_t>();
-        pp.snap_cov = snap_cov_.as<uint64_t>();
-        pp.snap_cap = snap_cap_;
-        if (attempt > 0 && prev_layers)
-            MG_HIP(hipMemcpyAsync(prev_layers, saved.ptr, tiles * 4, hipMemcpyDeviceToDevice, stream_), "restore cache");
+        pp.rec = rec_.as<uint4>();
+        pp.rec_len = rec_len_.as<uint32_t>();
+        pp.rec_cap = rec_cap_;
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 2, stream_);
-        pp.cap_ids = cap;
-        pp.cap_runs = rcap;
-        MG_HIP(hipFuncSetAttribute((const void *)k_carry, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_c),
-               "carry lds");
-        MG_HIP(hipFuncSetAttribute((const void *)k_paint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p),
-               "paint lds");
-        k_carry<<<global_rows, 64, lds_c, stream_>>>(pp);
+        k_row_walk<<<global_rows, 256, lds_w, stream_>>>(pp);
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
         MG_HIP(hipGetLastError(), "paint launch");
         uint32_t ov[2] = {0, 0};
         MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 8, hipMemcpyDeviceToHost, stream_), "read overflow");
         MG_HIP(hipStreamSynchronize(stream_), "paint");
-        if (!ov[0]) {
-            cap_ids_ = cap;
-            cap_runs_ = rcap;
-            saved.release();
-            return true;
-        }
-        if (ov[0] & 2u) snap_cap_ *= 2;
-        if ((ov[0] & 1u) && level < 5) ++level;
-        else if (ov[0] & 1u) break;
+        if (!ov[0]) return true;
+        if (ov[0] & 1u) break;
+        if (rec_cap_ >= 0xFFFF0000ull) break;
+        rec_cap_ = std::max<uint64_t>(rec_cap_ * 2, (uint64_t)ov[1] + (uint64_t)global_rows * 16384);
     }
-    saved.release();
-    err = "a tile holds more layers than the painter's LDS capacity";
+    err = (rec_cap_ >= 0xFFFF0000ull) ? "too many tile id records" : "a tile holds more layers than the row walk's capacity";
     return false;
 }
 
