@@ -26,17 +26,28 @@ METRIC = "frames/s & pixel-segments/s batched vs plain on 1 MI355X, bit-exact; p
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
 
 
-def stage_bytes(stats, width, height, n_layers):
-    """Algorithmic HBM bytes per frame and stage.
-    emit: read each line once (16 B points + 2 B id), write each kept segment;
-    sort: read + write every key once per digit pass;
-    paint: read the sorted stream, write the RGBA8 frame, read the props."""
-    L, S, P = stats.lines, stats.segments_painted, stats.sort_passes
+# kernel -> stats field with its per-frame time (hipEvents on the library's
+# stream, bracketing only that kernel's launches)
+KERNELS = {"k_emit": "emit_ms", "k_sort_pass": "sort_ms", "k_tile_runs": "runs_ms",
+           "k_row_walk": "walk_ms", "k_paint": "paint_ms"}
+
+
+def kernel_bytes(st, width, height, n_layers):
+    """(launches per frame, algorithmic HBM bytes per launch) per kernel.
+    L lines, S kept pixel segments, P digit passes, T tiles, R tile runs:
+      k_emit       18 L + 8 S   read each line (16 B points + 2 B id), write each segment
+      k_sort_pass  16 S         read + write every key once per digit pass
+      k_tile_runs  8 S + 8 T    read the sorted stream and the tile ranges
+      k_row_walk   8 R          the cover carry: 8 row covers per run
+      k_paint      8 S + 8 T + 4 W H + 32 L   accumulate (stream, tile ranges) and
+                                composite (RGBA8 frame, layer styles)"""
+    L, S, P, T, R = st.lines, st.segments_painted, st.sort_passes, st.tiles, st.runs
     return {
-        "setup": 18 * L + 8 * L,
-        "emit": 18 * L + 8 * S,
-        "sort": 16 * S * P,
-        "paint": 8 * S + 4 * width * height + 32 * n_layers,
+        "k_emit": (1, 18 * L + 8 * S),
+        "k_sort_pass": (P, 16 * S),
+        "k_tile_runs": (1, 8 * S + 8 * T),
+        "k_row_walk": (1, 8 * R),
+        "k_paint": (1, 8 * S + 8 * T + 4 * width * height + 32 * n_layers),
     }
 
 
@@ -61,6 +72,32 @@ def cpu_baseline(scene, fraction, threads):
                 seconds=dt, frames_per_s=1.0 / dt)
 
 
+def measured_traffic(workload):
+    """HBM bytes per launch per kernel from the committed rocprofv3 PMC
+    summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
+    the counter's reading for a 1 GiB device copy), or {} if there is none."""
+    path = os.path.join(ROOT, "profiles", f"hbm-traffic_{workload}.md")
+    if not os.path.exists(path):
+        return {}
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return json.loads(line.split(":", 1)[1])
+    return {}
+
+
+def copy_calibration(nbytes):
+    """One fill and one device-to-device copy of `nbytes`: the reference
+    kernels whose FETCH_SIZE / WRITE_SIZE counters scale the renderer's."""
+    import torch
+    a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(1)
+    torch.cuda.synchronize()
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": nbytes, "fill_bytes": nbytes}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,10 +105,15 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["polygons", "circles"], default="polygons")
     ap.add_argument("--paths", type=int, default=200_000)
-    ap.add_argument("--cpu-fraction", type=float, default=0.02)
+    ap.add_argument("--cpu-fraction", type=float, default=0.25)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-bytes", type=int, default=0,
+                    help="only copy a device buffer of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
+    if args.copy_bytes:
+        copy_calibration(args.copy_bytes)
+        return
     if args.gpus != 1:
         print("bench.py: this build renders on one GPU only; multi-GPU runs (tile-row bands "
               "exchanged with one all-to-all) are not implemented", file=sys.stderr)
@@ -85,9 +127,11 @@ def main():
         scene = workloads.polygons(count=args.paths)
         workload = (f"single large frame, plain GPU path: {scene.width}x{scene.height}, "
                     f"{args.paths} star/polygon paths, 5-50 edges, affine, mixed fill rules and blend modes")
+        profile_name = "polygons-3840x2160" if args.paths == 200_000 else None
     else:
         scene = workloads.circles()
         workload = "plain method: 1024x1024, 10,000 circles x 64 lines, opaque, NonZero, Over"
+        profile_name = None
     t_build = time.perf_counter()
     comp = scene.build(cassia_amd)
     t_build = time.perf_counter() - t_build
@@ -96,7 +140,7 @@ def main():
     for _ in range(args.warmup):
         comp.render_device(W, H, scene.background)
     lat = []
-    stage = {"setup": 0.0, "emit": 0.0, "sort": 0.0, "paint": 0.0}
+    kms = {k: 0.0 for k in KERNELS}
     dev_ms = 0.0
     t0 = time.perf_counter()
     for _ in range(args.steps):
@@ -104,24 +148,30 @@ def main():
         comp.render_device(W, H, scene.background)
         lat.append((time.perf_counter() - a) * 1e3)
         st = comp.stats()
-        stage["setup"] += st.setup_ms
-        stage["emit"] += st.emit_ms
-        stage["sort"] += st.sort_ms
-        stage["paint"] += st.paint_ms
+        for k, f in KERNELS.items():
+            kms[k] += getattr(st, f)
         dev_ms += st.device_ms
     elapsed = time.perf_counter() - t0
     st = comp.stats()
     K = args.steps
-    stage = {k: v / K for k, v in stage.items()}
+    kms = {k: v / K for k, v in kms.items()}
     ms_per_step = elapsed * 1e3 / K
     fps = K / elapsed
     segs = st.segments
-    nbytes = stage_bytes(st, W, H, scene.n_layers)
-    dominant = max(stage, key=stage.get)
-    achieved = nbytes[dominant] / (stage[dominant] * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": dominant,
-                "algorithmic_bytes": nbytes[dominant]}
+    kb = kernel_bytes(st, W, H, scene.n_layers)
+    kernels = {}
+    for k, ms in kms.items():
+        n, b = kb[k]
+        per_launch_ms = ms / n if n else 0.0
+        kernels[k] = {"ms_per_frame": round(ms, 4), "launches": n, "ms_per_launch": round(per_launch_ms, 4),
+                      "bytes_per_launch": b,
+                      "gbs": round(b / (per_launch_ms * 1e-3) / 1e9, 2) if per_launch_ms > 0 else None}
+    dominant = max(kms, key=kms.get)
+    d = kernels[dominant]
+    traffic = measured_traffic(profile_name).get(dominant) if profile_name else None
+    roofline = {"bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
+                "ms_per_launch": d["ms_per_launch"], "bytes_per_launch": d["bytes_per_launch"]}
     out = {
         "metric": METRIC,
         "value": segs * fps,
@@ -146,8 +196,11 @@ def main():
         "line_pixels": int(st.line_pixels),
         "sort_passes": int(st.sort_passes),
         "radix_bits": int(st.radix_bits),
-        "stage_ms": {k: round(v, 4) for k, v in stage.items()},
-        "stage_gbs": {k: round(nbytes[k] / (stage[k] * 1e-3) / 1e9, 2) if stage[k] > 0 else None for k in stage},
+        "tiles": int(st.tiles),
+        "tile_runs": int(st.runs),
+        "tile_records": int(st.records),
+        "setup_hist_ms": round(st.setup_ms + st.hist_ms, 4),
+        "kernels": kernels,
         "roofline": roofline,
         "scene_build_s": round(t_build, 2),
     }

@@ -67,7 +67,9 @@ class FrameStats(C.Structure):
     _fields_ = [("lines", C.c_uint64), ("line_pixels", C.c_uint64), ("segments", C.c_uint64),
                 ("segments_painted", C.c_uint64), ("tiles", C.c_uint64), ("sort_passes", C.c_uint32),
                 ("radix_bits", C.c_uint32), ("device_ms", C.c_double), ("setup_ms", C.c_double),
-                ("emit_ms", C.c_double), ("sort_ms", C.c_double), ("paint_ms", C.c_double)]
+                ("emit_ms", C.c_double), ("hist_ms", C.c_double), ("sort_ms", C.c_double),
+                ("runs_ms", C.c_double), ("walk_ms", C.c_double), ("paint_ms", C.c_double),
+                ("runs", C.c_uint64), ("records", C.c_uint64)]
 
 
 class CassiaStyling(C.Structure):
@@ -117,6 +119,8 @@ _sig("mg_buffer_layer_cache_clear", C.c_int, _vp, C.c_int32)
 _sig("mg_buffer_layer_cache_release", C.c_int, _vp, C.c_int32)
 _sig("mg_render", C.c_int, _vp, C.POINTER(_Buffer), _f, C.POINTER(_Rect))
 _sig("mg_render_device", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(_vp))
+_sig("mg_render_coverage", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(C.c_int32),
+     C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_size_t))
 _sig("mg_set_tuning", C.c_int, _vp, C.c_uint32)
 _sig("mg_pixel_segments", C.c_int, _vp, C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t))
 _sig("mg_last_frame_stats", C.c_int, _vp, C.POINTER(FrameStats))
@@ -560,6 +564,22 @@ class Composition:
         _check(_lib.mg_render_device(self._h, width, height, _fp(bg), C.byref(r) if r is not None else None,
                                      C.byref(out)))
         return out.value
+
+    def render_coverage(self, width, height, background_color, crop: Optional[Rect] = None, cap=1 << 16):
+        """Per-pixel coverage values A of every (tile, workbench id) of a render:
+        returns (A int32 (n, 64) column-major x*8+y, keys uint32 (n, 4) =
+        (scene, tile row, tile column, layer)), in no particular order."""
+        bg = _f32(background_color, 4)
+        r = _Rect(*crop.px[:2], *crop.px[2:]) if crop is not None else None
+        a = np.zeros((cap, 64), np.int32)
+        k = np.zeros((cap, 4), np.uint32)
+        n = C.c_size_t(0)
+        _check(_lib.mg_render_coverage(self._h, width, height, _fp(bg), C.byref(r) if r is not None else None,
+                                       a.ctypes.data_as(C.POINTER(C.c_int32)),
+                                       k.ctypes.data_as(C.POINTER(C.c_uint32)), cap, C.byref(n)))
+        if n.value > cap:
+            return self.render_coverage(width, height, background_color, crop, cap=int(n.value))
+        return a[:n.value], k[:n.value]
 
     def set_tuning(self, radix_bits=0):
         """Result-invariant knob: the sort's digit width in bits (4, 6, 8; 0 = automatic)."""

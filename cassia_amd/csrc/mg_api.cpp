@@ -497,7 +497,7 @@ MG_EXPORT mg_status mg_buffer_layer_cache_release(mg_composition *c, int32_t id)
 // `stride` into the device image `img` (composition.rs:212-328).
 static mg_status render_frame(mg_composition *c, size_t width, size_t stride, size_t rows_px, const float bg[4],
                               const mg_rect *crop, int32_t cache_id, int32_t *prev, uint8_t *img,
-                              bool want_tile_ops, mg::FrameResult &r) {
+                              bool want_tile_ops, mg::FrameResult &r, uint32_t coverage_cap = 0) {
     std::string err;
     const size_t row_len = (width + 7) / 8, tile_rows = (rows_px + 7) / 8;
     c->remove_disabled();
@@ -541,6 +541,8 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.scene_line_base.push_back(0);
     f.mode = 0;
     f.want_tile_ops = want_tile_ops;
+    f.want_coverage = coverage_cap > 0;
+    f.coverage_cap = coverage_cap;
     if (c->radix_bits) f.radix_bits = c->radix_bits;
     mg_status s;
     if ((s = c->sync_lines())) return s;
@@ -566,8 +568,13 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     c->stats.device_ms = r.device_ms;
     c->stats.setup_ms = r.stage_ms[0];
     c->stats.emit_ms = r.stage_ms[1];
-    c->stats.sort_ms = r.stage_ms[2];
-    c->stats.paint_ms = r.stage_ms[3];
+    c->stats.hist_ms = r.stage_ms[2];
+    c->stats.sort_ms = r.stage_ms[3];
+    c->stats.runs_ms = r.stage_ms[4];
+    c->stats.walk_ms = r.stage_ms[5];
+    c->stats.paint_ms = r.stage_ms[6];
+    c->stats.runs = r.runs;
+    c->stats.records = r.records;
     return MG_OK;
 }
 
@@ -642,6 +649,35 @@ MG_EXPORT mg_status mg_render_device(mg_composition *c, uint32_t width, uint32_t
     mg::FrameResult r;
     if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, false, r))) return s;
     *device_rgba = img;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_render_coverage(mg_composition *c, uint32_t width, uint32_t height, const float bg[4],
+                                       const mg_rect *crop, int32_t *a_out, uint32_t *key_out, size_t cap,
+                                       size_t *n) {
+    if (check_comp(c) || !bg || !n || width == 0 || height == 0 || (cap && (!a_out || !key_out)))
+        return fail(MG_ERR_INVALID_ARG, "bad argument");
+    if (cap > 0xFFFFFFFFull) return fail(MG_ERR_INVALID_ARG, "cap too large");
+    mg_status s = c->ensure_renderer();
+    if (s) return s;
+    std::string err;
+    const size_t bytes = (size_t)width * height * 4;
+    uint8_t *img = c->rdr->frame_image(bytes, err);
+    if (!img) return fail(MG_ERR_OOM, err);
+    mg::FrameResult r;
+    if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, false, r,
+                          (uint32_t)std::max<size_t>(cap, 1))))
+        return s;
+    std::vector<int32_t> a;
+    std::vector<uint32_t> key;
+    uint32_t count = 0;
+    if (!c->rdr->download_coverage(a, key, count, err)) return fail(MG_ERR_DEVICE, err);
+    const size_t m = std::min<size_t>({(size_t)count, cap, key.size() / 4});
+    if (m) {
+        std::memcpy(a_out, a.data(), m * 64 * sizeof(int32_t));
+        std::memcpy(key_out, key.data(), m * 4 * sizeof(uint32_t));
+    }
+    *n = count;
     return MG_OK;
 }
 

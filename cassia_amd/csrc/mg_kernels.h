@@ -42,6 +42,7 @@ struct PaintParams {
     uint2 *rec_hdr;
     uint32_t *rec_used;
     uint64_t rec_cap;
+    uint32_t *n_runs, *n_records;  // statistics: runs and records of the frame
     const uint2 *work;           // (global row, tile column) per paint work group
 };
 

@@ -899,7 +899,10 @@ __global__ __launch_bounds__(256) void k_tile_runs(PaintParams pp) {
     }
     if (lane == 0) {
         pp.tile_nr[(uint64_t)grow * pp.tstride + colp] = nr;
-        if (nr) pp.run_cov[ts + nr - 1] = open;
+        if (nr) {
+            pp.run_cov[ts + nr - 1] = open;
+            atomicAdd(pp.n_runs, nr);
+        }
     }
 }
 
@@ -951,6 +954,7 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
     __syncthreads();
     uint32_t nq = 0, cur = 0;
     uint32_t rec_next = 0, rec_end = 0;
+    bool counting = false;  // record store full: only count what the frame needs
     for (uint32_t colp = 0; colp <= h1; ++colp) {
         const bool fold = colp <= h0;
         const uint32_t nr = nr_l[colp];
@@ -1009,7 +1013,9 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
         const uint32_t ex = block_excl_scan_256(cu | (cq << 16), s_wsum, &all);
         const uint32_t n_union = all & 0xFFFFu, nq_new = all >> 16;
         uint32_t rec_off = 0;
-        if (!fold) {
+        if (!fold && counting) {
+            if (tid == 0) atomicAdd(pp.rec_used, n_union);
+        } else if (!fold) {
             if (rec_next + n_union > rec_end) {
                 if (tid == 0) {
                     const uint32_t sz = n_union > kRecChunk ? n_union : kRecChunk;
@@ -1017,24 +1023,23 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
                     s_rec[0] = base;
                     s_rec[1] = base + sz;
                     s_rec[2] = (uint64_t)base + sz > pp.rec_cap ? 1u : 0u;
+                    if (s_rec[2]) atomicOr(pp.overflow, 2u);
                 }
                 __syncthreads();
-                if (s_rec[2]) {
-                    if (tid == 0) atomicOr(pp.overflow, 2u);
-                    return;
-                }
+                counting = s_rec[2] != 0;
                 rec_next = s_rec[0];
                 rec_end = s_rec[1];
             }
             rec_off = rec_next;
             rec_next += n_union;
         }
+        const bool write_rec = !fold && !counting;
         const uint32_t nxt = cur ^ 1u;
         uint32_t ou = rec_off + (ex & 0xFFFFu), oq = ex >> 16;
         walk([&](uint32_t lay, int32_t qi, int32_t ri) {
             bool keep;
             const uint64_t v = next_cover(qi, ri, &keep);
-            if (!fold) {
+            if (write_rec) {
                 uint32_t key = lay, beg = 0, len = 0;
                 uint64_t cv = 0;
                 if (ri >= 0) {
@@ -1056,9 +1061,10 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
                 ++oq;
             }
         });
-        if (!fold && tid == 0) {
+        if (write_rec && tid == 0) {
             const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + (colp - 1);
             pp.rec_hdr[tile] = make_uint2(rec_off, n_union);
+            atomicAdd(pp.n_records, n_union);
         }
         __syncthreads();
         nq = nq_new;

@@ -46,7 +46,9 @@ struct FrameResult {
     uint32_t passes = 0, radix_bits = 0;
     uint64_t tiles = 0;
     double device_ms = 0.0;
-    double stage_ms[4] = {0, 0, 0, 0};  // setup+scan, emit, sort, paint
+    // setup+scan, emit, histogram+bases, sort passes, tile runs, row walk, paint
+    double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    uint64_t runs = 0, records = 0;     // tile runs and tile id records of the painted frame
     const uint64_t *sorted = nullptr;   // device pointer to the sorted words (mode 1)
 };
 
@@ -87,7 +89,7 @@ private:
     int cus_ = 0;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
-    hipEvent_t evs_[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t evs_[8] = {};
     uint64_t uploaded_version_ = 0;
     std::vector<SceneDesc> uploaded_scenes_;
     DevBuf pts_, ids_;
@@ -99,6 +101,7 @@ private:
     DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
+    uint64_t n_records_ = 0, n_runs_ = 0;
     std::vector<DevBuf> caches_;
     std::vector<size_t> cache_tiles_;
     const uint64_t *last_sorted_ = nullptr;

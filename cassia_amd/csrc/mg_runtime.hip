@@ -312,7 +312,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
     if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) ||
         !emit_status_.reserve(std::max<uint64_t>(nparts_emit, 1) * 8) || !hist_.reserve(16 * radix * 4) ||
-        !bases_.reserve(16 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4) ||
+        !bases_.reserve(16 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4 * dims.passes) ||
         !row_count_.reserve(((size_t)global_rows + 1) * 4) || !row_start_.reserve(((size_t)global_rows + 1) * 4) ||
         !row_status_.reserve(((global_rows + 1 + kScanTileHost - 1) / kScanTileHost) * 8)) {
         err = "out of device memory (segments)";
@@ -331,30 +331,37 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
                                                          scenes_.as<SceneDesc>(), dims, ctr, segs_a_.as<uint64_t>(),
                                                          cap_segs, emit_status_.as<uint64_t>(), pc + 1);
     }
+    MG_HIP(hipGetLastError(), "emit launch");
+    MG_HIP(hipEventRecord(evs_[2], stream_), "event");
     k_histogram<<<(uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((cap_segs + 2047) / 2048, (uint64_t)cus_ * 4)),
                   256, 0, stream_>>>(segs_a_.as<uint64_t>(), ctr, dims, hist_.as<uint32_t>());
     k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
-    MG_HIP(hipGetLastError(), "emit launch");
-    MG_HIP(hipEventRecord(evs_[2], stream_), "event");
+    // look-back status words of every digit pass, cleared outside the sort's timing
+    fill32(sort_status_.as<uint32_t>(), 0, nparts_sort * radix * dims.passes, stream_);
+    MG_HIP(hipGetLastError(), "histogram launch");
+    MG_HIP(hipEventRecord(evs_[3], stream_), "event");
     uint64_t *in = segs_a_.as<uint64_t>(), *out = segs_b_.as<uint64_t>();
     const uint32_t sort_grid = (uint32_t)std::max<uint64_t>(nparts_sort, 1);
     for (uint32_t p = 0; p < dims.passes; ++p) {
-        fill32(sort_status_.as<uint32_t>(), 0, nparts_sort * radix, stream_);
         const uint32_t shift = kPayloadBits + p * R;
         uint32_t *counter = pc + 8 + p;
+        uint32_t *status = sort_status_.as<uint32_t>() + (size_t)p * nparts_sort * radix;
         if (R == 8)
             k_sort_pass<8><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            sort_status_.as<uint32_t>(), counter);
+                                                            status, counter);
         else if (R == 6)
             k_sort_pass<6><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            sort_status_.as<uint32_t>(), counter);
+                                                            status, counter);
         else
             k_sort_pass<4><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            sort_status_.as<uint32_t>(), counter);
+                                                            status, counter);
         std::swap(in, out);
     }
     MG_HIP(hipGetLastError(), "sort launch");
-    MG_HIP(hipEventRecord(evs_[3], stream_), "event");
+    MG_HIP(hipEventRecord(evs_[4], stream_), "event");
+    n_runs_ = n_records_ = 0;
+    // paint re-records these at its kernel boundaries
+    for (int k = 5; k < 8; ++k) MG_HIP(hipEventRecord(evs_[k], stream_), "event");
     last_sorted_ = in;
     res.sorted = in;
     if (spec.mode != 1) {
@@ -377,13 +384,12 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
         if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, cap_segs, image, prev_layers, tiles, err))
             return false;
     }
-    MG_HIP(hipEventRecord(evs_[4], stream_), "event");
     MG_HIP(hipEventRecord(ev1_, stream_), "event");
     MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
     MG_HIP(hipStreamSynchronize(stream_), "frame");
     float ms = 0.0f;
     MG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
-    for (int k = 0; k < 4; ++k) {
+    for (int k = 0; k < 7; ++k) {
         float sm = 0.0f;
         MG_HIP(hipEventElapsedTime(&sm, evs_[k], evs_[k + 1]), "elapsed");
         res.stage_ms[k] = sm;
@@ -391,6 +397,8 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     res.segments = hc.segments;
     res.kept = hc.kept;
     res.device_ms = ms;
+    res.runs = n_runs_;
+    res.records = n_records_;
     return true;
 }
 
@@ -452,9 +460,13 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.cov_count = cov_count_.as<uint32_t>();
         pp.cov_cap = spec.coverage_cap;
     }
+    pp.n_runs = overflow_.as<uint32_t>() + 3;
+    pp.n_records = overflow_.as<uint32_t>() + 2;
+    fill32(overflow_.as<uint32_t>() + 3, 0, 1, stream_);
     k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows);
     k_tile_runs<<<dim3(global_rows, (tstride - 1 + 3) / 4), 256, 0, stream_>>>(pp);
     MG_HIP(hipGetLastError(), "tile runs launch");
+    MG_HIP(hipEventRecord(evs_[5], stream_), "event");
 
     // Row-walk LDS: the whole work group's share (one work group per CU is
     // enough for the tile rows); runs of one tile up to rcap.
@@ -488,16 +500,25 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.rec_len = rec_len_.as<uint32_t>();
         pp.rec_cap = rec_cap_;
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
-        fill32(overflow_.as<uint32_t>(), 0, 2, stream_);
+        fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
+        MG_HIP(hipEventRecord(evs_[5], stream_), "event");
         k_row_walk<<<global_rows, 256, lds_w, stream_>>>(pp);
+        MG_HIP(hipEventRecord(evs_[6], stream_), "event");
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
+        MG_HIP(hipEventRecord(evs_[7], stream_), "event");
         MG_HIP(hipGetLastError(), "paint launch");
-        uint32_t ov[2] = {0, 0};
-        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 8, hipMemcpyDeviceToHost, stream_), "read overflow");
+        uint32_t ov[4] = {0, 0, 0, 0};
+        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 16, hipMemcpyDeviceToHost, stream_), "read overflow");
         MG_HIP(hipStreamSynchronize(stream_), "paint");
-        if (!ov[0]) return true;
+        if (!ov[0]) {
+            n_records_ = ov[2];
+            n_runs_ = ov[3];
+            return true;
+        }
         if (ov[0] & 1u) break;
         if (rec_cap_ >= 0xFFFF0000ull) break;
+        // the walk kept counting after the store filled up: ov[1] covers the
+        // frame's records plus at most one partly used chunk per row
         rec_cap_ = std::max<uint64_t>(rec_cap_ * 2, (uint64_t)ov[1] + (uint64_t)global_rows * 16384);
     }
     err = (rec_cap_ >= 0xFFFF0000ull) ? "too many tile id records" : "a tile holds more layers than the row walk's capacity";

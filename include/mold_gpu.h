@@ -206,6 +206,22 @@ MG_EXPORT mg_status mg_render_device(mg_composition *comp, uint32_t width, uint3
    NONE tail included). The pointer stays valid until the next call on `comp`. */
 MG_EXPORT mg_status mg_pixel_segments(mg_composition *comp, const uint64_t **out, size_t *n);
 
+/* Per-pixel coverage of a render (the painter's integer accumulation,
+   painter/mod.rs:229-243, 347-362): renders like mg_render_device and, for
+   every tile inside the crop and every id of its workbench
+   (populate_layers, layer_workbench.rs:401-428, before any workbench pass),
+   records the 64 values A = 16 * P + area, where P is the wrapping i8 cover
+   prefix of the pixel's row and area the wrapping i16 area sum of the pixel.
+   Record r: a_out[64 r + (x * 8 + y)] for pixel (x, y) of the tile and
+   key_out[4 r ..] = (scene 0, tile row, tile column, layer order). Records
+   come in no particular order; at most `cap` are written and *n receives the
+   total. Coverage for NonZero / EvenOdd follows from A (painter/mod.rs:
+   53-71). No counterpart in the reference's API: the reference keeps these
+   values inside the painter; this entry point exists to check them. */
+MG_EXPORT mg_status mg_render_coverage(mg_composition *comp, uint32_t width, uint32_t height,
+                                       const float background[4], const mg_rect *crop, int32_t *a_out,
+                                       uint32_t *key_out, size_t cap, size_t *n);
+
 /* Result-invariant tuning knob of the renderer (0 = automatic): the sort's
    digit width in bits (4, 6 or 8). Every setting produces the same bytes;
    only the speed changes. */
@@ -217,9 +233,15 @@ typedef struct mg_frame_stats {
     uint64_t lines, line_pixels, segments, segments_painted, tiles;
     uint32_t sort_passes, radix_bits;
     double device_ms;          /* GPU time of the frame (hipEvents)      */
-    /* per stage (hipEvents on the library's stream): line setup + scan,
-       emission, sort (all digit passes), paint */
-    double setup_ms, emit_ms, sort_ms, paint_ms;
+    /* per stage (hipEvents on the library's stream) */
+    double setup_ms;           /* line setup + line-pixel scan           */
+    double emit_ms;            /* pixel-segment emission                 */
+    double hist_ms;            /* digit histograms + digit bases         */
+    double sort_ms;            /* all LSD digit passes                   */
+    double runs_ms;            /* row/tile ranges + per-tile layer runs  */
+    double walk_ms;            /* row-serial cover carry (id records)    */
+    double paint_ms;           /* tile paint: workbench + painter        */
+    uint64_t runs, records;    /* tile layer runs, tile id records       */
 } mg_frame_stats;
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
 

@@ -525,6 +525,24 @@ class Composition:
             raise RuntimeError("oracle render failed")
         return nseg.value
 
+    def render_coverage(self, width, height, background_color, crop: Optional[Rect] = None, cap=1 << 16):
+        """The painter's per-pixel A values of every (tile, workbench id) of a
+        render into a fresh width x height buffer (painter/mod.rs:229-243,
+        347-362): (A int32 (n, 64) column-major x*8+y, keys uint32 (n, 3) =
+        (tile row, tile column, layer))."""
+        while True:
+            a = np.zeros((cap, 64), np.int32)
+            k = np.zeros((cap, 3), np.uint32)
+            dbg = Debug()
+            dbg.cov_a = a.ctypes.data_as(C.POINTER(C.c_int32))
+            dbg.cov_key = k.ctypes.data_as(C.POINTER(C.c_uint32))
+            dbg.cap_records = cap
+            buf = np.zeros((width * height, 4), np.uint8)
+            self.render(Buffer(buf, width), background_color, crop, debug=dbg)
+            if dbg.n_records <= cap:
+                return a[:dbg.n_records], k[:dbg.n_records]
+            cap = int(dbg.n_records)
+
     def render(self, buffer: Buffer, background_color, crop: Optional[Rect] = None, debug=None):
         """composition.rs:212-328"""
         cache = buffer.layer_cache

@@ -157,3 +157,32 @@ def test_circles_fixture_via_cassia(gpu, orc):
     assert orc.lib().or_print(orc.ptr(ref.reshape(-1), orc._u8), W * H, W, W, None, 0, orc.ptr(segs, orc._u64),
                               len(segs), orc.ptr(clr, orc._f), None, pt, 10, None) == 0
     assert_same_image(ref, img, W)
+
+
+def sorted_coverage(a, k, gpu_keys):
+    """(key, A row) pairs in key order; GPU keys drop the scene index."""
+    keys = [tuple(r[1:]) if gpu_keys else tuple(r) for r in k.tolist()]
+    return sorted(zip(keys, a.tolist()))
+
+
+@pytest.mark.parametrize("case", ["plain-0", "plain-1", "curves", "deep", "crop"])
+def test_per_pixel_coverage_bit_exact(gpu, cpu, case):
+    """A = 16 * P + area of every pixel of every (tile, workbench id), against
+    the oracle's coverage dump."""
+    w, h, n, kw, crop = 83, 61, 30, dict(transforms=True), None
+    seed = {"plain-0": 700, "plain-1": 701, "curves": 702, "deep": 703, "crop": 704}[case]
+    if case == "curves":
+        kw["curves"] = True
+    if case == "deep":
+        w, h, n = 48, 40, 300
+    outs = []
+    for m, is_gpu in ((cpu, False), (gpu, True)):
+        comp = scenegen.build_scene(m, seed, w, h, n, **kw)
+        c = m.Rect(range(9, 70), range(5, 41)) if case == "crop" else None
+        a, k = comp.render_coverage(w, h, [1.0, 1.0, 1.0, 1.0], c)
+        outs.append(sorted_coverage(a, k, is_gpu))
+    ref, got = outs
+    assert len(ref) == len(got) > 0
+    for (kr, ar), (kg, ag) in zip(ref, got):
+        assert kr == kg, (kr, kg)
+        assert ar == ag, f"tile/layer {kr}: A differs"
