@@ -204,6 +204,15 @@ def main():
         "roofline": roofline,
         "scene_build_s": round(t_build, 2),
     }
+    # PCIe-inclusive rate: the frame rendered into a caller-owned host buffer
+    # (upload of the caller's pixels, render, download), as mg_render does
+    hb = np.zeros((W * H, 4), np.uint8)
+    host_ms = []
+    for _ in range(3):
+        a = time.perf_counter()
+        comp.render(cassia_amd.Buffer(hb, W), scene.background, None)
+        host_ms.append((time.perf_counter() - a) * 1e3)
+    out["host_buffer_ms_per_frame"] = round(float(np.median(host_ms)), 3)
     if not args.no_cpu:
         cb = cpu_baseline(scene, args.cpu_fraction, args.cpu_threads)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
