@@ -52,21 +52,21 @@ __global__ void k_setup(const float4 *pts, const uint16_t *ids, const LayerEntry
                         const SceneDesc *scenes, const uint32_t *scene_line_base, FrameDims dims,
                         LineSetup *out, uint32_t *lengths, uint32_t *err);
 __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *status,
-                           uint32_t *part_counter, uint64_t *total);
+                           uint32_t *stall, uint64_t *total);
 __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
-                       FrameDims dims, FrameCounters *counters, uint64_t *out, uint64_t capacity,
-                       uint64_t *status, uint32_t *part_counter);
-__global__ void k_histogram(const uint64_t *words, const FrameCounters *counters, FrameDims dims,
-                            uint32_t *hist);
+                       FrameDims dims, const FrameCounters *counters, uint64_t *out, const uint32_t *part_line);
+__global__ void k_part_lines(const uint32_t *offsets, const uint32_t *lengths, uint64_t n_lines,
+                             uint32_t part_size, uint32_t *part_line);
+__global__ void k_histogram(const uint64_t *words, FrameCounters *counters, FrameDims dims, uint32_t *hist);
 __global__ void k_row_starts(const uint64_t *sorted, const FrameCounters *counters, uint32_t rows,
                              uint32_t row_shift, uint32_t *row_start);
 __global__ void k_ingest(const uint64_t *in, uint64_t n_in, const SceneDesc *scenes, FrameDims dims,
-                         FrameCounters *counters, uint64_t *out, uint64_t *status, uint32_t *part_counter);
+                         const FrameCounters *counters, uint64_t *out);
 __global__ void k_digit_bases(const uint32_t *hist, uint32_t *bases, uint32_t radix);
 template <int R>
 __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
                             uint32_t shift, const uint32_t *digit_base, uint32_t *status,
-                            uint32_t *part_counter);
+                            uint32_t *stall, uint32_t from_slots);
 __global__ void k_tile_starts(PaintParams pp, uint32_t global_rows);
 __global__ void k_tile_runs(PaintParams pp);
 __global__ void k_row_walk(PaintParams pp);
@@ -74,7 +74,7 @@ __global__ void k_paint(PaintParams pp);
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride);
 size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 
-constexpr uint32_t kEmitTileHost = 1024;
+constexpr uint32_t kEmitTileHost = 2048;
 constexpr uint32_t kSortTileHost = 4096;
 constexpr uint32_t kScanTileHost = 2048;
 

@@ -48,30 +48,53 @@ constexpr uint64_t kStAgg = 1ull << 62;
 constexpr uint64_t kStInc = 2ull << 62;
 constexpr uint64_t kStVal = (1ull << 62) - 1;
 
+// Partitions are work-group indices. Work groups are dispatched in index
+// order (per XCD, and XCDs receive them in a fixed rotation), so the
+// predecessors of a running partition are running or finished, and every
+// look-back wait ends. The waits are still bounded: after kSpinLimit units of
+// 64-cycle sleep (about 7 ms) a look-back gives up, raises *stall and finishes
+// with a partial prefix (every write stays inside its buffer; the host
+// reports the frame as failed).
+constexpr uint32_t kSpinLimit = 1u << 18;
+
+// Sleeps 64 * n cycles for n in 1..32 (s_sleep takes an immediate).
+__device__ __forceinline__ void backoff(uint32_t n) {
+    for (uint32_t i = 0; i < n; ++i) __builtin_amdgcn_s_sleep(1);
+}
+
 // Exclusive prefix of the partitions before `part`; executed by one full
-// wave. Predecessors are always running or finished because partitions are
-// handed out in order by an atomic counter.
-__device__ uint64_t lookback_wave(const uint64_t *status, uint32_t part) {
+// wave. Each round reads 64 predecessors, consumes the ready ones nearest to
+// `part` (up to the first inclusive prefix) and waits, with exponential
+// backoff, only on the first one not yet published.
+__device__ uint64_t lookback_wave(const uint64_t *status, uint32_t part, uint32_t *stall) {
     const uint32_t lane = lane_id();
     uint64_t excl = 0;
     int64_t top = (int64_t)part - 1;
+    uint32_t spins = 0, nap = 1;
     while (top >= 0) {
-        int64_t idx = top - (int64_t)lane;
-        uint64_t s = idx >= 0 ? ld_acquire(status + idx) : kStInc;
-        uint64_t flag = s >> 62;
-        uint64_t not_ready = __ballot(flag == 0);
-        uint64_t inc = __ballot(flag == 2);
-        uint64_t first_inc = inc ? (uint64_t)__ffsll((long long)inc) - 1 : 64;
-        uint64_t blocked = not_ready & ((first_inc >= 64) ? ~0ull : ((1ull << first_inc) - 1));
-        if (blocked) {
-            __builtin_amdgcn_s_sleep(1);
-            continue;
-        }
-        uint64_t v = (lane <= first_inc && idx >= 0) ? (s & kStVal) : 0;
+        const int64_t idx = top - (int64_t)lane;
+        const uint64_t s = idx >= 0 ? ld_acquire(status + idx) : kStInc;
+        const uint64_t flag = s >> 62;
+        const uint64_t not_ready = __ballot(flag == 0);
+        const uint64_t inc = __ballot(flag == 2);
+        const uint32_t first_inc = inc ? (uint32_t)__ffsll((long long)inc) - 1 : 64u;
+        const uint32_t first_nr = not_ready ? (uint32_t)__ffsll((long long)not_ready) - 1 : 64u;
+        const uint32_t take = first_inc < first_nr ? first_inc + 1 : first_nr;
+        uint64_t v = (lane < take && idx >= 0) ? (s & kStVal) : 0;
         for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
         excl += v;
-        if (first_inc < 64) break;
-        top -= 64;
+        if (first_inc < first_nr) break;
+        top -= take;
+        if (take < 64) {
+            if ((spins += nap) > kSpinLimit) {
+                if (lane == 0) atomicOr(stall, 1u);
+                break;
+            }
+            backoff(nap);
+            nap = nap < 32 ? nap * 2 : 32;
+        } else {
+            nap = 1;
+        }
     }
     return excl;
 }
@@ -197,14 +220,12 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
 // Single-pass exclusive scan of u32 values (decoupled look-back), 2048
 // values per partition. out may alias in. total receives the sum (u64).
 __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n,
-                                                  uint64_t *status, uint32_t *part_counter,
+                                                  uint64_t *status, uint32_t *stall,
                                                   uint64_t *total) {
-    __shared__ uint32_t s_part, wsum[4];
+    __shared__ uint32_t wsum[4];
     __shared__ uint64_t s_excl;
     constexpr uint32_t IPT = 8;
-    if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
-    __syncthreads();
-    const uint32_t part = s_part;
+    const uint32_t part = blockIdx.x;
     const uint64_t base = (uint64_t)part * 256 * IPT + (uint64_t)threadIdx.x * IPT;
     uint32_t v[IPT], sum = 0;
     for (uint32_t k = 0; k < IPT; ++k) {
@@ -215,7 +236,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
     uint32_t ex = block_excl_scan_256(sum, wsum, &agg);
     if (threadIdx.x == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
     if (threadIdx.x < 64) {
-        uint64_t e = part ? lookback_wave(status, part) : 0;
+        uint64_t e = part ? lookback_wave(status, part, stall) : 0;
         if (threadIdx.x == 0) {
             s_excl = e;
             if (part) st_release(status + part, kStInc | (e + agg));
@@ -290,9 +311,12 @@ __device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, u
 }
 
 constexpr uint32_t kEmitThreads = 256;
-constexpr uint32_t kEmitIPT = 4;
+constexpr uint32_t kEmitIPT = 8;
 constexpr uint32_t kEmitTile = kEmitThreads * kEmitIPT;
 constexpr uint32_t kMaxPasses = 8;
+// Slot sentinels (bit 63 set; no word has it: render words use 57 bits and
+// emitted PixelSegments are never NONE).
+constexpr uint64_t kSlotNone = ~0ull, kSlotCull = ~1ull;
 constexpr uint32_t kMaxRadix = 2048;
 
 // Render-mode filter + render key of one PixelSegment. Returns false when
@@ -313,131 +337,117 @@ __device__ __forceinline__ bool render_word(uint64_t w, const SceneDesc &S, cons
     return true;
 }
 
-// Emission: one thread per kEmitIPT consecutive line-pixels (blocked), one
-// partition of kEmitTile line-pixels per work group, partitions claimed in
-// order. Output positions follow generation order (line, pixel, slot), so the
-// stable sort that follows yields the canonical order.
+// Emission: every line-pixel p owns the two slots 2p, 2p+1 of the word
+// buffer, the layout of the reference's rasterizer output ([seg, seg|NONE],
+// rasterizer/mod.rs:195, 272-279). A slot holds a word, kSlotNone (no second
+// segment) or kSlotCull (a render-mode segment that can never reach the
+// frame). Slot order is generation order (line, pixel, slot); the first sort
+// pass drops the sentinels while it scatters, so no compaction pass and no
+// cross-work-group prefix is needed here. One work group per kEmitTile
+// consecutive line-pixels, kEmitIPT consecutive line-pixels per thread.
 __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ lines,
                                               const uint32_t *__restrict__ offsets,
                                               const SceneDesc *__restrict__ scenes, FrameDims dims,
-                                              FrameCounters *__restrict__ counters, uint64_t *__restrict__ out,
-                                              uint64_t capacity, uint64_t *status, uint32_t *part_counter) {
-    __shared__ uint32_t s_part, wsum[4];
-    __shared__ uint64_t s_excl;
-    __shared__ uint32_t s_line[2];
+                                              const FrameCounters *__restrict__ counters,
+                                              uint64_t *__restrict__ out, const uint32_t *__restrict__ part_line) {
     __shared__ uint32_t s_off[kEmitTile + 1];
     const uint64_t P = counters->line_pixels;
     const uint32_t nparts = (uint32_t)((P + kEmitTile - 1) / kEmitTile);
-    const uint32_t lane = lane_id();
-    if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
-    __syncthreads();
-    const uint32_t part = s_part;
+    const uint32_t part = blockIdx.x;
     if (part >= nparts) return;
     const uint64_t b0 = (uint64_t)part * kEmitTile;
     const uint64_t b1 = b0 + kEmitTile < P ? b0 + kEmitTile : P;
-    // lines covering [b0, b1): last line with offset <= x
-    if (threadIdx.x < 2) {
-        uint64_t x = threadIdx.x == 0 ? b0 : b1 - 1;
-        uint32_t lo = 0, hi = dims.n_lines;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if ((uint64_t)offsets[mid] <= x) lo = mid; else hi = mid;
-        }
-        s_line[threadIdx.x] = lo;
-    }
-    __syncthreads();
-    const uint32_t l0 = s_line[0], l1 = s_line[1];
+    // lines covering [b0, b1): from the line holding the partition's first
+    // line-pixel to the one holding the next partition's (a superset)
+    const uint32_t l0 = part_line[part];
+    const uint32_t l1 = part + 1 < nparts ? part_line[part + 1] : dims.n_lines - 1;
     const bool in_lds = l1 - l0 + 1 <= kEmitTile;
     if (in_lds)
         for (uint32_t k = threadIdx.x; k <= l1 - l0; k += blockDim.x) s_off[k] = offsets[l0 + k];
     __syncthreads();
     const uint64_t t0 = b0 + (uint64_t)threadIdx.x * kEmitIPT;
-    uint64_t segs[2 * kEmitIPT];
-    uint32_t cnt = 0, produced = 0;
-    if (t0 < b1) {
-        uint32_t li;
-        if (in_lds) {
-            uint32_t lo = 0, hi = l1 - l0 + 1;
-            while (hi - lo > 1) {
-                uint32_t mid = (lo + hi) >> 1;
-                if ((uint64_t)s_off[mid] <= t0) lo = mid; else hi = mid;
-            }
-            li = l0 + lo;
-        } else {
-            uint32_t lo = l0, hi = l1 + 1;
-            while (hi - lo > 1) {
-                uint32_t mid = (lo + hi) >> 1;
-                if ((uint64_t)offsets[mid] <= t0) lo = mid; else hi = mid;
-            }
-            li = lo;
+    if (t0 >= b1) return;
+    uint32_t li;
+    if (in_lds) {
+        uint32_t lo = 0, hi = l1 - l0 + 1;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if ((uint64_t)s_off[mid] <= t0) lo = mid; else hi = mid;
         }
-        LineSetup L = lines[li];
-        uint64_t first = offsets[li];
-        uint64_t next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
-        while (next <= t0) {  // zero-length lines share offsets
+        li = l0 + lo;
+    } else {
+        uint32_t lo = l0, hi = l1 + 1;
+        while (hi - lo > 1) {
+            uint32_t mid = (lo + hi) >> 1;
+            if ((uint64_t)offsets[mid] <= t0) lo = mid; else hi = mid;
+        }
+        li = lo;
+    }
+    LineSetup L = lines[li];
+    uint64_t first = offsets[li];
+    uint64_t next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
+    while (next <= t0) {  // zero-length lines share offsets
+        ++li;
+        L = lines[li];
+        first = next;
+        next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
+    }
+    const SceneDesc *S = scenes + (L.packed >> 19);
+    uint4 *slots = reinterpret_cast<uint4 *>(out);
+    for (uint32_t k = 0; k < kEmitIPT; ++k) {
+        const uint64_t p = t0 + k;
+        if (p >= b1) break;
+        while (p >= next) {
             ++li;
             L = lines[li];
             first = next;
             next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
+            S = scenes + (L.packed >> 19);
         }
-        uint32_t sc = L.packed >> 19;
-        const SceneDesc *S = scenes + sc;
-        for (uint32_t k = 0; k < kEmitIPT; ++k) {
-            uint64_t p = t0 + k;
-            if (p >= b1) break;
-            while (p >= next) {
-                ++li;
-                L = lines[li];
-                first = next;
-                next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
-                S = scenes + (L.packed >> 19);
-            }
-            uint64_t w[2];
-            int n = rasterize_pixel(L, (int32_t)(p - first), w);
-            produced += (uint32_t)n;
-            for (int q = 0; q < n; ++q) {
-                if (dims.mode == 0) {
-                    uint64_t rw;
-                    uint32_t grow;
-                    if (render_word(w[q], *S, dims, &rw, &grow)) segs[cnt++] = rw;
-                } else {
-                    segs[cnt++] = w[q];
-                }
+        uint64_t w[2];
+        const int n = rasterize_pixel(L, (int32_t)(p - first), w);
+        uint64_t v[2] = {kSlotNone, kSlotNone};
+        for (int q = 0; q < n; ++q) {
+            if (dims.mode == 0) {
+                uint64_t rw;
+                uint32_t grow;
+                v[q] = render_word(w[q], *S, dims, &rw, &grow) ? rw : kSlotCull;
+            } else {
+                v[q] = w[q];
             }
         }
+        slots[p] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
     }
-    uint32_t agg;
-    uint32_t ex = block_excl_scan_256(cnt, wsum, &agg);
-    if (threadIdx.x == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
-    if (threadIdx.x < 64) {
-        uint64_t e = part ? lookback_wave(status, part) : 0;
-        if (threadIdx.x == 0) {
-            s_excl = e;
-            if (part) st_release(status + part, kStInc | (e + agg));
-            if (part == nparts - 1) counters->kept = e + agg;
-        }
-    }
-    __syncthreads();
-    const uint64_t dst = s_excl + ex;
-    for (uint32_t q = 0; q < cnt; ++q)
-        if (dst + q < capacity) out[dst + q] = segs[q];
-    uint32_t prod = produced;
-    for (int off = 32; off > 0; off >>= 1) prod += __shfl_xor(prod, off);
-    if (lane == 0 && prod) atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)prod);
+}
+
+// part_line[p] = the line whose line-pixels contain p * part_size.
+__global__ __launch_bounds__(256) void k_part_lines(const uint32_t *__restrict__ offsets,
+                                                    const uint32_t *__restrict__ lengths, uint64_t n_lines,
+                                                    uint32_t part_size, uint32_t *__restrict__ part_line) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n_lines) return;
+    const uint32_t len = lengths[i];
+    if (!len) return;
+    const uint64_t off = offsets[i];
+    const uint64_t p0 = (off + part_size - 1) / part_size, p1 = (off + len - 1) / part_size;
+    for (uint64_t p = p0; p <= p1; ++p) part_line[p] = (uint32_t)i;
 }
 
 // Digit histograms of every sort pass in one read of the stream (persistent
 // grid, 8 words per thread in flight, LDS histograms flushed once per work
 // group).
 __global__ __launch_bounds__(256) void k_histogram(const uint64_t *__restrict__ words,
-                                                   const FrameCounters *__restrict__ counters, FrameDims dims,
+                                                   FrameCounters *__restrict__ counters, FrameDims dims,
                                                    uint32_t *__restrict__ hist) {
     __shared__ uint32_t s_hist[kMaxPasses * 256];
+    __shared__ uint32_t s_cnt[2];
     const uint32_t radix = 1u << dims.radix_bits;
     const bool lds_hist = dims.passes * radix <= kMaxPasses * 256;
     for (uint32_t k = threadIdx.x; k < kMaxPasses * 256; k += blockDim.x) s_hist[k] = 0;
+    if (threadIdx.x < 2) s_cnt[threadIdx.x] = 0;
     __syncthreads();
-    const uint64_t n = counters->kept;
+    const uint64_t n = 2 * counters->line_pixels;  // word slots
+    uint32_t kept = 0, culled = 0;
     constexpr uint32_t IPT = 8;
     const uint64_t stride = (uint64_t)gridDim.x * blockDim.x * IPT;
     for (uint64_t base = (uint64_t)blockIdx.x * blockDim.x * IPT; base < n; base += stride) {
@@ -445,11 +455,13 @@ __global__ __launch_bounds__(256) void k_histogram(const uint64_t *__restrict__ 
 #pragma unroll
         for (uint32_t k = 0; k < IPT; ++k) {
             const uint64_t i = base + k * blockDim.x + threadIdx.x;
-            w[k] = i < n ? words[i] : ~0ull;
+            w[k] = i < n ? words[i] : kSlotNone;
         }
 #pragma unroll
         for (uint32_t k = 0; k < IPT; ++k) {
-            const bool valid = w[k] != ~0ull;
+            const bool valid = !(w[k] >> 63);
+            kept += valid ? 1u : 0u;
+            culled += w[k] == kSlotCull ? 1u : 0u;
             const uint64_t vmask = __ballot(valid);
             uint32_t shift = kPayloadBits;
             for (uint32_t ps = 0; ps < dims.passes; ++ps, shift += dims.radix_bits) {
@@ -469,10 +481,22 @@ __global__ __launch_bounds__(256) void k_histogram(const uint64_t *__restrict__ 
             }
         }
     }
+    for (int off = 32; off > 0; off >>= 1) {
+        kept += __shfl_xor(kept, off);
+        culled += __shfl_xor(culled, off);
+    }
+    if (lane_id() == 0) {
+        atomicAdd(&s_cnt[0], kept);
+        atomicAdd(&s_cnt[1], culled);
+    }
     __syncthreads();
     if (lds_hist)
         for (uint32_t k = threadIdx.x; k < dims.passes * radix; k += blockDim.x)
             if (s_hist[k]) atomicAdd(&hist[k], s_hist[k]);
+    if (threadIdx.x == 0 && (s_cnt[0] | s_cnt[1])) {
+        atomicAdd((unsigned long long *)&counters->kept, (unsigned long long)s_cnt[0]);
+        atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)s_cnt[0] + s_cnt[1]);
+    }
 }
 
 // First sorted word of every global tile row (row_start[rows] = kept): one
@@ -495,47 +519,21 @@ __global__ void k_row_starts(const uint64_t *__restrict__ sorted, const FrameCou
 // styling are dropped, the rest become render words in input order. Scene 0.
 __global__ __launch_bounds__(256) void k_ingest(const uint64_t *__restrict__ in, uint64_t n_in,
                                                 const SceneDesc *__restrict__ scenes, FrameDims dims,
-                                                FrameCounters *__restrict__ counters, uint64_t *__restrict__ out,
-                                                uint64_t *status, uint32_t *part_counter) {
-    __shared__ uint32_t s_part, wsum[4];
-    __shared__ uint64_t s_excl;
-    const SceneDesc S = scenes[0];
-    const uint32_t nparts = (uint32_t)((n_in + kEmitTile - 1) / kEmitTile);
-    const uint32_t lane = lane_id();
-    if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
-    __syncthreads();
-    const uint32_t part = s_part;
-    if (part >= nparts) return;
-    const uint64_t t0 = (uint64_t)part * kEmitTile + (uint64_t)threadIdx.x * kEmitIPT;
-    uint64_t segs[kEmitIPT];
-    uint32_t cnt = 0, live = 0;
-    for (uint32_t k = 0; k < kEmitIPT; ++k) {
-        const uint64_t p = t0 + k;
-        if (p >= n_in) break;
-        const uint64_t w = in[p];
-        if (w >> 63) continue;
-        ++live;
-        if (((uint32_t)(w >> 22) & 0xFFFFu) >= S.n_props) continue;
+                                                const FrameCounters *__restrict__ counters,
+                                                uint64_t *__restrict__ out) {
+    const uint64_t n_slots = 2 * counters->line_pixels;
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (i >= n_slots) return;
+    const uint64_t w = i < n_in ? in[i] : kNone;
+    uint64_t v = kSlotNone;
+    if (!(w >> 63)) {
+        const SceneDesc &S = scenes[0];
         uint64_t rw;
         uint32_t grow;
-        if (render_word(w, S, dims, &rw, &grow)) segs[cnt++] = rw;
+        v = kSlotCull;
+        if (((uint32_t)(w >> 22) & 0xFFFFu) < S.n_props && render_word(w, S, dims, &rw, &grow)) v = rw;
     }
-    uint32_t agg;
-    const uint32_t ex = block_excl_scan_256(cnt, wsum, &agg);
-    if (threadIdx.x == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
-    if (threadIdx.x < 64) {
-        const uint64_t e = part ? lookback_wave(status, part) : 0;
-        if (threadIdx.x == 0) {
-            s_excl = e;
-            if (part) st_release(status + part, kStInc | (e + agg));
-            if (part == nparts - 1) counters->kept = e + agg;
-        }
-    }
-    __syncthreads();
-    const uint64_t dst = s_excl + ex;
-    for (uint32_t q = 0; q < cnt; ++q) out[dst + q] = segs[q];
-    for (int off = 32; off > 0; off >>= 1) live += __shfl_xor(live, off);
-    if (lane == 0 && live) atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)live);
+    out[i] = v;
 }
 
 // Exclusive scan of each pass's digit histogram (one block per pass).
@@ -577,22 +575,22 @@ template <int R>
 __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                                    const FrameCounters *__restrict__ counters, uint32_t shift,
                                                    const uint32_t *__restrict__ digit_base, uint32_t *status,
-                                                   uint32_t *part_counter) {
+                                                   uint32_t *stall_flag, uint32_t from_slots) {
     constexpr uint32_t RADIX = 1u << R;
     constexpr uint32_t DPT = (RADIX + 255) / 256;  // digits per thread
     __shared__ uint64_t s_keys[kSortTile];
     __shared__ uint32_t s_wcnt[4][RADIX];
     __shared__ uint32_t s_dexcl[RADIX];
     __shared__ uint32_t s_glob[RADIX];
-    __shared__ uint32_t s_part, wsum[4];
-    const uint64_t n = counters->kept;
+    __shared__ uint32_t wsum[4];
+    // the first pass reads the emitted slots and drops their sentinels
+    const uint64_t n = from_slots ? 2 * counters->line_pixels : counters->kept;
     const uint32_t nparts = (uint32_t)((n + kSortTile - 1) / kSortTile);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     {
-        if (threadIdx.x == 0) s_part = atomicAdd(part_counter, 1u);
         for (uint32_t k = threadIdx.x; k < 4 * RADIX; k += 256) (&s_wcnt[0][0])[k] = 0;
         __syncthreads();
-        const uint32_t part = s_part;
+        const uint32_t part = blockIdx.x;
         if (part >= nparts) return;
         const uint64_t base = (uint64_t)part * kSortTile;
         const uint32_t valid_n = (uint32_t)(n - base < kSortTile ? n - base : kSortTile);
@@ -602,6 +600,10 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
             uint32_t li = w * 1024 + i * 64 + lane;
             bool valid = li < valid_n;
             uint64_t key = valid ? in[base + li] : 0ull;
+            if (key >> 63) {
+                valid = false;
+                key = 0;
+            }
             uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
             uint64_t peers = __ballot(valid);
 #pragma unroll
@@ -648,15 +650,33 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
         for (uint32_t k = 0; k < DPT; ++k) {
             uint32_t d = threadIdx.x * DPT + k;
             if (d >= RADIX) continue;
-            uint32_t excl = 0;
+            // predecessors read 8 at a time: one round trip per 8 steps
+            uint32_t excl = 0, spins = 0, nap = 1;
             int64_t p = (int64_t)part - 1;
             while (p >= 0) {
-                uint32_t s = ld_acquire32(status + (uint64_t)p * RADIX + d);
-                uint32_t f = s >> 30;
-                if (f == 0) { __builtin_amdgcn_s_sleep(1); continue; }
-                excl += s & kSt32Val;
-                if (f == 2) break;
-                --p;
+                uint32_t s[8];
+#pragma unroll
+                for (int k = 0; k < 8; ++k)
+                    s[k] = p - k >= 0 ? ld_acquire32(status + (uint64_t)(p - k) * RADIX + d) : kSt32Inc;
+                bool done = false, stall = false;
+#pragma unroll
+                for (int k = 0; k < 8; ++k) {
+                    if (done || stall) continue;
+                    const uint32_t f = s[k] >> 30;
+                    if (f == 0) { stall = true; continue; }
+                    excl += s[k] & kSt32Val;
+                    --p;
+                    if (f == 2) done = true;
+                }
+                if (done) break;
+                if (stall) {
+                    if ((spins += nap) > kSpinLimit) {
+                        atomicOr(stall_flag, 1u);
+                        break;
+                    }
+                    backoff(nap);
+                    nap = nap < 32 ? nap * 2 : 32;
+                }
             }
             if (part) st_release32(status + (uint64_t)part * RADIX + d, kSt32Inc | (excl + tots[k]));
             s_glob[d] = digit_base[d] + excl - s_dexcl[d];
@@ -668,7 +688,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
             s_keys[s_dexcl[d] + s_wcnt[w][d] + rank[i]] = keys[i];
         }
         __syncthreads();
-        for (uint32_t k = threadIdx.x; k < valid_n; k += 256) {
+        for (uint32_t k = threadIdx.x; k < all; k += 256) {
             uint64_t key = s_keys[k];
             uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
             out[(uint64_t)s_glob[d] + k] = key;
@@ -677,11 +697,11 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
 }
 
 template __global__ void k_sort_pass<4>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *);
+                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
 template __global__ void k_sort_pass<6>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *);
+                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
 template __global__ void k_sort_pass<8>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *);
+                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
 
 // ------------------------------------------------------------------ paint
 //

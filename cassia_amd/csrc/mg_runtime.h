@@ -99,6 +99,7 @@ private:
     DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
     DevBuf row_count_, row_start_, row_scene_, row_status_;
     DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
+    DevBuf part_line_;
     DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     uint64_t n_records_ = 0, n_runs_ = 0;

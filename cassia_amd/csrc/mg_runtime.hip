@@ -62,7 +62,7 @@ Renderer::~Renderer() {
                      &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &image_, &overflow_,
-                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &tile_start_, &tile_nr_,
+                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &tile_start_, &tile_nr_,
                      &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_};
     for (DevBuf *b : all) b->release();
     for (DevBuf &b : caches_) b.release();
@@ -271,7 +271,9 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     MG_HIP(hipEventRecord(ev0_, stream_), "event");
     MG_HIP(hipEventRecord(evs_[0], stream_), "event");
     fill64((uint64_t *)ctr, 0, sizeof(FrameCounters) / 8, stream_);
-    fill32(err_.as<uint32_t>(), 0, 1, stream_);
+    // err_[0]: a line longer than i32 pixels; err_[1]: a look-back wait gave up
+    fill32(err_.as<uint32_t>(), 0, 2, stream_);
+    uint32_t *stall = err_.as<uint32_t>() + 1;
     fill32(pc, 0, 64, stream_);
     const bool ingest = spec.mode == 2;
     if (L && !ingest) {
@@ -283,7 +285,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
         if (!scan_status_.reserve(nparts * 8)) { err = "out of device memory"; return false; }
         fill64(scan_status_.as<uint64_t>(), 0, nparts, stream_);
         k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(lengths_.as<uint32_t>(), offsets_.as<uint32_t>(), L,
-                                                        scan_status_.as<uint64_t>(), pc + 0, &ctr->line_pixels);
+                                                        scan_status_.as<uint64_t>(), stall, &ctr->line_pixels);
     }
     MG_HIP(hipGetLastError(), "setup/scan launch");
     FrameCounters hc{};
@@ -308,28 +310,28 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     res.line_pixels = ingest ? 0 : P;
     MG_HIP(hipEventRecord(evs_[1], stream_), "event");
     const uint64_t cap_segs = std::max<uint64_t>(2 * P, 1);
-    const uint64_t nparts_emit = ((ingest ? spec.n_in : P) + kEmitTileHost - 1) / kEmitTileHost;
+    const uint64_t nparts_emit = (P + kEmitTileHost - 1) / kEmitTileHost;
     const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
-    if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) ||
-        !emit_status_.reserve(std::max<uint64_t>(nparts_emit, 1) * 8) || !hist_.reserve(16 * radix * 4) ||
+    if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) || !hist_.reserve(16 * radix * 4) ||
         !bases_.reserve(16 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4 * dims.passes) ||
-        !row_count_.reserve(((size_t)global_rows + 1) * 4) || !row_start_.reserve(((size_t)global_rows + 1) * 4) ||
-        !row_status_.reserve(((global_rows + 1 + kScanTileHost - 1) / kScanTileHost) * 8)) {
+        !row_start_.reserve(((size_t)global_rows + 1) * 4)) {
         err = "out of device memory (segments)";
         return false;
     }
     fill32(hist_.as<uint32_t>(), 0, dims.passes * radix, stream_);
-    fill32(row_count_.as<uint32_t>(), 0, global_rows + 1, stream_);
+    // every line-pixel owns two word slots; the ingest path uses the same layout
     if (ingest && spec.n_in) {
-        fill64(emit_status_.as<uint64_t>(), 0, nparts_emit, stream_);
-        k_ingest<<<(uint32_t)nparts_emit, 256, 0, stream_>>>(spec.in_segs, spec.n_in, scenes_.as<SceneDesc>(), dims,
-                                                           ctr, segs_a_.as<uint64_t>(), emit_status_.as<uint64_t>(),
-                                                           pc + 1);
+        fill64((uint64_t *)&ctr->line_pixels, P, 1, stream_);
+        k_ingest<<<(uint32_t)((2 * P + 255) / 256), 256, 0, stream_>>>(spec.in_segs, spec.n_in,
+                                                                       scenes_.as<SceneDesc>(), dims, ctr,
+                                                                       segs_a_.as<uint64_t>());
     } else if (P) {
-        fill64(emit_status_.as<uint64_t>(), 0, nparts_emit, stream_);
+        if (!part_line_.reserve(nparts_emit * 4)) { err = "out of device memory"; return false; }
+        k_part_lines<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(offsets_.as<uint32_t>(), lengths_.as<uint32_t>(),
+                                                                      L, kEmitTileHost, part_line_.as<uint32_t>());
         k_emit<<<(uint32_t)nparts_emit, 256, 0, stream_>>>(setup_.as<LineSetup>(), offsets_.as<uint32_t>(),
                                                          scenes_.as<SceneDesc>(), dims, ctr, segs_a_.as<uint64_t>(),
-                                                         cap_segs, emit_status_.as<uint64_t>(), pc + 1);
+                                                         part_line_.as<uint32_t>());
     }
     MG_HIP(hipGetLastError(), "emit launch");
     MG_HIP(hipEventRecord(evs_[2], stream_), "event");
@@ -344,17 +346,17 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     const uint32_t sort_grid = (uint32_t)std::max<uint64_t>(nparts_sort, 1);
     for (uint32_t p = 0; p < dims.passes; ++p) {
         const uint32_t shift = kPayloadBits + p * R;
-        uint32_t *counter = pc + 8 + p;
         uint32_t *status = sort_status_.as<uint32_t>() + (size_t)p * nparts_sort * radix;
+        const uint32_t from_slots = p == 0 ? 1u : 0u;
         if (R == 8)
             k_sort_pass<8><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, counter);
+                                                            status, stall, from_slots);
         else if (R == 6)
             k_sort_pass<6><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, counter);
+                                                            status, stall, from_slots);
         else
             k_sort_pass<4><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, counter);
+                                                            status, stall, from_slots);
         std::swap(in, out);
     }
     MG_HIP(hipGetLastError(), "sort launch");
@@ -386,7 +388,13 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     }
     MG_HIP(hipEventRecord(ev1_, stream_), "event");
     MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
+    uint32_t hstall = 0;
+    MG_HIP(hipMemcpyAsync(&hstall, stall, 4, hipMemcpyDeviceToHost, stream_), "read stall");
     MG_HIP(hipStreamSynchronize(stream_), "frame");
+    if (hstall) {
+        err = "a decoupled look-back wait timed out; the frame is invalid";
+        return false;
+    }
     float ms = 0.0f;
     MG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
     for (int k = 0; k < 7; ++k) {
