@@ -735,7 +735,7 @@ size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride) {
     return (size_t)cap * 24 + (size_t)rcap * 16 + (size_t)tstride * 8 + 64;
 }
 size_t paint_lds_bytes_host(uint32_t cap, bool clips) {
-    return (size_t)kStageWin * 8 + (clips ? (size_t)cap * 4 : 0);
+    return (size_t)kStageWin * 8 + 2 * 64 * 4 + (clips ? (size_t)cap * 4 : 0);
 }
 
 __device__ __forceinline__ int32_t cov_byte(uint64_t c, uint32_t y) { return as_i8((int32_t)(c >> (8 * y))); }
@@ -1111,27 +1111,39 @@ __device__ __forceinline__ void stage_load(Stage &st, const uint64_t *segs, uint
 // P is the i8 running cover prefix of row py over columns < px starting from
 // the carried cover cy, area the i16 area sum of cell (px, py); the record's
 // run is words [beg, beg + len) of the sorted stream, all inside [.., lim).
-__device__ __forceinline__ int32_t run_area(Stage &st, const uint64_t *segs, uint32_t beg, uint32_t len,
-                                            uint32_t lim, int32_t cy, uint32_t px, uint32_t py) {
-    int32_t P = cy, ar = 0;
-    uint32_t s = beg;
+// The lanes take the run's words 64 at a time and add area and cover into
+// the wave's 64 cells (LDS, no return values); each lane then reads its own
+// cell and the column prefix is a stride-8 scan across lanes (lane = x*8+y).
+__device__ __forceinline__ int32_t run_area(Stage &st, int32_t *cells, const uint64_t *segs, uint32_t beg,
+                                            uint32_t len, uint32_t lim, int32_t cy) {
+    const uint32_t lane = lane_id();
+    int32_t *c_area = cells, *c_cov = cells + 64;
+    c_area[lane] = 0;
+    c_cov[lane] = 0;
     const uint32_t e = beg + len;
     const uint32_t hi = lim > e ? lim : e;
-    while (s < e) {
-        if (s < st.base || s >= st.end) stage_load(st, segs, s, hi - s < kStageWin ? hi : s + kStageWin);
-        const uint32_t stop = e < st.end ? e : st.end;
-        for (; s < stop; ++s) {
+    for (uint32_t s0 = beg; s0 < e; s0 += 64) {
+        if (s0 < st.base || s0 + 64 > st.end) {
+            if (!(s0 >= st.base && e <= st.end))
+                stage_load(st, segs, s0, hi - s0 < kStageWin ? hi : s0 + kStageWin);
+        }
+        const uint32_t s = s0 + lane;
+        if (s < e) {
             const uint64_t w = st.w[s - st.base];
-            const uint32_t x = (uint32_t)(w >> 16) & 7u, y = (uint32_t)(w >> 19) & 7u;
-            const int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
-            const int32_t area = (int32_t)((int64_t)(w << 48) >> 54);
-            if (y == py) {
-                if (x < px) P += cover;
-                else if (x == px) ar += area;
-            }
+            const uint32_t cell = ((uint32_t)(w >> 16) & 7u) * 8u + ((uint32_t)(w >> 19) & 7u);
+            atomicAdd(&c_area[cell], (int32_t)((int64_t)(w << 48) >> 54));
+            atomicAdd(&c_cov[cell], (int32_t)((int64_t)(w << 58) >> 58));
         }
     }
-    return 16 * as_i8(P) + as_i16(ar);
+    const int32_t ar = c_area[lane], cv = c_cov[lane];
+    int32_t inc = cv, t;
+    t = __shfl_up(inc, 8);
+    if (lane >= 8) inc += t;
+    t = __shfl_up(inc, 16);
+    if (lane >= 16) inc += t;
+    t = __shfl_up(inc, 32);
+    if (lane >= 32) inc += t;
+    return 16 * as_i8(cy + inc - cv) + as_i16(ar);
 }
 
 // One batch of up to 64 records, lane k holding record k0 + k.
@@ -1177,12 +1189,49 @@ __device__ __forceinline__ void stage_batch(Stage &st, const uint64_t *segs, con
     if (b.lim - first <= kStageWin && (first < st.base || b.lim > st.end)) stage_load(st, segs, first, b.lim);
 }
 
-__device__ __forceinline__ int32_t record_area(Stage &st, const uint64_t *segs, const Batch &b, uint32_t q,
-                                               uint32_t px, uint32_t py) {
+__device__ __forceinline__ int32_t record_area(Stage &st, int32_t *cells, const uint64_t *segs, const Batch &b,
+                                               uint32_t q, uint32_t py) {
     const uint32_t key = rlane(b.key, q);
     const int32_t cy = (key & kRecCarry) ? cov_byte(rlane64(b.cov, q), py) : 0;
     if (!(key & kRecRun)) return 16 * cy;
-    return run_area(st, segs, rlane(b.beg, q), rlane(b.len, q), b.lim, cy, px, py);
+    return run_area(st, cells, segs, rlane(b.beg, q), rlane(b.len, q), b.lim, cy);
+}
+
+// blend_at for one pixel (painter/mod.rs:364-400): one dispatch on the
+// layer's (uniform) blend mode for all three channels.
+__device__ __forceinline__ void blend_pixel(uint32_t mode, float &c0, float &c1, float &c2, float &ca, float f0,
+                                            float f1, float f2, float a) {
+    const float inv = 1.0f - a;
+    float b0, b1, b2;
+    switch (mode) {
+#define MG_BLEND_CASE(m)                  \
+    case m:                               \
+        b0 = blend_lane(m, c0, f0) * a;   \
+        b1 = blend_lane(m, c1, f1) * a;   \
+        b2 = blend_lane(m, c2, f2) * a;   \
+        break;
+        MG_BLEND_CASE(BM_MULTIPLY)
+        MG_BLEND_CASE(BM_SCREEN)
+        MG_BLEND_CASE(BM_OVERLAY)
+        MG_BLEND_CASE(BM_DARKEN)
+        MG_BLEND_CASE(BM_LIGHTEN)
+        MG_BLEND_CASE(BM_COLOR_DODGE)
+        MG_BLEND_CASE(BM_COLOR_BURN)
+        MG_BLEND_CASE(BM_HARD_LIGHT)
+        MG_BLEND_CASE(BM_SOFT_LIGHT)
+        MG_BLEND_CASE(BM_DIFFERENCE)
+        MG_BLEND_CASE(BM_EXCLUSION)
+#undef MG_BLEND_CASE
+    default:  // Over, and modes outside the table (blend_lane's default)
+        b0 = f0 * a;
+        b1 = f1 * a;
+        b2 = f2 * a;
+        break;
+    }
+    c0 = __builtin_fmaf(c0, inv, b0);
+    c1 = __builtin_fmaf(c1, inv, b1);
+    c2 = __builtin_fmaf(c2, inv, b2);
+    ca = __builtin_fmaf(ca, inv, a);
 }
 
 // One wave paints one tile (LayerWorkbench::drive_tile_painting,
@@ -1208,7 +1257,8 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     Stage st;
     st.w = (uint64_t *)lds_raw;
     st.base = st.end = 0;
-    uint32_t *fl = clips ? (uint32_t *)(lds_raw + kStageWin * 8) : nullptr;
+    int32_t *cells = (int32_t *)(lds_raw + kStageWin * 8);
+    uint32_t *fl = clips ? (uint32_t *)(lds_raw + kStageWin * 8 + 2 * 64 * 4) : nullptr;
     const uint64_t *segs = pp.segs;
     if (fl) {
         for (uint32_t k = lane; k < n; k += 64)
@@ -1224,7 +1274,7 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
             stage_batch(st, segs, b);
             const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
             for (uint32_t q = 0; q < cnt; ++q) {
-                const int32_t A = record_area(st, segs, b, q, px, py);
+                const int32_t A = record_area(st, cells, segs, b, q, py);
                 uint32_t r = 0;
                 if (lane == 0) r = atomicAdd(pp.cov_count, 1u);
                 r = __shfl(r, 0);
@@ -1388,7 +1438,7 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
                 if (!(f & kMaskBit)) continue;
                 const uint32_t lay = rlane(b.key, q) & 0xFFFFu;
                 if (has_clip && clip_last < lay) has_clip = false;
-                const int32_t A = record_area(st, segs, b, q, px, py);
+                const int32_t A = record_area(st, cells, segs, b, q, py);
                 const float cov = coverage_from_area(A, f & PF_EVENODD);
                 if (f & PF_CLIP) {
                     // clip_at (painter/mod.rs:402-416)
@@ -1413,15 +1463,7 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
                 // blend_at (painter/mod.rs:364-400)
                 float a = f3 * cov;
                 if (apply_clip) a *= clip_mask;
-                const float inv = 1.0f - a;
-                const uint32_t mode = (f >> PF_BLEND_SHIFT) & 15u;
-                const float b0 = blend_lane(mode, c0, f0) * a;
-                const float b1 = blend_lane(mode, c1, f1) * a;
-                const float b2 = blend_lane(mode, c2, f2) * a;
-                c0 = __builtin_fmaf(c0, inv, b0);
-                c1 = __builtin_fmaf(c1, inv, b1);
-                c2 = __builtin_fmaf(c2, inv, b2);
-                ca = __builtin_fmaf(ca, inv, a);
+                blend_pixel((f >> PF_BLEND_SHIFT) & 15u, c0, c1, c2, ca, f0, f1, f2, a);
             }
         }
         pix = pixel_bytes(c0, c1, c2, ca);
