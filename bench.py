@@ -108,6 +108,7 @@ def main():
     ap.add_argument("--cpu-fraction", type=float, default=0.25)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--radix", type=int, default=0, help="sort digit width (0 = automatic)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only copy a device buffer of this many bytes (HBM counter calibration)")
     args = ap.parse_args()
@@ -136,6 +137,7 @@ def main():
     comp = scene.build(cassia_amd)
     t_build = time.perf_counter() - t_build
     W, H = scene.width, scene.height
+    comp.set_tuning(args.radix)
 
     for _ in range(args.warmup):
         comp.render_device(W, H, scene.background)

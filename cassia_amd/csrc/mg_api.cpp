@@ -710,8 +710,8 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
 
 MG_EXPORT mg_status mg_set_tuning(mg_composition *c, uint32_t radix_bits) {
     if (check_comp(c)) return MG_ERR_INVALID_ARG;
-    if (radix_bits != 0 && radix_bits != 4 && radix_bits != 6 && radix_bits != 8)
-        return fail(MG_ERR_INVALID_ARG, "radix_bits must be 0, 4, 6 or 8");
+    if (radix_bits != 0 && radix_bits != 4 && radix_bits != 6 && radix_bits != 8 && radix_bits != 9)
+        return fail(MG_ERR_INVALID_ARG, "radix_bits must be 0, 4, 6, 8 or 9");
     c->radix_bits = radix_bits;
     return MG_OK;
 }

@@ -55,6 +55,10 @@ __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64
                            uint32_t *stall, uint64_t *total);
 __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
                        FrameDims dims, const FrameCounters *counters, uint64_t *out, const uint32_t *part_line);
+__global__ void k_order_hist(const LineSetup *setup, uint64_t n_lines, uint32_t order_mask, uint32_t *hist);
+__global__ void k_order_scatter(const LineSetup *setup, const uint32_t *lengths, uint64_t n_lines,
+                                uint32_t order_mask, uint32_t *base, LineSetup *setup_perm,
+                                uint32_t *lengths_perm);
 __global__ void k_part_lines(const uint32_t *offsets, const uint32_t *lengths, uint64_t n_lines,
                              uint32_t part_size, uint32_t *part_line);
 __global__ void k_histogram(const uint64_t *words, FrameCounters *counters, FrameDims dims, uint32_t *hist);

@@ -420,6 +420,66 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     }
 }
 
+// Lines grouped by layer order (a counting sort; the order inside one layer
+// is arbitrary): histogram of the orders, exclusive scan, scatter of the
+// line setup records and lengths. Consecutive lines mostly share an order
+// (the lines of one path), so each run of equal orders inside a wave takes
+// one atomic, issued by the run's first lane.
+__device__ __forceinline__ uint32_t run_head_lane(bool valid, uint32_t order, uint64_t *heads_out) {
+    const uint32_t lane = lane_id();
+    uint32_t prev = __shfl_up(order, 1);
+    const bool head = valid && (lane == 0 || prev != order);
+    const uint64_t heads = __ballot(head);
+    *heads_out = heads;
+    const uint64_t le = heads & ((2ull << lane) - 1ull);
+    return le ? 63u - (uint32_t)__clzll((long long)le) : 0u;
+}
+
+__global__ __launch_bounds__(256) void k_order_hist(const LineSetup *__restrict__ setup, uint64_t n_lines,
+                                                    uint32_t order_mask, uint32_t *__restrict__ hist) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const bool valid = i < n_lines;
+    const uint32_t order = valid ? setup[i].packed & order_mask : 0xFFFFFFFFu;
+    uint64_t heads;
+    run_head_lane(valid, order, &heads);
+    const uint32_t lane = lane_id();
+    const uint32_t vn = (uint32_t)__popcll(__ballot(valid));  // valid lanes are a prefix
+    if ((heads >> lane) & 1ull) {
+        const uint64_t after = heads & ~((2ull << lane) - 1ull);
+        const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+        const uint32_t end = next < vn ? next : vn;
+        atomicAdd(&hist[order], end - lane);
+    }
+}
+
+__global__ __launch_bounds__(256) void k_order_scatter(const LineSetup *__restrict__ setup,
+                                                       const uint32_t *__restrict__ lengths, uint64_t n_lines,
+                                                       uint32_t order_mask, uint32_t *__restrict__ base,
+                                                       LineSetup *__restrict__ setup_perm,
+                                                       uint32_t *__restrict__ lengths_perm) {
+    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const bool valid = i < n_lines;
+    LineSetup L;
+    if (valid) L = setup[i];
+    const uint32_t order = valid ? L.packed & order_mask : 0xFFFFFFFFu;
+    uint64_t heads;
+    const uint32_t head = run_head_lane(valid, order, &heads);
+    const uint32_t lane = lane_id();
+    const uint32_t vn = (uint32_t)__popcll(__ballot(valid));  // valid lanes are a prefix
+    uint32_t first = 0;
+    if ((heads >> lane) & 1ull) {
+        const uint64_t after = heads & ~((2ull << lane) - 1ull);
+        const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+        const uint32_t end = next < vn ? next : vn;
+        first = atomicAdd(&base[order], end - lane);
+    }
+    const uint32_t pos = __shfl(first, (int)head) + (lane - head);
+    if (valid) {
+        setup_perm[pos] = L;
+        lengths_perm[pos] = lengths[i];
+    }
+}
+
 // part_line[p] = the line whose line-pixels contain p * part_size.
 __global__ __launch_bounds__(256) void k_part_lines(const uint32_t *__restrict__ offsets,
                                                     const uint32_t *__restrict__ lengths, uint64_t n_lines,
@@ -463,7 +523,7 @@ __global__ __launch_bounds__(256) void k_histogram(const uint64_t *__restrict__ 
             kept += valid ? 1u : 0u;
             culled += w[k] == kSlotCull ? 1u : 0u;
             const uint64_t vmask = __ballot(valid);
-            uint32_t shift = kPayloadBits;
+            uint32_t shift = dims.sort_shift;
             for (uint32_t ps = 0; ps < dims.passes; ++ps, shift += dims.radix_bits) {
                 const uint32_t dg = (uint32_t)(w[k] >> shift) & (radix - 1);
                 // high digits are mostly equal across a wave: one add for all
@@ -701,6 +761,8 @@ template __global__ void k_sort_pass<4>(const uint64_t *, uint64_t *, const Fram
 template __global__ void k_sort_pass<6>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
                                         const uint32_t *, uint32_t *, uint32_t *, uint32_t);
 template __global__ void k_sort_pass<8>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
+                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
+template __global__ void k_sort_pass<9>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
                                         const uint32_t *, uint32_t *, uint32_t *, uint32_t);
 
 // ------------------------------------------------------------------ paint

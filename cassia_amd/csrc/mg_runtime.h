@@ -34,7 +34,7 @@ struct FrameSpec {
     uint32_t mode = 0;                  // 0 render, 1 canonical stream, 2 render an ingested stream
     const uint64_t *in_segs = nullptr;  // mode 2: device PixelSegment stream
     uint64_t n_in = 0;
-    uint32_t radix_bits = 8;
+    uint32_t radix_bits = 0;            // sort digit width; 0 = automatic
     bool want_tile_ops = false;
     bool want_coverage = false;
     uint32_t coverage_cap = 0;
@@ -99,7 +99,7 @@ private:
     DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
     DevBuf row_count_, row_start_, row_scene_, row_status_;
     DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
-    DevBuf part_line_;
+    DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
     DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     uint64_t n_records_ = 0, n_runs_ = 0;

@@ -80,7 +80,8 @@ struct FrameDims {
     uint32_t global_rows;
     uint32_t col_bits, layer_bits, key_bits;
     uint32_t radix_bits, passes;
-    uint32_t mode;  // 0: render (cull + render key), 1: canonical full stream
+    uint32_t mode;        // 0: render (cull + render key), 1: canonical full stream, 2: ingest
+    uint32_t sort_shift;  // bit of the first sort digit
 };
 
 // Device-side counters of a frame.
@@ -88,7 +89,7 @@ struct FrameCounters {
     uint64_t line_pixels;     // P
     uint64_t segments;        // non-NONE pixel segments
     uint64_t kept;            // segments emitted for sorting
-    uint64_t rows_total;      // sum of per-tile-row counts (== kept in render mode)
+    uint64_t order_total;     // lines counted by the line-order histogram
 };
 
 }  // namespace mg

@@ -223,7 +223,7 @@ MG_EXPORT mg_status mg_render_coverage(mg_composition *comp, uint32_t width, uin
                                        uint32_t *key_out, size_t cap, size_t *n);
 
 /* Result-invariant tuning knob of the renderer (0 = automatic): the sort's
-   digit width in bits (4, 6 or 8). Every setting produces the same bytes;
+   digit width in bits (4, 6, 8 or 9). Every setting produces the same bytes;
    only the speed changes. */
 MG_EXPORT mg_status mg_set_tuning(mg_composition *comp, uint32_t radix_bits);
 

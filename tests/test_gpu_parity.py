@@ -93,7 +93,7 @@ def test_pixel_segments_canonical_stream(gpu, cpu):
         assert np.array_equal(s_cpu, s_gpu)
 
 
-@pytest.mark.parametrize("radix", [4, 6, 8])
+@pytest.mark.parametrize("radix", [4, 6, 8, 9])
 def test_tuning_is_result_invariant(gpu, cpu, radix):
     w, h = 150, 70
     comp = scenegen.build_scene(cpu, 600, w, h, 60, transforms=True, clips=True)
