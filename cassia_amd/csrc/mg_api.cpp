@@ -122,6 +122,12 @@ struct mg_composition {
     uint32_t radix_bits = 0;
     uint8_t *device_frame = nullptr;
     size_t device_frame_bytes = 0;
+    // frame tables (layers by id, props by order) of the last render, valid
+    // while `version` and the layer cache id are unchanged
+    mg::FrameSpec spec;
+    uint64_t spec_version = 0;
+    int32_t spec_cache_id = -2;
+    bool spec_clips = false;
 
     size_t actual_len() const {
         size_t n = 0;
@@ -500,12 +506,20 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
                               bool want_tile_ops, mg::FrameResult &r, uint32_t coverage_cap = 0) {
     std::string err;
     const size_t row_len = (width + 7) / 8, tile_rows = (rows_px + 7) / 8;
-    c->remove_disabled();
-    for (auto &kv : c->layers)
-        if (kv.second.enabled && kv.second.has_order) c->orders_to_layers[kv.second.order] = kv.first;
-    mg::FrameSpec f;
-    bool has_clips = false;
-    c->tables(cache_id, f, has_clips);
+    if (c->spec_version != c->version || c->spec_cache_id != cache_id) {
+        // every change of lines, layers, props or cache state bumps `version`
+        c->remove_disabled();
+        for (auto &kv : c->layers)
+            if (kv.second.enabled && kv.second.has_order) c->orders_to_layers[kv.second.order] = kv.first;
+        c->spec = mg::FrameSpec{};
+        c->tables(cache_id, c->spec, c->spec_clips);
+        c->spec_version = c->version;
+        c->spec_cache_id = cache_id;
+    }
+    mg::FrameSpec &f = c->spec;
+    const bool has_clips = c->spec_clips;
+    f.scenes.clear();
+    f.scene_line_base.clear();
     f.tables_version = c->version * 64 + (uint64_t)(cache_id + 1);
     mg::SceneDesc sd{};
     sd.line_base = 0;
@@ -543,7 +557,7 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.want_tile_ops = want_tile_ops;
     f.want_coverage = coverage_cap > 0;
     f.coverage_cap = coverage_cap;
-    if (c->radix_bits) f.radix_bits = c->radix_bits;
+    f.radix_bits = c->radix_bits;
     mg_status s;
     if ((s = c->sync_lines())) return s;
     if (!c->rdr->run(f, c->ids.size(), img, prev, r, err))

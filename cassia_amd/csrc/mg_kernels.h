@@ -81,5 +81,6 @@ size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 constexpr uint32_t kEmitTileHost = 2048;
 constexpr uint32_t kSortTileHost = 4096;
 constexpr uint32_t kScanTileHost = 2048;
+constexpr uint32_t kWalkThreadsHost = 1024;
 
 }  // namespace mg

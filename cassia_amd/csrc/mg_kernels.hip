@@ -120,6 +120,28 @@ __device__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *wsum /* 4 */, uint
     return base + x - v;
 }
 
+// Block-wide exclusive scan of one u32 per thread over W waves.
+template <uint32_t W>
+__device__ uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* W */, uint32_t *total) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    uint32_t x = v;
+    for (int off = 1; off < 64; off <<= 1) {
+        uint32_t y = __shfl_up(x, off);
+        if (lane >= (uint32_t)off) x += y;
+    }
+    if (lane == 63) wsum[w] = x;
+    __syncthreads();
+    uint32_t base = 0, all = 0;
+    for (uint32_t k = 0; k < W; ++k) {
+        uint32_t s = wsum[k];
+        if (k < w) base += s;
+        all += s;
+    }
+    __syncthreads();
+    *total = all;
+    return base + x - v;
+}
+
 // ------------------------------------------------------------ fill kernels
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
@@ -792,6 +814,7 @@ constexpr uint32_t kRunEvenOdd = 1u << 16;                    // run key: layer 
 constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flags
 constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
 constexpr uint32_t kRecChunk = 16384;                         // records reserved per row at a time
+constexpr uint32_t kWalkThreads = 1024;                       // threads per tile row in the row walk
 
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride) {
     return (size_t)cap * 24 + (size_t)rcap * 16 + (size_t)tstride * 8 + 64;
@@ -1004,9 +1027,9 @@ __device__ __forceinline__ uint32_t merge_split(const uint32_t *qk, uint32_t nq,
 // tiles left of the crop and tile -1 are folded without dropping empty
 // covers; inside the crop, covers that become empty are dropped,
 // layer_workbench.rs:194,475).
-__global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
+__global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_wsum[kWalkThreads / 64];
     __shared__ uint32_t s_rec[3];
     const uint32_t tid = threadIdx.x;
     const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs, tstride = pp.tstride;
@@ -1029,7 +1052,7 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
     if (j < S.crop_v0 || j >= S.crop_v1) return;
     const uint32_t h0 = S.crop_h0, h1 = scene_h1(S);
     if (h0 >= h1) return;
-    for (uint32_t k = tid; k < tstride; k += 256) {
+    for (uint32_t k = tid; k < tstride; k += kWalkThreads) {
         ts_l[k] = pp.tile_start[(uint64_t)grow * tstride + k];
         nr_l[k] = k + 1 < tstride ? pp.tile_nr[(uint64_t)grow * tstride + k] : 0u;
     }
@@ -1037,25 +1060,53 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
     uint32_t nq = 0, cur = 0;
     uint32_t rec_next = 0, rec_end = 0;
     bool counting = false;  // record store full: only count what the frame needs
-    for (uint32_t colp = 0; colp <= h1; ++colp) {
+    // the next column to walk: fold columns without runs change nothing
+    auto next_colp = [&](uint32_t c) {
+        uint32_t x = c + 1;
+        while (x <= h0 && nr_l[x] == 0) ++x;
+        return x;
+    };
+    // Runs of the next column are prefetched into registers while the
+    // current column is merged (up to kPrefetch per thread), then written to
+    // LDS after the column's last barrier.
+    constexpr uint32_t kPrefetch = 2;
+    uint32_t pf_key[kPrefetch], pf_beg[kPrefetch];
+    uint64_t pf_cov[kPrefetch];
+    uint32_t pf_n = 0;  // runs of `colp` already in LDS
+    for (uint32_t colp = nr_l[0] ? 0u : next_colp(0); colp <= h1; colp = next_colp(colp)) {
         const bool fold = colp <= h0;
         const uint32_t nr = nr_l[colp];
-        if (fold && nr == 0) continue;
         if (nr > rcap || nq + nr > cap) {
             if (tid == 0) atomicOr(pp.overflow, 1u);
             return;
         }
         const uint32_t ts = ts_l[colp], te = ts_l[colp + 1];
-        for (uint32_t k = tid; k < nr; k += 256) {
+        for (uint32_t k = pf_n + tid; k < nr; k += kWalkThreads) {
             rk[k] = pp.run_key[ts + k];
             rb[k] = pp.run_beg[ts + k];
             rc[k] = pp.run_cov[ts + k];
         }
         __syncthreads();
+        const uint32_t ncol = next_colp(colp);
+        uint32_t pf_next = 0;
+        if (ncol <= h1) {
+            const uint32_t nr2 = nr_l[ncol], ts2 = ts_l[ncol];
+            pf_next = nr2 < kPrefetch * kWalkThreads ? nr2 : kPrefetch * kWalkThreads;
+            if (pf_next > rcap) pf_next = rcap;
+#pragma unroll
+            for (uint32_t j = 0; j < kPrefetch; ++j) {
+                const uint32_t k = tid + kWalkThreads * j;
+                if (k < pf_next) {
+                    pf_key[j] = pp.run_key[ts2 + k];
+                    pf_beg[j] = pp.run_beg[ts2 + k];
+                    pf_cov[j] = pp.run_cov[ts2 + k];
+                }
+            }
+        }
         const uint32_t *qkc = qk[cur];
         const uint64_t *qcc = qc[cur];
         const uint32_t total = nq + nr;
-        const uint32_t per = (total + 255) / 256;
+        const uint32_t per = (total + kWalkThreads - 1) / kWalkThreads;
         const uint32_t d0 = tid * per < total ? tid * per : total;
         const uint32_t d1 = d0 + per < total ? d0 + per : total;
         const uint32_t i0 = merge_split(qkc, nq, rk, nr, d0);
@@ -1092,7 +1143,7 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
             cq += keep ? 1u : 0u;
         });
         uint32_t all;
-        const uint32_t ex = block_excl_scan_256(cu | (cq << 16), s_wsum, &all);
+        const uint32_t ex = block_excl_scan<kWalkThreads / 64>(cu | (cq << 16), s_wsum, &all);
         const uint32_t n_union = all & 0xFFFFu, nq_new = all >> 16;
         uint32_t rec_off = 0;
         if (!fold && counting) {
@@ -1149,6 +1200,16 @@ __global__ __launch_bounds__(256) void k_row_walk(PaintParams pp) {
             atomicAdd(pp.n_records, n_union);
         }
         __syncthreads();
+#pragma unroll
+        for (uint32_t j = 0; j < kPrefetch; ++j) {
+            const uint32_t k = tid + kWalkThreads * j;
+            if (k < pf_next) {
+                rk[k] = pf_key[j];
+                rb[k] = pf_beg[j];
+                rc[k] = pf_cov[j];
+            }
+        }
+        pf_n = pf_next;
         nq = nq_new;
         cur = nxt;
     }

@@ -102,6 +102,8 @@ private:
     DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
     DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
+    std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
+    uint32_t n_work_ = 0;
     uint64_t n_records_ = 0, n_runs_ = 0;
     std::vector<DevBuf> caches_;
     std::vector<size_t> cache_tiles_;

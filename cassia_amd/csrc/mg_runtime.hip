@@ -450,24 +450,32 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
                      const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
                      std::string &err) {
     if (global_rows == 0) return true;
-    // one paint work group per tile inside the crop
-    std::vector<uint32_t> work;
+    // one paint work group per tile inside the crop; the list is rebuilt
+    // only when the scene descriptors change
     uint32_t tstride = 2;
     bool clips = false;
-    for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
-        const SceneDesc &s = spec.scenes[sc];
-        const uint32_t h1 = std::min(s.crop_h1, s.row_len);
-        tstride = std::max(tstride, h1 + 2);
+    for (const SceneDesc &s : spec.scenes) {
+        tstride = std::max(tstride, std::min(s.crop_h1, s.row_len) + 2);
         clips |= (s.flags & SF_CLIPS) != 0;
-        for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j)
-            for (uint32_t c = s.crop_h0; c < h1; ++c) {
-                work.push_back(s.row_base + j);
-                work.push_back(c);
-            }
     }
-    const uint32_t n_work = (uint32_t)(work.size() / 2);
+    if (work_scenes_.size() != spec.scenes.size() ||
+        std::memcmp(work_scenes_.data(), spec.scenes.data(), spec.scenes.size() * sizeof(SceneDesc)) != 0) {
+        std::vector<uint32_t> work;
+        for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
+            const SceneDesc &s = spec.scenes[sc];
+            const uint32_t h1 = std::min(s.crop_h1, s.row_len);
+            for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j)
+                for (uint32_t c = s.crop_h0; c < h1; ++c) {
+                    work.push_back(s.row_base + j);
+                    work.push_back(c);
+                }
+        }
+        n_work_ = (uint32_t)(work.size() / 2);
+        if (n_work_ && !upload_vec(work_, work, stream_, err)) return false;
+        work_scenes_ = spec.scenes;
+    }
+    const uint32_t n_work = n_work_;
     if (n_work == 0) return true;
-    if (!upload_vec(work_, work, stream_, err)) return false;
     const size_t n_cols = (size_t)global_rows * tstride;
     if (!rec_hdr_.reserve(std::max<size_t>(tiles, 1) * 8) || !tile_start_.reserve(n_cols * 4) ||
         !tile_nr_.reserve(n_cols * 4) || !run_key_.reserve(words_cap * 4) || !run_beg_.reserve(words_cap * 4) ||
@@ -546,7 +554,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
         MG_HIP(hipEventRecord(evs_[5], stream_), "event");
-        k_row_walk<<<global_rows, 256, lds_w, stream_>>>(pp);
+        k_row_walk<<<global_rows, kWalkThreadsHost, lds_w, stream_>>>(pp);
         MG_HIP(hipEventRecord(evs_[6], stream_), "event");
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
         MG_HIP(hipEventRecord(evs_[7], stream_), "event");
