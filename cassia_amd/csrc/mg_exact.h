@@ -11,12 +11,13 @@
 
 namespace mg {
 
-// `f32 as i32`: truncate toward zero, saturate, NaN -> 0.
+// `f32 as i32`: truncate toward zero, saturate, NaN -> 0. Written with
+// selects only (the operand is clamped into range before the conversion).
 __host__ __device__ __forceinline__ int32_t f2i_sat(float v) {
-    if (!(v == v)) return 0;
-    if (v >= 2147483648.0f) return 2147483647;
-    if (v <= -2147483648.0f) return (int32_t)0x80000000u;
-    return (int32_t)v;
+    const float c = __builtin_fminf(__builtin_fmaxf(v, -2147483648.0f), 2147483520.0f);
+    int32_t r = (int32_t)c;
+    r = v >= 2147483648.0f ? 2147483647 : r;
+    return v == v ? r : 0;
 }
 
 // `f32 as u8`: truncate, saturate, NaN -> 0.

@@ -275,30 +275,33 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
 
 // -------------------------------------------------------------- emission
 
+// PixelSegment of one rasterized piece (raster_segment.rs:142-160), without
+// branches on the octant (lanes of a wave hold different lines):
+//   tiles()     (rasterizer/mod.rs:134-147): tile coordinates swapped for
+//               octants 1, 2, 5, 6 (mask 0x66);
+//   area_cover() (rasterizer/mod.rs:149-172): x/y swapped for the same
+//               octants, the area axis reversed for octants 2, 6 (0x44) and
+//               the cover axis for octants 3, 4, 5, 6 (0x78).
 __device__ __forceinline__ uint64_t pack_segment(uint32_t layer, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
                                                  int32_t bx, int32_t by, uint32_t octant) {
-    // tiles() (rasterizer/mod.rs:134-147)
-    if (octant == 1 || octant == 2 || octant == 5 || octant == 6) { int32_t t = bx; bx = by; by = t; }
-    int32_t ti = as_i16(bx >> 3);
-    int32_t tj = as_i16(by >> 3);
-    uint32_t tx = (uint32_t)(bx & 7), ty = (uint32_t)(by & 7);
-    // area_cover() (rasterizer/mod.rs:149-172)
-    int32_t ax0, ax1, ay0, ay1;
-    switch (octant) {
-    case 1: ax0 = y0; ax1 = y1; ay0 = x0; ay1 = x1; break;
-    case 2: ax0 = y1; ax1 = y0; ay0 = x0; ay1 = x1; break;
-    case 3: case 4: ax0 = x0; ax1 = x1; ay0 = y1; ay1 = y0; break;
-    case 5: ax0 = y0; ax1 = y1; ay0 = x1; ay1 = x0; break;
-    case 6: ax0 = y1; ax1 = y0; ay0 = x1; ay1 = x0; break;
-    default: ax0 = x0; ax1 = x1; ay0 = y0; ay1 = y1; break;
-    }
-    int32_t border = wrap_add(ax0 & ~15, 16);
-    int32_t h = wrap_sub(ay1, ay0);
-    int32_t tri = as_i16(wrap_mul(h, wrap_sub(ax1, ax0)) / 2);
-    int32_t rect = as_i16(wrap_mul(h, wrap_sub(border, ax1)));
-    int32_t area = as_i16(tri + rect);
-    int32_t cover = as_i8(h);
-    int32_t tib = as_i16((ti < -1 ? -1 : ti) + 256);
+    const bool sw = (0x66u >> octant) & 1u;
+    const bool fa = (0x44u >> octant) & 1u;
+    const bool fb = (0x78u >> octant) & 1u;
+    const int32_t tbx = sw ? by : bx, tby = sw ? bx : by;
+    const int32_t ti = as_i16(tbx >> 3);
+    const int32_t tj = as_i16(tby >> 3);
+    const uint32_t tx = (uint32_t)(tbx & 7), ty = (uint32_t)(tby & 7);
+    const int32_t a0 = sw ? y0 : x0, a1 = sw ? y1 : x1;
+    const int32_t c0 = sw ? x0 : y0, c1 = sw ? x1 : y1;
+    const int32_t ax0 = fa ? a1 : a0, ax1 = fa ? a0 : a1;
+    const int32_t ay0 = fb ? c1 : c0, ay1 = fb ? c0 : c1;
+    const int32_t border = wrap_add(ax0 & ~15, 16);
+    const int32_t h = wrap_sub(ay1, ay0);
+    const int32_t tri = as_i16(wrap_mul(h, wrap_sub(ax1, ax0)) / 2);
+    const int32_t rect = as_i16(wrap_mul(h, wrap_sub(border, ax1)));
+    const int32_t area = as_i16(tri + rect);
+    const int32_t cover = as_i8(h);
+    const int32_t tib = as_i16((ti < -1 ? -1 : ti) + 256);
     uint64_t w = ((uint64_t)(uint32_t)tj & 0xFFFull) << 51;
     w |= ((uint64_t)(uint32_t)tib & 0x1FFFull) << 38;
     w |= (uint64_t)(layer & 0xFFFFu) << 22;
@@ -309,27 +312,27 @@ __device__ __forceinline__ uint64_t pack_segment(uint32_t layer, int32_t x0, int
     return w;
 }
 
-// Rasterizer::rasterize for one line-pixel (rasterizer/mod.rs:233-281).
+// Rasterizer::rasterize for one line-pixel (rasterizer/mod.rs:233-281): one
+// piece, or two when the span crosses a pixel-row border. Both pieces are
+// always computed (no divergence); the return value says how many count.
 __device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, uint64_t out[2]) {
-    uint32_t layer = l.packed & 0xFFFFu, octant = (l.packed >> 16) & 7u;
-    int32_t x0i = wrap_add(l.start, pi);
-    int32_t x1i = wrap_add(x0i, 1);
-    float x0 = fmax_rs((float)x0i, l.start_f32);
-    float x1 = fmin_rs((float)x1i, l.end_f32);
-    float y0 = __builtin_fmaf(l.slope, x0, l.py_slope_px);
-    float y1 = __builtin_fmaf(l.slope, x1, l.py_slope_px);
-    int32_t x0s = sub_pixel(x0), x1s = sub_pixel(x1), y0s = sub_pixel(y0), y1s = sub_pixel(y1);
-    if ((y0s & 15) != 0 && (y1s & 15) != 0 && (y0s >> 4) != (y1s >> 4)) {
-        int32_t y = (y0s > y1s ? y0s : y1s) >> 4;
-        int32_t ys = (int32_t)((uint32_t)y << 4);
-        float x = __builtin_fmaf(l.slope_recip, (float)y, l.px_slope_recip_py);
-        int32_t xs = sub_pixel(x);
-        out[0] = pack_segment(layer, x0s, xs, y0s, ys, x0i, (ys < y0s ? ys : y0s) >> 4, octant);
-        out[1] = pack_segment(layer, xs, x1s, ys, y1s, x0i, (ys < y1s ? ys : y1s) >> 4, octant);
-        return 2;
-    }
-    out[0] = pack_segment(layer, x0s, x1s, y0s, y1s, x0i, (y0s < y1s ? y0s : y1s) >> 4, octant);
-    return 1;
+    const uint32_t layer = l.packed & 0xFFFFu, octant = (l.packed >> 16) & 7u;
+    const int32_t x0i = wrap_add(l.start, pi);
+    const int32_t x1i = wrap_add(x0i, 1);
+    const float x0 = fmax_rs((float)x0i, l.start_f32);
+    const float x1 = fmin_rs((float)x1i, l.end_f32);
+    const float y0 = __builtin_fmaf(l.slope, x0, l.py_slope_px);
+    const float y1 = __builtin_fmaf(l.slope, x1, l.py_slope_px);
+    const int32_t x0s = sub_pixel(x0), x1s = sub_pixel(x1), y0s = sub_pixel(y0), y1s = sub_pixel(y1);
+    const bool split = (y0s & 15) != 0 && (y1s & 15) != 0 && (y0s >> 4) != (y1s >> 4);
+    const int32_t y = (y0s > y1s ? y0s : y1s) >> 4;
+    const int32_t ys = (int32_t)((uint32_t)y << 4);
+    const float x = __builtin_fmaf(l.slope_recip, (float)y, l.px_slope_recip_py);
+    const int32_t xs = sub_pixel(x);
+    const int32_t ex = split ? xs : x1s, ey = split ? ys : y1s;
+    out[0] = pack_segment(layer, x0s, ex, y0s, ey, x0i, (ey < y0s ? ey : y0s) >> 4, octant);
+    out[1] = pack_segment(layer, xs, x1s, ys, y1s, x0i, (ys < y1s ? ys : y1s) >> 4, octant);
+    return split ? 2 : 1;
 }
 
 constexpr uint32_t kEmitThreads = 256;
@@ -359,6 +362,26 @@ __device__ __forceinline__ bool render_word(uint64_t w, const SceneDesc &S, cons
     return true;
 }
 
+// The scene fields of the render-mode filter, and the filter itself: the
+// render word, or kSlotCull when the segment can never influence the frame.
+struct SceneCull {
+    int32_t v0, v1, h1;
+    uint32_t row_base;
+};
+__device__ __forceinline__ SceneCull scene_cull(const SceneDesc &S) {
+    return SceneCull{(int32_t)S.crop_v0, (int32_t)S.crop_v1, (int32_t)S.crop_h1, S.row_base};
+}
+__device__ __forceinline__ uint64_t render_word_cull(uint64_t w, const SceneCull &c, const FrameDims &d) {
+    const int32_t tj = (int32_t)((int64_t)(w << 1) >> 52);  // sign-extended 12-bit field
+    const int32_t col = (int32_t)((w >> 38) & 0x1FFFu) - 256;
+    const uint32_t colp = col < 0 ? 0u : (uint32_t)col + 1u;
+    const uint32_t layer = (uint32_t)(w >> 22) & 0xFFFFu;
+    const uint32_t row = c.row_base + (uint32_t)tj;
+    const uint64_t key = ((uint64_t)row << (d.col_bits + d.layer_bits)) | ((uint64_t)colp << d.layer_bits) | layer;
+    const bool keep = tj >= c.v0 && tj < c.v1 && col < c.h1;
+    return keep ? (key << kPayloadBits) | (w & ((1ull << kPayloadBits) - 1)) : kSlotCull;
+}
+
 // Emission: every line-pixel p owns the two slots 2p, 2p+1 of the word
 // buffer, the layout of the reference's rasterizer output ([seg, seg|NONE],
 // rasterizer/mod.rs:195, 272-279). A slot holds a word, kSlotNone (no second
@@ -366,13 +389,58 @@ __device__ __forceinline__ bool render_word(uint64_t w, const SceneDesc &S, cons
 // frame). Slot order is generation order (line, pixel, slot); the first sort
 // pass drops the sentinels while it scatters, so no compaction pass and no
 // cross-work-group prefix is needed here. One work group per kEmitTile
-// consecutive line-pixels, kEmitIPT consecutive line-pixels per thread.
+// consecutive line-pixels; thread t takes line-pixels b0 + t + 256 k, so each
+// store instruction of a wave writes 64 consecutive slot pairs (1 KB).
+constexpr uint32_t kEmitLines = 512;  // lines of a partition staged in LDS
+
+// Line-pixels b0 + threadIdx.x + 256 k of [.., b1) (see k_emit). With
+// Staged, `offs` and `lines` are LDS tables of lines l0 .. (indexed l - l0);
+// otherwise they are the global arrays.
+template <bool Staged>
+__device__ __forceinline__ void emit_span(const LineSetup *lines, const uint32_t *offs, uint32_t l0, uint32_t l1,
+                                          uint64_t b0, uint64_t b1, uint64_t P,
+                                          const SceneDesc *__restrict__ scenes, const FrameDims &dims,
+                                          uint4 *__restrict__ slots) {
+    const uint32_t base = Staged ? l0 : 0u;
+    // offset of line l in [l0, l1 + 1]: first line-pixel of l
+    auto off = [&](uint32_t l) -> uint64_t { return l >= dims.n_lines ? P : (uint64_t)offs[l - base]; };
+    uint64_t p = b0 + threadIdx.x;
+    if (p >= b1) return;
+    // the line holding p: last line in [l0, l1] with offset <= p
+    uint32_t lo = l0, hi = l1 + 1;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (off(mid) <= p) lo = mid; else hi = mid;
+    }
+    uint32_t li = lo;
+    uint64_t first = off(li), next = off(li + 1);
+    for (; p < b1; p += 256) {
+        while (p >= next) {  // zero-length lines share offsets
+            ++li;
+            first = next;
+            next = off(li + 1);
+        }
+        const LineSetup L = lines[li - base];
+        const SceneCull sc = scene_cull(scenes[L.packed >> 19]);
+        uint64_t w[2];
+        const int n = rasterize_pixel(L, (int32_t)(p - first), w);
+        uint64_t v[2] = {w[0], w[1]};
+        if (dims.mode == 0) {
+#pragma unroll
+            for (int q = 0; q < 2; ++q) v[q] = render_word_cull(w[q], sc, dims);
+        }
+        if (n < 2) v[1] = kSlotNone;
+        slots[p] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
+    }
+}
+
 __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ lines,
                                               const uint32_t *__restrict__ offsets,
                                               const SceneDesc *__restrict__ scenes, FrameDims dims,
                                               const FrameCounters *__restrict__ counters,
                                               uint64_t *__restrict__ out, const uint32_t *__restrict__ part_line) {
-    __shared__ uint32_t s_off[kEmitTile + 1];
+    __shared__ uint32_t s_off[kEmitLines + 1];
+    __shared__ LineSetup s_lines[kEmitLines];
     const uint64_t P = counters->line_pixels;
     const uint32_t nparts = (uint32_t)((P + kEmitTile - 1) / kEmitTile);
     const uint32_t part = blockIdx.x;
@@ -383,62 +451,16 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     // line-pixel to the one holding the next partition's (a superset)
     const uint32_t l0 = part_line[part];
     const uint32_t l1 = part + 1 < nparts ? part_line[part + 1] : dims.n_lines - 1;
-    const bool in_lds = l1 - l0 + 1 <= kEmitTile;
-    if (in_lds)
-        for (uint32_t k = threadIdx.x; k <= l1 - l0; k += blockDim.x) s_off[k] = offsets[l0 + k];
-    __syncthreads();
-    const uint64_t t0 = b0 + (uint64_t)threadIdx.x * kEmitIPT;
-    if (t0 >= b1) return;
-    uint32_t li;
-    if (in_lds) {
-        uint32_t lo = 0, hi = l1 - l0 + 1;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if ((uint64_t)s_off[mid] <= t0) lo = mid; else hi = mid;
-        }
-        li = l0 + lo;
-    } else {
-        uint32_t lo = l0, hi = l1 + 1;
-        while (hi - lo > 1) {
-            uint32_t mid = (lo + hi) >> 1;
-            if ((uint64_t)offsets[mid] <= t0) lo = mid; else hi = mid;
-        }
-        li = lo;
-    }
-    LineSetup L = lines[li];
-    uint64_t first = offsets[li];
-    uint64_t next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
-    while (next <= t0) {  // zero-length lines share offsets
-        ++li;
-        L = lines[li];
-        first = next;
-        next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
-    }
-    const SceneDesc *S = scenes + (L.packed >> 19);
+    const uint32_t nl = l1 - l0 + 1;
     uint4 *slots = reinterpret_cast<uint4 *>(out);
-    for (uint32_t k = 0; k < kEmitIPT; ++k) {
-        const uint64_t p = t0 + k;
-        if (p >= b1) break;
-        while (p >= next) {
-            ++li;
-            L = lines[li];
-            first = next;
-            next = li + 1 < dims.n_lines ? offsets[li + 1] : P;
-            S = scenes + (L.packed >> 19);
-        }
-        uint64_t w[2];
-        const int n = rasterize_pixel(L, (int32_t)(p - first), w);
-        uint64_t v[2] = {kSlotNone, kSlotNone};
-        for (int q = 0; q < n; ++q) {
-            if (dims.mode == 0) {
-                uint64_t rw;
-                uint32_t grow;
-                v[q] = render_word(w[q], *S, dims, &rw, &grow) ? rw : kSlotCull;
-            } else {
-                v[q] = w[q];
-            }
-        }
-        slots[p] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
+    if (nl <= kEmitLines) {
+        for (uint32_t k = threadIdx.x; k <= nl; k += blockDim.x)
+            s_off[k] = l0 + k < dims.n_lines ? offsets[l0 + k] : (uint32_t)P;
+        for (uint32_t k = threadIdx.x; k < nl; k += blockDim.x) s_lines[k] = lines[l0 + k];
+        __syncthreads();
+        emit_span<true>(s_lines, s_off, l0, l1, b0, b1, P, scenes, dims, slots);
+    } else {  // a partition of very short or zero-length lines
+        emit_span<false>(lines, offsets, l0, l1, b0, b1, P, scenes, dims, slots);
     }
 }
 
