@@ -1379,6 +1379,58 @@ __device__ __forceinline__ void blend_pixel(uint32_t mode, float &c0, float &c1,
     ca = __builtin_fmaf(ca, inv, a);
 }
 
+// The painter over records [skipped, n) of a tile (painter/mod.rs:260-400):
+// coverage, clip masks (only in scenes with clips), column skipping and
+// blending into this lane's pixel colour.
+template <bool Clips>
+__device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsEntry *props, const uint32_t *fl,
+                                             const uint4 *rec, const uint32_t *rlen, uint32_t n, uint32_t skipped,
+                                             Stage &st, int32_t *cells, const uint64_t *segs, uint32_t px,
+                                             uint32_t py, uint32_t c, uint32_t j, float &c0, float &c1, float &c2,
+                                             float &ca) {
+    bool has_clip = false;
+    uint32_t clip_last = 0;
+    float clip_mask = 0.0f;
+    for (uint32_t k0 = skipped; k0 < n; k0 += 64) {
+        Batch b;
+        load_batch(pp, props, fl, rec, rlen, n, k0, b);
+        stage_batch(st, segs, b);
+        const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
+        for (uint32_t q = 0; q < cnt; ++q) {
+            const uint32_t f = rlane(b.flags, q);
+            if (Clips && !(f & kMaskBit)) continue;
+            const uint32_t lay = rlane(b.key, q) & 0xFFFFu;
+            if (Clips && has_clip && clip_last < lay) has_clip = false;
+            const int32_t A = record_area(st, cells, segs, b, q, py);
+            const float cov = coverage_from_area(A, f & PF_EVENODD);
+            if (Clips && (f & PF_CLIP)) {
+                // clip_at (painter/mod.rs:402-416)
+                if (!has_clip) {
+                    has_clip = true;
+                    clip_last = (lay + props[lay].clip_layers) & 0xFFFFu;
+                }
+                clip_mask = cov;
+                continue;
+            }
+            const bool apply_clip = Clips && (f & PF_CLIPPED) && !(f & kSkipClipBit);
+            const uint64_t nz = __ballot(cov != 0.0f);
+            const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
+            if (col_zero || (apply_clip && !has_clip)) continue;
+            float f0 = rlane_f(b.color.x, q), f1 = rlane_f(b.color.y, q), f2 = rlane_f(b.color.z, q),
+                  f3 = rlane_f(b.color.w, q);
+            if (f & PF_GRADIENT) {
+                float g[4];
+                gradient_color(props[lay], pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, g);
+                f0 = g[0]; f1 = g[1]; f2 = g[2]; f3 = g[3];
+            }
+            // blend_at (painter/mod.rs:364-400)
+            float a = f3 * cov;
+            if (apply_clip) a *= clip_mask;
+            blend_pixel((f >> PF_BLEND_SHIFT) & 15u, c0, c1, c2, ca, f0, f1, f2, a);
+        }
+    }
+}
+
 // One wave paints one tile (LayerWorkbench::drive_tile_painting,
 // layer_workbench.rs:430-486, and Painter, painter/mod.rs:260-493).
 __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
@@ -1570,47 +1622,11 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     uint32_t pix = 0;
     if (op == 2) {
         float c0 = S.clear[0], c1 = S.clear[1], c2 = S.clear[2], ca = S.clear[3];
-        bool has_clip = false;
-        uint32_t clip_last = 0;
-        float clip_mask = 0.0f;
-        for (uint32_t k0 = skipped; k0 < n; k0 += 64) {
-            Batch b;
-            load_batch(pp, props, fl, rec, rlen, n, k0, b);
-            stage_batch(st, segs, b);
-            const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
-            for (uint32_t q = 0; q < cnt; ++q) {
-                const uint32_t f = rlane(b.flags, q);
-                if (!(f & kMaskBit)) continue;
-                const uint32_t lay = rlane(b.key, q) & 0xFFFFu;
-                if (has_clip && clip_last < lay) has_clip = false;
-                const int32_t A = record_area(st, cells, segs, b, q, py);
-                const float cov = coverage_from_area(A, f & PF_EVENODD);
-                if (f & PF_CLIP) {
-                    // clip_at (painter/mod.rs:402-416)
-                    if (!has_clip) {
-                        has_clip = true;
-                        clip_last = (lay + props[lay].clip_layers) & 0xFFFFu;
-                    }
-                    clip_mask = cov;
-                    continue;
-                }
-                const bool apply_clip = (f & PF_CLIPPED) && !(f & kSkipClipBit);
-                const uint64_t nz = __ballot(cov != 0.0f);
-                const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
-                if (col_zero || (apply_clip && !has_clip)) continue;
-                float f0 = rlane_f(b.color.x, q), f1 = rlane_f(b.color.y, q), f2 = rlane_f(b.color.z, q),
-                      f3 = rlane_f(b.color.w, q);
-                if (f & PF_GRADIENT) {
-                    float g[4];
-                    gradient_color(props[lay], pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, g);
-                    f0 = g[0]; f1 = g[1]; f2 = g[2]; f3 = g[3];
-                }
-                // blend_at (painter/mod.rs:364-400)
-                float a = f3 * cov;
-                if (apply_clip) a *= clip_mask;
-                blend_pixel((f >> PF_BLEND_SHIFT) & 15u, c0, c1, c2, ca, f0, f1, f2, a);
-            }
-        }
+        if (clips)
+            paint_layers<true>(pp, props, fl, rec, rlen, n, skipped, st, cells, segs, px, py, c, j, c0, c1, c2, ca);
+        else
+            paint_layers<false>(pp, props, nullptr, rec, rlen, n, skipped, st, cells, segs, px, py, c, j, c0, c1,
+                                c2, ca);
         pix = pixel_bytes(c0, c1, c2, ca);
     } else if (op == 1) {
         const float sc4[4] = {solid0, solid1, solid2, solid3};
