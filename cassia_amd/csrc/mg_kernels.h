@@ -53,8 +53,16 @@ __global__ void k_setup(const float4 *pts, const uint16_t *ids, const LayerEntry
                         LineSetup *out, uint32_t *lengths, uint32_t *err);
 __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *status,
                            uint32_t *stall, uint64_t *total);
+// Per-partition slot counts written by the emission: pass 0's digit counts
+// (digit-major, one column per partition) and slices of the other totals.
+struct SlotCounts {
+    uint32_t *cnt0;
+    uint32_t *acc;
+};
+constexpr uint32_t kCountSlicesHost = 256;
 __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
-                       FrameDims dims, const FrameCounters *counters, uint64_t *out, const uint32_t *part_line);
+                       FrameDims dims, const FrameCounters *counters, uint64_t *out, const uint32_t *part_line,
+                       SlotCounts sc);
 __global__ void k_order_hist(const LineSetup *setup, uint64_t n_lines, uint32_t order_mask, uint32_t *hist);
 __global__ void k_order_scatter(const LineSetup *setup, const uint32_t *lengths, uint64_t n_lines,
                                 uint32_t order_mask, uint32_t *base, LineSetup *setup_perm,
@@ -65,12 +73,13 @@ __global__ void k_histogram(const uint64_t *words, FrameCounters *counters, Fram
 __global__ void k_row_starts(const uint64_t *sorted, const FrameCounters *counters, uint32_t rows,
                              uint32_t row_shift, uint32_t *row_start);
 __global__ void k_ingest(const uint64_t *in, uint64_t n_in, const SceneDesc *scenes, FrameDims dims,
-                         const FrameCounters *counters, uint64_t *out);
+                         const FrameCounters *counters, uint64_t *out, SlotCounts sc);
+__global__ void k_count_reduce(const uint32_t *acc, FrameDims dims, uint32_t *hist, FrameCounters *counters);
 __global__ void k_digit_bases(const uint32_t *hist, uint32_t *bases, uint32_t radix);
 template <int R>
 __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
                             uint32_t shift, const uint32_t *digit_base, uint32_t *status,
-                            uint32_t *stall, uint32_t from_slots);
+                            uint32_t *stall, const uint32_t *first_off);
 __global__ void k_tile_starts(PaintParams pp, uint32_t global_rows);
 __global__ void k_tile_runs(PaintParams pp);
 __global__ void k_row_walk(PaintParams pp);

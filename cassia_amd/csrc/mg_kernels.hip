@@ -335,9 +335,13 @@ __device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, u
     return split ? 2 : 1;
 }
 
+constexpr uint32_t kSortThreads = 256;
+constexpr uint32_t kSortKPT = 16;
+constexpr uint32_t kSortTile = kSortThreads * kSortKPT;  // 4096 keys
 constexpr uint32_t kEmitThreads = 256;
 constexpr uint32_t kEmitIPT = 8;
 constexpr uint32_t kEmitTile = kEmitThreads * kEmitIPT;
+static_assert(2 * kEmitTile == kSortTile, "an emission partition fills exactly one sort partition");
 constexpr uint32_t kMaxPasses = 8;
 // Slot sentinels (bit 63 set; no word has it: render words use 57 bits and
 // emitted PixelSegments are never NONE).
@@ -392,6 +396,43 @@ __device__ __forceinline__ uint64_t render_word_cull(uint64_t w, const SceneCull
 // consecutive line-pixels; thread t takes line-pixels b0 + t + 256 k, so each
 // store instruction of a wave writes 64 consecutive slot pairs (1 KB).
 constexpr uint32_t kEmitLines = 512;  // lines of a partition staged in LDS
+constexpr uint32_t kCountSlices = 256;  // accumulation slices of the later passes' digit totals
+
+// Per-partition counts of the word slots (a partition is kSortTile slots).
+// LDS layout `s_cnt`: the digits of every sort pass (pass-major), then kept
+// words, then culled words. At the end of the partition, pass 0's digit
+// counts go to cnt0[digit * nparts + part] (scanned into the first pass's
+// output offsets, so that pass needs no look-back) and the other counts are
+// added into slice part % kCountSlices of `acc` (reduced to totals).
+
+__device__ __forceinline__ uint32_t count_rows(const FrameDims &d) { return d.passes * (1u << d.radix_bits) + 2; }
+
+__device__ __forceinline__ void count_zero(uint32_t *s_cnt, const FrameDims &d) {
+    for (uint32_t k = threadIdx.x; k < count_rows(d); k += blockDim.x) s_cnt[k] = 0;
+}
+
+__device__ __forceinline__ void count_slot(uint32_t *s_cnt, uint64_t v, const FrameDims &d) {
+    const uint32_t radix = 1u << d.radix_bits, nd = d.passes * radix;
+    if (v >> 63) {
+        if (v == kSlotCull) atomicAdd(&s_cnt[nd + 1], 1u);
+        return;
+    }
+    atomicAdd(&s_cnt[nd], 1u);
+    uint32_t shift = d.sort_shift;
+    for (uint32_t ps = 0; ps < d.passes; ++ps, shift += d.radix_bits)
+        atomicAdd(&s_cnt[ps * radix + ((uint32_t)(v >> shift) & (radix - 1))], 1u);
+}
+
+// Called by the whole work group after its last count_slot.
+__device__ __forceinline__ void count_flush(const uint32_t *s_cnt, const FrameDims &d, const SlotCounts &sc,
+                                            uint32_t part, uint32_t nparts) {
+    __syncthreads();
+    const uint32_t radix = 1u << d.radix_bits, rest = count_rows(d) - radix;
+    for (uint32_t k = threadIdx.x; k < radix; k += blockDim.x) sc.cnt0[(uint64_t)k * nparts + part] = s_cnt[k];
+    uint32_t *a = sc.acc + (uint64_t)(part % kCountSlices) * rest;
+    for (uint32_t k = threadIdx.x; k < rest; k += blockDim.x)
+        if (s_cnt[radix + k]) atomicAdd(&a[k], s_cnt[radix + k]);
+}
 
 // Line-pixels b0 + threadIdx.x + 256 k of [.., b1) (see k_emit). With
 // Staged, `offs` and `lines` are LDS tables of lines l0 .. (indexed l - l0);
@@ -400,7 +441,7 @@ template <bool Staged>
 __device__ __forceinline__ void emit_span(const LineSetup *lines, const uint32_t *offs, uint32_t l0, uint32_t l1,
                                           uint64_t b0, uint64_t b1, uint64_t P,
                                           const SceneDesc *__restrict__ scenes, const FrameDims &dims,
-                                          uint4 *__restrict__ slots) {
+                                          uint4 *__restrict__ slots, uint32_t *s_cnt) {
     const uint32_t base = Staged ? l0 : 0u;
     // offset of line l in [l0, l1 + 1]: first line-pixel of l
     auto off = [&](uint32_t l) -> uint64_t { return l >= dims.n_lines ? P : (uint64_t)offs[l - base]; };
@@ -431,6 +472,8 @@ __device__ __forceinline__ void emit_span(const LineSetup *lines, const uint32_t
         }
         if (n < 2) v[1] = kSlotNone;
         slots[p] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
+        count_slot(s_cnt, v[0], dims);
+        count_slot(s_cnt, v[1], dims);
     }
 }
 
@@ -438,13 +481,16 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
                                               const uint32_t *__restrict__ offsets,
                                               const SceneDesc *__restrict__ scenes, FrameDims dims,
                                               const FrameCounters *__restrict__ counters,
-                                              uint64_t *__restrict__ out, const uint32_t *__restrict__ part_line) {
+                                              uint64_t *__restrict__ out, const uint32_t *__restrict__ part_line,
+                                              SlotCounts sc) {
     __shared__ uint32_t s_off[kEmitLines + 1];
     __shared__ LineSetup s_lines[kEmitLines];
+    extern __shared__ uint32_t s_cnt[];
     const uint64_t P = counters->line_pixels;
     const uint32_t nparts = (uint32_t)((P + kEmitTile - 1) / kEmitTile);
     const uint32_t part = blockIdx.x;
     if (part >= nparts) return;
+    count_zero(s_cnt, dims);
     const uint64_t b0 = (uint64_t)part * kEmitTile;
     const uint64_t b1 = b0 + kEmitTile < P ? b0 + kEmitTile : P;
     // lines covering [b0, b1): from the line holding the partition's first
@@ -458,9 +504,35 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
             s_off[k] = l0 + k < dims.n_lines ? offsets[l0 + k] : (uint32_t)P;
         for (uint32_t k = threadIdx.x; k < nl; k += blockDim.x) s_lines[k] = lines[l0 + k];
         __syncthreads();
-        emit_span<true>(s_lines, s_off, l0, l1, b0, b1, P, scenes, dims, slots);
+        emit_span<true>(s_lines, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt);
     } else {  // a partition of very short or zero-length lines
-        emit_span<false>(lines, offsets, l0, l1, b0, b1, P, scenes, dims, slots);
+        __syncthreads();
+        emit_span<false>(lines, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt);
+    }
+    count_flush(s_cnt, dims, sc, part, nparts);
+}
+
+// Totals of the later passes' digits (into hist[radix ..]) and of the kept
+// and culled words (into the frame counters): one work group per row.
+__global__ __launch_bounds__(256) void k_count_reduce(const uint32_t *__restrict__ acc, FrameDims dims,
+                                                      uint32_t *__restrict__ hist, FrameCounters *__restrict__ counters) {
+    __shared__ uint32_t wsum[4];
+    const uint32_t radix = 1u << dims.radix_bits, rest = dims.passes * radix + 2 - radix;
+    const uint32_t r = blockIdx.x;
+    if (r >= rest) return;
+    uint32_t v = 0;
+    for (uint32_t k = threadIdx.x; k < kCountSlices; k += 256) v += acc[(uint64_t)k * rest + r];
+    uint32_t all;
+    block_excl_scan_256(v, wsum, &all);
+    if (threadIdx.x != 0) return;
+    const uint32_t nd = rest - 2;
+    if (r < nd) {
+        hist[radix + r] = all;
+    } else if (r == nd) {
+        counters->kept = all;
+        atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)all);
+    } else {
+        atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)all);
     }
 }
 
@@ -624,20 +696,30 @@ __global__ void k_row_starts(const uint64_t *__restrict__ sorted, const FrameCou
 __global__ __launch_bounds__(256) void k_ingest(const uint64_t *__restrict__ in, uint64_t n_in,
                                                 const SceneDesc *__restrict__ scenes, FrameDims dims,
                                                 const FrameCounters *__restrict__ counters,
-                                                uint64_t *__restrict__ out) {
+                                                uint64_t *__restrict__ out, SlotCounts sc) {
+    extern __shared__ uint32_t s_cnt[];
     const uint64_t n_slots = 2 * counters->line_pixels;
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    if (i >= n_slots) return;
-    const uint64_t w = i < n_in ? in[i] : kNone;
-    uint64_t v = kSlotNone;
-    if (!(w >> 63)) {
-        const SceneDesc &S = scenes[0];
-        uint64_t rw;
-        uint32_t grow;
-        v = kSlotCull;
-        if (((uint32_t)(w >> 22) & 0xFFFFu) < S.n_props && render_word(w, S, dims, &rw, &grow)) v = rw;
+    const uint32_t nparts = (uint32_t)((n_slots + kSortTile - 1) / kSortTile);
+    const uint32_t part = blockIdx.x;
+    if (part >= nparts) return;
+    count_zero(s_cnt, dims);
+    __syncthreads();
+    const SceneDesc &S = scenes[0];
+    for (uint32_t k = 0; k < kSortTile / 256; ++k) {
+        const uint64_t i = (uint64_t)part * kSortTile + k * 256 + threadIdx.x;
+        if (i >= n_slots) break;
+        const uint64_t w = i < n_in ? in[i] : kNone;
+        uint64_t v = kSlotNone;
+        if (!(w >> 63)) {
+            uint64_t rw;
+            uint32_t grow;
+            v = kSlotCull;
+            if (((uint32_t)(w >> 22) & 0xFFFFu) < S.n_props && render_word(w, S, dims, &rw, &grow)) v = rw;
+        }
+        out[i] = v;
+        count_slot(s_cnt, v, dims);
     }
-    out[i] = v;
+    count_flush(s_cnt, dims, sc, part, nparts);
 }
 
 // Exclusive scan of each pass's digit histogram (one block per pass).
@@ -664,9 +746,6 @@ __global__ __launch_bounds__(256) void k_digit_bases(const uint32_t *hist, uint3
 
 // ------------------------------------------------------------------- sort
 
-constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortKPT = 16;
-constexpr uint32_t kSortTile = kSortThreads * kSortKPT;  // 4096 keys
 constexpr uint32_t kSt32Agg = 1u << 30;
 constexpr uint32_t kSt32Inc = 2u << 30;
 constexpr uint32_t kSt32Val = (1u << 30) - 1u;
@@ -679,7 +758,7 @@ template <int R>
 __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                                    const FrameCounters *__restrict__ counters, uint32_t shift,
                                                    const uint32_t *__restrict__ digit_base, uint32_t *status,
-                                                   uint32_t *stall_flag, uint32_t from_slots) {
+                                                   uint32_t *stall_flag, const uint32_t *__restrict__ first_off) {
     constexpr uint32_t RADIX = 1u << R;
     constexpr uint32_t DPT = (RADIX + 255) / 256;  // digits per thread
     __shared__ uint64_t s_keys[kSortTile];
@@ -687,7 +766,9 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
     __shared__ uint32_t s_dexcl[RADIX];
     __shared__ uint32_t s_glob[RADIX];
     __shared__ uint32_t wsum[4];
-    // the first pass reads the emitted slots and drops their sentinels
+    // the first pass reads the emitted slots and drops their sentinels; its
+    // output offsets per (digit, partition) come scanned from the emission
+    const bool from_slots = first_off != nullptr;
     const uint64_t n = from_slots ? 2 * counters->line_pixels : counters->kept;
     const uint32_t nparts = (uint32_t)((n + kSortTile - 1) / kSortTile);
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
@@ -747,13 +828,18 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
             if (d < RADIX) {
                 s_dexcl[d] = ex;
                 ex += tots[k];
-                st_release32(status + (uint64_t)part * RADIX + d, (part == 0 ? kSt32Inc : kSt32Agg) | tots[k]);
+                if (!from_slots)
+                    st_release32(status + (uint64_t)part * RADIX + d, (part == 0 ? kSt32Inc : kSt32Agg) | tots[k]);
             }
         }
         // decoupled look-back per digit
         for (uint32_t k = 0; k < DPT; ++k) {
             uint32_t d = threadIdx.x * DPT + k;
             if (d >= RADIX) continue;
+            if (from_slots) {
+                s_glob[d] = first_off[(uint64_t)d * nparts + part] - s_dexcl[d];
+                continue;
+            }
             // predecessors read 8 at a time: one round trip per 8 steps
             uint32_t excl = 0, spins = 0, nap = 1;
             int64_t p = (int64_t)part - 1;
@@ -801,13 +887,13 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
 }
 
 template __global__ void k_sort_pass<4>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
 template __global__ void k_sort_pass<6>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
 template __global__ void k_sort_pass<8>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
 template __global__ void k_sort_pass<9>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, uint32_t);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
 
 // ------------------------------------------------------------------ paint
 //

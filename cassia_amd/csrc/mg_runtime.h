@@ -100,6 +100,7 @@ private:
     DevBuf row_count_, row_start_, row_scene_, row_status_;
     DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
+    DevBuf cnt0_, cnt_acc_, cnt_status_;
     DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list

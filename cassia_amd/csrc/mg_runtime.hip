@@ -62,7 +62,7 @@ Renderer::~Renderer() {
                      &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &image_, &overflow_,
-                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &tile_start_, &tile_nr_,
+                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_nr_,
                      &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_};
     for (DevBuf *b : all) b->release();
     for (DevBuf &b : caches_) b.release();
@@ -351,25 +351,43 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
         return false;
     }
     fill32(hist_.as<uint32_t>(), 0, dims.passes * radix, stream_);
+    // Per-partition slot counts: pass 0 digits digit-major (scanned in place
+    // into the first pass's offsets), the rest in kCountSlicesHost slices.
+    const uint32_t count_rows = dims.passes * radix + 2, rest_rows = count_rows - radix;
+    const uint64_t n_cnt0 = (uint64_t)radix * nparts_sort;
+    const uint64_t cnt_parts = (n_cnt0 + kScanTileHost - 1) / kScanTileHost;
+    if (!cnt0_.reserve(std::max<uint64_t>(n_cnt0, 1) * 4) || !cnt_acc_.reserve((uint64_t)kCountSlicesHost * rest_rows * 4) ||
+        !cnt_status_.reserve(std::max<uint64_t>(cnt_parts, 1) * 8)) {
+        err = "out of device memory (slot counts)";
+        return false;
+    }
+    fill32(cnt_acc_.as<uint32_t>(), 0, (uint64_t)kCountSlicesHost * rest_rows, stream_);
+    SlotCounts counts{cnt0_.as<uint32_t>(), cnt_acc_.as<uint32_t>()};
+    const size_t count_lds = (size_t)count_rows * 4;
     // every line-pixel owns two word slots; the ingest path uses the same layout
     if (ingest && spec.n_in) {
         fill64((uint64_t *)&ctr->line_pixels, P, 1, stream_);
-        k_ingest<<<(uint32_t)((2 * P + 255) / 256), 256, 0, stream_>>>(spec.in_segs, spec.n_in,
-                                                                       scenes_.as<SceneDesc>(), dims, ctr,
-                                                                       segs_a_.as<uint64_t>());
+        k_ingest<<<(uint32_t)nparts_sort, 256, count_lds, stream_>>>(spec.in_segs, spec.n_in, scenes_.as<SceneDesc>(),
+                                                                   dims, ctr, segs_a_.as<uint64_t>(), counts);
     } else if (P) {
         if (!part_line_.reserve(nparts_emit * 4)) { err = "out of device memory"; return false; }
         const uint32_t *lens = order_lines ? lengths_perm_.as<uint32_t>() : lengths_.as<uint32_t>();
         k_part_lines<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(offsets_.as<uint32_t>(), lens, L,
                                                                       kEmitTileHost, part_line_.as<uint32_t>());
         const LineSetup *setup = order_lines ? setup_perm_.as<LineSetup>() : setup_.as<LineSetup>();
-        k_emit<<<(uint32_t)nparts_emit, 256, 0, stream_>>>(setup, offsets_.as<uint32_t>(), scenes_.as<SceneDesc>(),
-                                                         dims, ctr, segs_a_.as<uint64_t>(), part_line_.as<uint32_t>());
+        k_emit<<<(uint32_t)nparts_emit, 256, count_lds, stream_>>>(setup, offsets_.as<uint32_t>(),
+                                                                 scenes_.as<SceneDesc>(), dims, ctr,
+                                                                 segs_a_.as<uint64_t>(), part_line_.as<uint32_t>(),
+                                                                 counts);
     }
     MG_HIP(hipGetLastError(), "emit launch");
     MG_HIP(hipEventRecord(evs_[2], stream_), "event");
-    k_histogram<<<(uint32_t)std::max<uint64_t>(1, std::min<uint64_t>((cap_segs + 2047) / 2048, (uint64_t)cus_ * 4)),
-                  256, 0, stream_>>>(segs_a_.as<uint64_t>(), ctr, dims, hist_.as<uint32_t>());
+    if (P) {
+        k_count_reduce<<<rest_rows, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), dims, hist_.as<uint32_t>(), ctr);
+        fill64(cnt_status_.as<uint64_t>(), 0, cnt_parts, stream_);
+        k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
+                                                           cnt_status_.as<uint64_t>(), stall, &ctr->order_total);
+    }
     k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
     // look-back status words of every digit pass, cleared outside the sort's timing
     fill32(sort_status_.as<uint32_t>(), 0, nparts_sort * radix * dims.passes, stream_);
@@ -380,7 +398,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     for (uint32_t p = 0; p < dims.passes; ++p) {
         const uint32_t shift = dims.sort_shift + p * R;
         uint32_t *status = sort_status_.as<uint32_t>() + (size_t)p * nparts_sort * radix;
-        const uint32_t from_slots = p == 0 ? 1u : 0u;
+        const uint32_t *from_slots = p == 0 ? cnt0_.as<uint32_t>() : nullptr;
         if (R == 9)
             k_sort_pass<9><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
                                                             status, stall, from_slots);
