@@ -336,10 +336,10 @@ __device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, u
 }
 
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortKPT = 16;
-constexpr uint32_t kSortTile = kSortThreads * kSortKPT;  // 4096 keys
+constexpr uint32_t kSortKPT = 8;
+constexpr uint32_t kSortTile = kSortThreads * kSortKPT;  // keys per sort partition
 constexpr uint32_t kEmitThreads = 256;
-constexpr uint32_t kEmitIPT = 8;
+constexpr uint32_t kEmitIPT = kSortKPT / 2;
 constexpr uint32_t kEmitTile = kEmitThreads * kEmitIPT;
 static_assert(2 * kEmitTile == kSortTile, "an emission partition fills exactly one sort partition");
 constexpr uint32_t kMaxPasses = 8;
@@ -750,8 +750,8 @@ constexpr uint32_t kSt32Agg = 1u << 30;
 constexpr uint32_t kSt32Inc = 2u << 30;
 constexpr uint32_t kSt32Val = (1u << 30) - 1u;
 
-// One stable LSD digit pass. Each wave ranks its 1024 keys in 16 steps of
-// 64 (striped, so step-major order equals input order) with ballot-based
+// One stable LSD digit pass. Each wave ranks its 64 * kSortKPT keys in
+// kSortKPT steps of 64 (striped, so step-major order equals input order) with ballot-based
 // peer matching; the block's digit counts are published for decoupled
 // look-back; keys are reordered in LDS and written as digit runs.
 template <int R>
@@ -782,7 +782,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
         uint64_t keys[kSortKPT];
         uint32_t rank[kSortKPT];
         for (uint32_t i = 0; i < kSortKPT; ++i) {
-            uint32_t li = w * 1024 + i * 64 + lane;
+            uint32_t li = w * (64 * kSortKPT) + i * 64 + lane;
             bool valid = li < valid_n;
             uint64_t key = valid ? in[base + li] : 0ull;
             if (key >> 63) {
@@ -1174,43 +1174,47 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
         while (x <= h0 && nr_l[x] == 0) ++x;
         return x;
     };
-    // Runs of the next column are prefetched into registers while the
-    // current column is merged (up to kPrefetch per thread), then written to
-    // LDS after the column's last barrier.
-    constexpr uint32_t kPrefetch = 2;
-    uint32_t pf_key[kPrefetch], pf_beg[kPrefetch];
-    uint64_t pf_cov[kPrefetch];
-    uint32_t pf_n = 0;  // runs of `colp` already in LDS
-    for (uint32_t colp = nr_l[0] ? 0u : next_colp(0); colp <= h1; colp = next_colp(colp)) {
+    // Runs are prefetched into registers two columns ahead (one run per
+    // thread, two register sets taking turns), so their loads overlap a
+    // whole column's merge; runs beyond kWalkThreads are loaded when used.
+    struct Prefetch {
+        uint32_t key, beg, n;
+        uint64_t cov;
+    };
+    auto issue = [&](Prefetch &f, uint32_t col) {
+        f.n = 0;
+        if (col > h1) return;
+        const uint32_t nr2 = nr_l[col], ts2 = ts_l[col];
+        f.n = nr2 < kWalkThreads ? nr2 : kWalkThreads;
+        if (f.n > rcap) f.n = rcap;
+        if (tid < f.n) {
+            f.key = pp.run_key[ts2 + tid];
+            f.beg = pp.run_beg[ts2 + tid];
+            f.cov = pp.run_cov[ts2 + tid];
+        }
+    };
+    // One column: the cover queue entering tile colp - 1 meets its runs.
+    // Returns false when the work group must stop (capacity exceeded).
+    auto step = [&](uint32_t colp, Prefetch &f, uint32_t col_ahead) -> bool {
         const bool fold = colp <= h0;
         const uint32_t nr = nr_l[colp];
         if (nr > rcap || nq + nr > cap) {
             if (tid == 0) atomicOr(pp.overflow, 1u);
-            return;
+            return false;
         }
         const uint32_t ts = ts_l[colp], te = ts_l[colp + 1];
-        for (uint32_t k = pf_n + tid; k < nr; k += kWalkThreads) {
+        if (tid < f.n) {
+            rk[tid] = f.key;
+            rb[tid] = f.beg;
+            rc[tid] = f.cov;
+        }
+        for (uint32_t k = f.n + tid; k < nr; k += kWalkThreads) {
             rk[k] = pp.run_key[ts + k];
             rb[k] = pp.run_beg[ts + k];
             rc[k] = pp.run_cov[ts + k];
         }
+        issue(f, col_ahead);
         __syncthreads();
-        const uint32_t ncol = next_colp(colp);
-        uint32_t pf_next = 0;
-        if (ncol <= h1) {
-            const uint32_t nr2 = nr_l[ncol], ts2 = ts_l[ncol];
-            pf_next = nr2 < kPrefetch * kWalkThreads ? nr2 : kPrefetch * kWalkThreads;
-            if (pf_next > rcap) pf_next = rcap;
-#pragma unroll
-            for (uint32_t j = 0; j < kPrefetch; ++j) {
-                const uint32_t k = tid + kWalkThreads * j;
-                if (k < pf_next) {
-                    pf_key[j] = pp.run_key[ts2 + k];
-                    pf_beg[j] = pp.run_beg[ts2 + k];
-                    pf_cov[j] = pp.run_cov[ts2 + k];
-                }
-            }
-        }
         const uint32_t *qkc = qk[cur];
         const uint64_t *qcc = qc[cur];
         const uint32_t total = nq + nr;
@@ -1308,18 +1312,22 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
             atomicAdd(pp.n_records, n_union);
         }
         __syncthreads();
-#pragma unroll
-        for (uint32_t j = 0; j < kPrefetch; ++j) {
-            const uint32_t k = tid + kWalkThreads * j;
-            if (k < pf_next) {
-                rk[k] = pf_key[j];
-                rb[k] = pf_beg[j];
-                rc[k] = pf_cov[j];
-            }
-        }
-        pf_n = pf_next;
         nq = nq_new;
         cur = nxt;
+        return true;
+    };
+    uint32_t c0 = nr_l[0] ? 0u : next_colp(0), c1 = next_colp(c0), c2 = next_colp(c1);
+    Prefetch A, B;
+    issue(A, c0);
+    issue(B, c1);
+    while (c0 <= h1) {
+        if (!step(c0, A, c2)) return;
+        const uint32_t c3 = next_colp(c2);
+        if (c1 > h1) break;
+        if (!step(c1, B, c3)) return;
+        c0 = c2;
+        c1 = c3;
+        c2 = next_colp(c3);
     }
 }
 
