@@ -9,6 +9,14 @@ scan, pixel-segment emission, the LSD radix sort and the tile-row paint into
 an RGBA8 frame. Lines, layer tables and the frame stay resident in HBM; no
 host<->device copy is inside the timed region.
 
+Batched workload (the "batched" object of the JSON line): 4,096 independent
+glyph scenes of power-law canvas sizes 32..512 px, 1-200 paths of 2-6
+quadratic Beziers flattened to 8-32 lines each (cassia_amd.workloads.
+glyph_batch). Plain = one mg_render_device frame per scene; batched = the
+same 4,096 compositions as one mg_batch_render frame, with the autotuner's
+sort radix. Every batched frame is compared byte for byte with its plain
+render.
+
 Prints one JSON line (see the keys in `main`).
 """
 import argparse
@@ -72,6 +80,89 @@ def cpu_baseline(scene, fraction, threads):
                 seconds=dt, frames_per_s=1.0 / dt)
 
 
+def progress(msg):
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes):
+    """Plain per-scene frames vs one batched frame over the same compositions."""
+    t = time.perf_counter()
+    scenes = workloads.glyph_batch(n_scenes=n_scenes)
+    comps = [s.build(cassia_amd) for s in scenes]
+    t_build = time.perf_counter() - t
+    progress(f"{n_scenes} glyph scenes built in {t_build:.1f} s")
+    frames = [(s.width, s.height, s.background) for s in scenes]
+    # plain: one frame per scene (a first pass sizes each composition's buffers)
+    for c, (w, h, bg) in zip(comps, frames):
+        c.render_device(w, h, bg)
+    plain_dev_ms = 0.0
+    plain_segs = 0
+    t = time.perf_counter()
+    for c, (w, h, bg) in zip(comps, frames):
+        c.render_device(w, h, bg)
+    plain_s = time.perf_counter() - t
+    progress(f"plain: {n_scenes} frames in {plain_s:.2f} s")
+    for c in comps:
+        st = c.stats()
+        plain_dev_ms += st.device_ms
+        plain_segs += st.segments
+    # batched: the first render runs the autotuner
+    b = cassia_amd.Batch()
+    t = time.perf_counter()
+    b.render(comps, frames)
+    tune_s = time.perf_counter() - t
+    tn = b.tuning()
+    progress(f"autotuned in {tune_s:.2f} s: radix {tn.chosen_radix_bits}")
+    for _ in range(warmup):
+        b.render(comps, frames)
+    lat = []
+    dev_ms = 0.0
+    t = time.perf_counter()
+    for _ in range(steps):
+        a = time.perf_counter()
+        b.render(comps, frames)
+        lat.append((time.perf_counter() - a) * 1e3)
+        dev_ms += b.stats().device_ms
+    batch_s = (time.perf_counter() - t) / steps
+    st = b.stats()
+    progress(f"batched: {batch_s * 1e3:.2f} ms per batch")
+    # bit-identity: every batched frame against its plain render
+    same = 0
+    for i, (c, (w, h, bg)) in enumerate(zip(comps, frames)):
+        hb = np.zeros((w * h, 4), np.uint8)
+        c.render(cassia_amd.Buffer(hb, w), bg, None)
+        same += int(np.array_equal(hb, b.read(i)))
+    n = len(scenes)
+    return {
+        "workload": (f"{n} glyph scenes, canvas round(32*16^(u^3)) px square, 1-200 paths of 2-6 quadratic "
+                     "Beziers x 8-32 lines, EvenOdd, opaque solid, Over"),
+        "scenes": n,
+        "lines": int(st.lines),
+        "pixels": int(sum(w * h for w, h, _ in frames)),
+        "pixel_segments": int(st.segments),
+        "plain_frames_per_s": n / plain_s,
+        "plain_ms_per_batch": plain_s * 1e3,
+        "plain_device_ms_per_batch": plain_dev_ms,
+        "plain_pixel_segments_per_s": plain_segs / plain_s,
+        "batched_frames_per_s": n / batch_s,
+        "batched_ms_per_batch": batch_s * 1e3,
+        "batched_device_ms": dev_ms / steps,
+        "batched_pixel_segments_per_s": st.segments / batch_s,
+        "batched_p50_ms": float(np.percentile(lat, 50)),
+        "batched_p99_ms": float(np.percentile(lat, 99)),
+        "speedup_frames_per_s": plain_s / batch_s,
+        "bit_identical_frames": same,
+        "bit_identical": same == n,
+        "radix_bits": int(st.radix_bits),
+        "sort_passes": int(st.sort_passes),
+        "autotune": {"candidates": {int(tn.radix_bits[k]): round(tn.device_ms[k], 3) for k in range(tn.n_candidates)},
+                     "chosen": int(tn.chosen_radix_bits), "seconds": round(tune_s, 3)},
+        "stage_ms": {k: round(getattr(st, k), 3) for k in ("setup_ms", "emit_ms", "hist_ms", "sort_ms", "runs_ms",
+                                                             "walk_ms", "paint_ms")},
+        "scene_build_s": round(t_build, 2),
+    }
+
+
 def measured_traffic(workload):
     """HBM bytes per launch per kernel from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
@@ -108,6 +199,8 @@ def main():
     ap.add_argument("--cpu-fraction", type=float, default=0.25)
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scenes measurement")
+    ap.add_argument("--batch-scenes", type=int, default=4096)
     ap.add_argument("--radix", type=int, default=0, help="sort digit width (0 = automatic)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only copy a device buffer of this many bytes (HBM counter calibration)")
@@ -136,6 +229,7 @@ def main():
     t_build = time.perf_counter()
     comp = scene.build(cassia_amd)
     t_build = time.perf_counter() - t_build
+    progress(f"{scene.name} built in {t_build:.1f} s")
     W, H = scene.width, scene.height
     comp.set_tuning(args.radix)
 
@@ -154,6 +248,7 @@ def main():
             kms[k] += getattr(st, f)
         dev_ms += st.device_ms
     elapsed = time.perf_counter() - t0
+    progress(f"{args.steps} frames in {elapsed:.2f} s")
     st = comp.stats()
     K = args.steps
     kms = {k: v / K for k, v in kms.items()}
@@ -215,6 +310,9 @@ def main():
         comp.render(cassia_amd.Buffer(hb, W), scene.background, None)
         host_ms.append((time.perf_counter() - a) * 1e3)
     out["host_buffer_ms_per_frame"] = round(float(np.median(host_ms)), 3)
+    if not args.no_batch:
+        del comp
+        out["batched"] = bench_batched(cassia_amd, workloads, args.steps, args.warmup, args.batch_scenes)
     if not args.no_cpu:
         cb = cpu_baseline(scene, args.cpu_fraction, args.cpu_threads)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -72,6 +72,36 @@ class FrameStats(C.Structure):
                 ("runs", C.c_uint64), ("records", C.c_uint64)]
 
 
+def _props_dtype():
+    """numpy record type with the layout of mg_props (bulk props arrays)."""
+    names, formats, offsets = [], [], []
+    for name, _t in _Props._fields_:
+        names.append(name)
+        offsets.append(getattr(_Props, name).offset)
+        if name == "stops":
+            formats.append(np.uint64)
+        elif name == "color":
+            formats.append((np.float32, 4))
+        elif name in ("gradient_start", "gradient_end"):
+            formats.append((np.float32, 2))
+        else:
+            formats.append(np.uint32)
+    return np.dtype(dict(names=names, formats=formats, offsets=offsets, itemsize=C.sizeof(_Props)))
+
+
+_PROPS_DTYPE = _props_dtype()
+
+
+class _BatchFrame(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("background", C.c_float * 4),
+                ("has_crop", C.c_int32), ("crop", _Rect)]
+
+
+class BatchTuning(C.Structure):
+    _fields_ = [("n_candidates", C.c_uint32), ("radix_bits", C.c_uint32 * 4), ("device_ms", C.c_double * 4),
+                ("chosen_radix_bits", C.c_uint32)]
+
+
 class CassiaStyling(C.Structure):
     _fields_ = [("fill", C.c_float * 4), ("fillRule", C.c_uint32), ("blendMode", C.c_uint32),
                 ("_padding", C.c_uint32 * 2)]
@@ -112,6 +142,8 @@ _sig("mg_layer_order", C.c_int, _vp, C.c_uint16, C.POINTER(C.c_uint16))
 _sig("mg_layer_set_order", C.c_int, _vp, C.c_uint16, C.c_uint16)
 _sig("mg_layer_props", C.c_int, _vp, C.c_uint16, C.POINTER(_Props))
 _sig("mg_layer_set_props", C.c_int, _vp, C.c_uint16, C.POINTER(_Props))
+_sig("mg_create_layers", C.c_int, _vp, C.c_size_t, C.POINTER(C.c_uint16))
+_sig("mg_layers_set_props", C.c_int, _vp, C.POINTER(C.c_uint16), _vp, C.c_size_t)
 _sig("mg_remove_disabled", C.c_int, _vp)
 _sig("mg_lines_len", C.c_int, _vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t))
 _sig("mg_create_buffer_layer_cache", C.c_int, _vp, C.POINTER(C.c_int32))
@@ -124,6 +156,14 @@ _sig("mg_render_coverage", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_
 _sig("mg_set_tuning", C.c_int, _vp, C.c_uint32)
 _sig("mg_pixel_segments", C.c_int, _vp, C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t))
 _sig("mg_last_frame_stats", C.c_int, _vp, C.POINTER(FrameStats))
+_sig("mg_batch_create", C.c_int, C.POINTER(_vp))
+_sig("mg_batch_destroy", None, _vp)
+_sig("mg_batch_render", C.c_int, _vp, C.POINTER(_vp), C.POINTER(_BatchFrame), C.c_size_t, C.POINTER(_vp),
+     C.POINTER(C.POINTER(C.c_uint64)))
+_sig("mg_batch_read", C.c_int, _vp, C.c_size_t, C.POINTER(C.c_uint8), C.c_size_t)
+_sig("mg_batch_set_tuning", C.c_int, _vp, C.c_uint32)
+_sig("mg_batch_last_tuning", C.c_int, _vp, C.POINTER(BatchTuning))
+_sig("mg_batch_stats", C.c_int, _vp, C.POINTER(FrameStats))
 _sig("cassia_init", None, C.c_uint32, C.c_uint32)
 _sig("cassia_render", None, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(CassiaStyling), C.c_size_t)
 _sig("cassia_shutdown", None)
@@ -403,7 +443,10 @@ class Layer:
         return self
 
     def props(self):
-        return self._c._props.get(self.id, Props())
+        p = self._c._props.get(self.id)
+        if p is None:
+            p = self._c._bulk_props(self.id)
+        return p if p is not None else Props()
 
     def set_props(self, props):
         keep = []
@@ -460,6 +503,7 @@ class Composition:
         _check(_lib.mg_composition_create(C.byref(h)))
         self._h = h
         self._props = {}
+        self._bulk = []  # (id -> row, colors, rules, blends) of set_solid_props calls, latest last
 
     def __del__(self):
         if getattr(self, "_h", None):
@@ -493,6 +537,35 @@ class Composition:
         ids = np.ascontiguousarray(np.asarray(layer_ids, dtype=np.uint16).reshape(-1))
         assert len(ids) == len(a)
         _check(_lib.mg_insert_lines_multi(self._h, ids.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(a), len(a)))
+
+    def create_layers(self, n):
+        """n create_layer calls in one: returns the ids (uint16 array)."""
+        out = np.zeros(n, np.uint16)
+        _check(_lib.mg_create_layers(self._h, n, out.ctypes.data_as(C.POINTER(C.c_uint16))))
+        return out
+
+    def set_solid_props(self, layer_ids, colors, fill_rules, blend_modes):
+        """Layer::set_props on many layers with Draw(Style{fill: Solid(colors[i]),
+        blend_mode}) props and fill rule `fill_rules[i]`, in one call."""
+        ids = np.ascontiguousarray(np.asarray(layer_ids, dtype=np.uint16).reshape(-1))
+        n = len(ids)
+        a = np.zeros(n, _PROPS_DTYPE)
+        a["fill_rule"] = np.broadcast_to(np.asarray(fill_rules, np.uint32), (n,))
+        a["blend_mode"] = np.broadcast_to(np.asarray(blend_modes, np.uint32), (n,))
+        a["color"] = np.asarray(colors, np.float32).reshape(n, 4)
+        _check(_lib.mg_layers_set_props(self._h, ids.ctypes.data_as(C.POINTER(C.c_uint16)), a.ctypes.data, n))
+        for i in ids.tolist():
+            self._props.pop(i, None)
+        self._bulk.append(({int(v): k for k, v in enumerate(ids.tolist())}, a))
+
+    def _bulk_props(self, layer_id):
+        for rows, a in reversed(self._bulk):
+            k = rows.get(int(layer_id))
+            if k is not None:
+                r = a[k]
+                style = Style(fill=Fill.Solid([float(v) for v in r["color"]]), blend_mode=BlendMode(int(r["blend_mode"])))
+                return Props(fill_rule=FillRule(int(r["fill_rule"])), func=Func.Draw(style))
+        return None
 
     def get(self, layer_id):
         v = C.c_int(0)
@@ -588,6 +661,76 @@ class Composition:
     def stats(self):
         s = FrameStats()
         _check(_lib.mg_last_frame_stats(self._h, C.byref(s)))
+        return s
+
+
+class Batch:
+    """Many compositions rendered as one device frame (mg_batch_render): frame
+    i of a render is the bytes `comps[i].render_device(w, h, bg, crop)` gives,
+    inside the crop."""
+
+    def __init__(self):
+        h = _vp()
+        _check(_lib.mg_batch_create(C.byref(h)))
+        self._h = h
+        self._frames = []
+        self._offsets = []
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            _lib.mg_batch_destroy(self._h)
+            self._h = None
+
+    def render(self, comps, frames):
+        """`frames[i]` = (width, height, background[, crop Rect]). Returns the
+        device address of the frame buffer; `offsets` gives each frame's byte
+        offset in it."""
+        n = len(comps)
+        if n != len(frames):
+            raise ValueError("one frame per composition")
+        hs = (_vp * max(n, 1))(*[c._h for c in comps])
+        fs = (_BatchFrame * max(n, 1))()
+        for i, fr in enumerate(frames):
+            w, h, bg = fr[0], fr[1], fr[2]
+            crop = fr[3] if len(fr) > 3 else None
+            fs[i].width, fs[i].height = w, h
+            for k in range(4):
+                fs[i].background[k] = float(bg[k])
+            if crop is not None:
+                fs[i].has_crop = 1
+                fs[i].crop = _Rect(*crop.px[:2], *crop.px[2:])
+        dev = _vp()
+        offs = C.POINTER(C.c_uint64)()
+        _check(_lib.mg_batch_render(self._h, hs, fs, n, C.byref(dev), C.byref(offs)))
+        self._frames = [(fr[0], fr[1]) for fr in frames]
+        self._offsets = [offs[i] for i in range(n)]
+        return dev.value
+
+    @property
+    def offsets(self):
+        return list(self._offsets)
+
+    def read(self, i):
+        """Frame i of the last render as a (height * width, 4) uint8 array."""
+        if not 0 <= i < len(self._frames):
+            raise IndexError(f"frame {i} of a batch of {len(self._frames)}")
+        w, h = self._frames[i]
+        out = np.zeros((w * h, 4), np.uint8)
+        _check(_lib.mg_batch_read(self._h, i, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.nbytes))
+        return out
+
+    def set_tuning(self, radix_bits=0):
+        """Sort digit width (4, 6, 8 or 9), or 0 = autotune on the next render."""
+        _check(_lib.mg_batch_set_tuning(self._h, radix_bits))
+
+    def tuning(self):
+        t = BatchTuning()
+        _check(_lib.mg_batch_last_tuning(self._h, C.byref(t)))
+        return t
+
+    def stats(self):
+        s = FrameStats()
+        _check(_lib.mg_batch_stats(self._h, C.byref(s)))
         return s
 
 

@@ -3,6 +3,7 @@
 // every pixel operation runs on the GPU (mg_runtime.hip, mg_kernels.hip).
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <functional>
@@ -27,6 +28,13 @@ mg_status fail(mg_status s, const std::string &msg) {
 }
 
 const float kIdentity[6] = {1.0f, 0.0f, 0.0f, 1.0f, 0.0f, 0.0f};
+
+// Owner ids of frame tables: the device engine keeps the tables of the last
+// frame it ran and reuses them when owner and version both match.
+uint64_t next_owner_id() {
+    static std::atomic<uint64_t> n{1};
+    return n.fetch_add(1);
+}
 
 // surpass::painter::Props with an owned stop list.
 struct Props {
@@ -115,13 +123,15 @@ struct mg_composition {
     std::map<uint16_t, uint16_t> orders_to_layers;
     std::map<std::pair<uintptr_t, size_t>, std::pair<size_t, size_t>> layouts;
     uint32_t caches = 0;
-    std::unique_ptr<mg::Renderer> rdr;
+    std::shared_ptr<mg::Renderer> rdr;  // the device's shared frame engine
+    mg::LineStore dlines;               // resident lines
+    mg::LayerCaches dcaches;            // BufferLayerCache arrays
+    mg::DevBuf frame;                   // device frame of the last render
+    const uint64_t owner = next_owner_id();
     std::vector<uint64_t> pixsegs;
     mg_frame_stats stats{};
     uint64_t version = 1;      // bumped by every change of layers, props or lines
     uint32_t radix_bits = 0;
-    uint8_t *device_frame = nullptr;
-    size_t device_frame_bytes = 0;
     // frame tables (layers by id, props by order) of the last render, valid
     // while `version` and the layer cache id are unchanged
     mg::FrameSpec spec;
@@ -166,24 +176,41 @@ struct mg_composition {
         }
     }
 
+    ~mg_composition() {
+        if (!rdr) return;
+        std::lock_guard<std::mutex> g(rdr->mu);
+        (void)hipSetDevice(rdr->device());
+        (void)hipStreamSynchronize(rdr->stream());
+        dlines.release();
+        dcaches.release_all();
+        frame.release();
+    }
+
     mg_status ensure_renderer() {
         if (rdr) return MG_OK;
-        auto r = std::make_unique<mg::Renderer>();
         std::string err;
-        if (!r->init(err)) return fail(MG_ERR_DEVICE, err);
-        rdr = std::move(r);
+        rdr = mg::Renderer::shared(err);
+        if (!rdr) return fail(MG_ERR_DEVICE, err);
         return MG_OK;
     }
 
+    // Called with the engine locked.
     mg_status sync_lines() {
         std::string err;
         if (uploaded > ids.size()) uploaded = 0;
         if (uploaded < ids.size()) {
-            if (!rdr->upload_lines(&pts[4 * uploaded], &ids[uploaded], uploaded, ids.size() - uploaded, err))
-                return fail(MG_ERR_OOM, err);
+            if (hipSetDevice(rdr->device()) != hipSuccess ||
+                !dlines.upload(&pts[4 * uploaded], &ids[uploaded], uploaded, ids.size() - uploaded, rdr->stream(), err))
+                return fail(MG_ERR_OOM, err.empty() ? "hipSetDevice failed" : err);
             uploaded = ids.size();
         }
         return MG_OK;
+    }
+
+    // Device frame of `bytes`, owned by the composition.
+    uint8_t *frame_image(size_t bytes) {
+        if (hipSetDevice(rdr->device()) != hipSuccess || !frame.reserve(bytes)) return nullptr;
+        return frame.as<uint8_t>();
     }
 
     void insert(uint16_t id, const float *lines, size_t n) {
@@ -458,6 +485,24 @@ MG_EXPORT mg_status mg_layer_set_props(mg_composition *c, mg_layer_id id, const 
     return MG_OK;
 }
 
+MG_EXPORT mg_status mg_create_layers(mg_composition *c, size_t n, mg_layer_id *out) {
+    if (check_comp(c) || (n && !out)) return fail(MG_ERR_INVALID_ARG, "null argument");
+    for (size_t i = 0; i < n; ++i) {
+        mg_status s = mg_create_layer(c, &out[i]);
+        if (s) return s;
+    }
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_layers_set_props(mg_composition *c, const mg_layer_id *ids, const mg_props *props, size_t n) {
+    if (check_comp(c) || (n && (!ids || !props))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    for (size_t i = 0; i < n; ++i) {
+        mg_status s = mg_layer_set_props(c, ids[i], &props[i]);
+        if (s) return s;
+    }
+    return MG_OK;
+}
+
 MG_EXPORT mg_status mg_remove_disabled(mg_composition *c) {
     if (check_comp(c)) return MG_ERR_INVALID_ARG;
     c->remove_disabled();
@@ -486,8 +531,9 @@ MG_EXPORT mg_status mg_buffer_layer_cache_clear(mg_composition *c, int32_t id) {
     if (check_comp(c) || id < 0 || id >= 32 || !((c->caches >> id) & 1u))
         return fail(MG_ERR_INVALID_ARG, "unknown cache");
     if (c->rdr) {
-        std::string err;
-        if (!c->rdr->cache_clear(id, err)) return fail(MG_ERR_DEVICE, err);
+        std::lock_guard<std::mutex> g(c->rdr->mu);
+        c->dcaches.clear(id, c->rdr->stream());
+        if (hipGetLastError() != hipSuccess) return fail(MG_ERR_DEVICE, "cache clear failed");
     }
     return MG_OK;
 }
@@ -495,51 +541,64 @@ MG_EXPORT mg_status mg_buffer_layer_cache_clear(mg_composition *c, int32_t id) {
 MG_EXPORT mg_status mg_buffer_layer_cache_release(mg_composition *c, int32_t id) {
     if (check_comp(c) || id < 0 || id >= 32) return fail(MG_ERR_INVALID_ARG, "unknown cache");
     c->caches &= ~(1u << id);
-    if (c->rdr) c->rdr->cache_release(id);
+    if (c->rdr) {
+        std::lock_guard<std::mutex> g(c->rdr->mu);
+        (void)hipStreamSynchronize(c->rdr->stream());
+        c->dcaches.release(id);
+    }
     return MG_OK;
 }
 
-// Composition::render for one frame of `width` x `rows_px` pixels at
-// `stride` into the device image `img` (composition.rs:212-328).
-static mg_status render_frame(mg_composition *c, size_t width, size_t stride, size_t rows_px, const float bg[4],
-                              const mg_rect *crop, int32_t cache_id, int32_t *prev, uint8_t *img,
-                              bool want_tile_ops, mg::FrameResult &r, uint32_t coverage_cap = 0) {
-    std::string err;
+// Frame tables of a composition (remove_disabled, orders_to_layers and the
+// layer / props tables, composition.rs:229-290), rebuilt only when the
+// composition changed: every change of lines, layers, props or cache state
+// bumps `version`.
+static void prepare_tables(mg_composition *c, int32_t cache_id) {
+    if (c->spec_version == c->version && c->spec_cache_id == cache_id) return;
+    c->remove_disabled();
+    for (auto &kv : c->layers)
+        if (kv.second.enabled && kv.second.has_order) c->orders_to_layers[kv.second.order] = kv.first;
+    c->spec = mg::FrameSpec{};
+    c->tables(cache_id, c->spec, c->spec_clips);
+    c->spec_version = c->version;
+    c->spec_cache_id = cache_id;
+}
+
+static void fill_stats(mg_frame_stats &st, const mg::FrameResult &r, size_t lines) {
+    st.lines = lines;
+    st.line_pixels = r.line_pixels;
+    st.segments = r.segments;
+    st.segments_painted = r.kept;
+    st.tiles = r.tiles;
+    st.sort_passes = r.passes;
+    st.radix_bits = r.radix_bits;
+    st.device_ms = r.device_ms;
+    st.setup_ms = r.stage_ms[0];
+    st.emit_ms = r.stage_ms[1];
+    st.hist_ms = r.stage_ms[2];
+    st.sort_ms = r.stage_ms[3];
+    st.runs_ms = r.stage_ms[4];
+    st.walk_ms = r.stage_ms[5];
+    st.paint_ms = r.stage_ms[6];
+    st.runs = r.runs;
+    st.records = r.records;
+}
+
+// Scene descriptor of one frame of `width` x `rows_px` pixels at `stride`
+// (Rect::new for the crop, buffer_layout.rs:43-50).
+static mg::SceneDesc scene_desc(size_t width, size_t stride, size_t rows_px, const float bg[4], const mg_rect *crop) {
     const size_t row_len = (width + 7) / 8, tile_rows = (rows_px + 7) / 8;
-    if (c->spec_version != c->version || c->spec_cache_id != cache_id) {
-        // every change of lines, layers, props or cache state bumps `version`
-        c->remove_disabled();
-        for (auto &kv : c->layers)
-            if (kv.second.enabled && kv.second.has_order) c->orders_to_layers[kv.second.order] = kv.first;
-        c->spec = mg::FrameSpec{};
-        c->tables(cache_id, c->spec, c->spec_clips);
-        c->spec_version = c->version;
-        c->spec_cache_id = cache_id;
-    }
-    mg::FrameSpec &f = c->spec;
-    const bool has_clips = c->spec_clips;
-    f.scenes.clear();
-    f.scene_line_base.clear();
-    f.tables_version = c->version * 64 + (uint64_t)(cache_id + 1);
     mg::SceneDesc sd{};
-    sd.line_base = 0;
-    sd.n_lines = (uint32_t)c->ids.size();
-    sd.layer_base = 0;
-    sd.n_layer_entries = (uint32_t)f.layers.size();
-    sd.props_base = 0;
-    sd.n_props = (uint32_t)f.props.size();
     sd.width = (uint32_t)width;
     sd.rows_px = (uint32_t)rows_px;
     sd.stride = (uint32_t)stride;
     sd.row_len = (uint32_t)row_len;
     sd.tile_rows = (uint32_t)tile_rows;
-    sd.row_base = 0;
     sd.crop_h0 = 0;
     sd.crop_h1 = (uint32_t)row_len;
     sd.crop_v0 = 0;
     sd.crop_v1 = (uint32_t)tile_rows;
     if (crop) {
-        // Rect::new (buffer_layout.rs:43-50): pixel ranges to tile ranges
         sd.crop_h0 = crop->x0 / 8;
         sd.crop_h1 = (uint32_t)std::min<uint64_t>(((uint64_t)crop->x1 + 7) / 8, row_len);
         sd.crop_v0 = crop->y0 / 8;
@@ -547,20 +606,45 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
         if (sd.crop_h0 > sd.crop_h1) sd.crop_h0 = sd.crop_h1;
         if (sd.crop_v0 > sd.crop_v1) sd.crop_v0 = sd.crop_v1;
     }
-    sd.cache_base = 0;
+    std::memcpy(sd.clear, bg, sizeof sd.clear);
+    return sd;
+}
+
+// Composition::render for one frame of `width` x `rows_px` pixels at
+// `stride` into the device image `img` (composition.rs:212-328).
+static mg_status render_frame(mg_composition *c, size_t width, size_t stride, size_t rows_px, const float bg[4],
+                              const mg_rect *crop, int32_t cache_id, int32_t *prev, uint8_t *img, size_t img_bytes,
+                              bool want_tile_ops, mg::FrameResult &r, uint32_t coverage_cap = 0) {
+    std::string err;
+    prepare_tables(c, cache_id);
+    mg::FrameSpec &f = c->spec;
+    const bool has_clips = c->spec_clips;
+    f.scenes.clear();
+    f.scene_line_base.clear();
+    f.tables_owner = c->owner;
+    f.tables_version = c->version * 64 + (uint64_t)(cache_id + 1);
+    mg::SceneDesc sd = scene_desc(width, stride, rows_px, bg, crop);
+    sd.line_base = 0;
+    sd.n_lines = (uint32_t)c->ids.size();
+    sd.layer_base = 0;
+    sd.n_layer_entries = (uint32_t)f.layers.size();
+    sd.props_base = 0;
+    sd.n_props = (uint32_t)f.props.size();
+    sd.row_base = 0;
+    sd.tile_base = 0;
     sd.flags = (has_clips ? mg::SF_CLIPS : 0) | (cache_id >= 0 ? mg::SF_CACHE : 0);
     sd.out_offset = 0;
-    std::memcpy(sd.clear, bg, sizeof sd.clear);
     f.scenes.push_back(sd);
     f.scene_line_base.push_back(0);
     f.mode = 0;
+    f.image_bytes = img_bytes;
     f.want_tile_ops = want_tile_ops;
     f.want_coverage = coverage_cap > 0;
     f.coverage_cap = coverage_cap;
     f.radix_bits = c->radix_bits;
     mg_status s;
     if ((s = c->sync_lines())) return s;
-    if (!c->rdr->run(f, c->ids.size(), img, prev, r, err))
+    if (!c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), img, prev, r, err))
         return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
     if (cache_id >= 0) {
         bool changed = false;
@@ -572,23 +656,7 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
         }
         if (changed) ++c->version;
     }
-    c->stats.lines = c->ids.size();
-    c->stats.line_pixels = r.line_pixels;
-    c->stats.segments = r.segments;
-    c->stats.segments_painted = r.kept;
-    c->stats.tiles = r.tiles;
-    c->stats.sort_passes = r.passes;
-    c->stats.radix_bits = r.radix_bits;
-    c->stats.device_ms = r.device_ms;
-    c->stats.setup_ms = r.stage_ms[0];
-    c->stats.emit_ms = r.stage_ms[1];
-    c->stats.hist_ms = r.stage_ms[2];
-    c->stats.sort_ms = r.stage_ms[3];
-    c->stats.runs_ms = r.stage_ms[4];
-    c->stats.walk_ms = r.stage_ms[5];
-    c->stats.paint_ms = r.stage_ms[6];
-    c->stats.runs = r.runs;
-    c->stats.records = r.records;
+    fill_stats(c->stats, r, c->ids.size());
     return MG_OK;
 }
 
@@ -596,6 +664,7 @@ MG_EXPORT mg_status mg_render(mg_composition *c, const mg_buffer *buf, const flo
     if (check_comp(c) || !buf || !bg || !buf->rgba) return fail(MG_ERR_INVALID_ARG, "null argument");
     mg_status s = c->ensure_renderer();
     if (s) return s;
+    std::lock_guard<std::mutex> g(c->rdr->mu);
     std::string err;
     const int32_t cache_id = buf->layer_cache;
     if (cache_id >= 0 && (cache_id >= 32 || !((c->caches >> cache_id) & 1u)))
@@ -620,17 +689,17 @@ MG_EXPORT mg_status mg_render(mg_composition *c, const mg_buffer *buf, const flo
         const size_t tiles_len = ((buf->width + 7) / 8) * (((len / bw) + 7) / 8);
         if (tiles_len != row_len * tile_rows) return fail(MG_ERR_BUFFER, "layer cache size mismatch");
         bool resized = false;
-        prev = c->rdr->cache_array(cache_id, tiles_len, resized, err);
+        prev = c->dcaches.array(cache_id, tiles_len, c->rdr->stream(), resized, err);
         if (!prev) return fail(MG_ERR_OOM, err);
     }
     const size_t bytes = len * 4;
-    uint8_t *img = c->rdr->frame_image(bytes, err);
-    if (!img) return fail(MG_ERR_OOM, err);
+    uint8_t *img = c->frame_image(bytes);
+    if (!img) return fail(MG_ERR_OOM, "out of device memory (frame)");
     // the caller's pixels go first: tiles the painter skips keep them
     if (hipMemcpyAsync(img, buf->rgba, bytes, hipMemcpyHostToDevice, c->rdr->stream()) != hipSuccess)
         return fail(MG_ERR_DEVICE, "upload of the frame buffer failed");
     mg::FrameResult r;
-    if ((s = render_frame(c, width, stride, rows_px, bg, crop, cache_id, prev, img, buf->flusher != nullptr, r)))
+    if ((s = render_frame(c, width, stride, rows_px, bg, crop, cache_id, prev, img, bytes, buf->flusher != nullptr, r)))
         return s;
     if (hipMemcpyAsync(buf->rgba, img, bytes, hipMemcpyDeviceToHost, c->rdr->stream()) != hipSuccess ||
         hipStreamSynchronize(c->rdr->stream()) != hipSuccess)
@@ -656,12 +725,13 @@ MG_EXPORT mg_status mg_render_device(mg_composition *c, uint32_t width, uint32_t
         return fail(MG_ERR_INVALID_ARG, "bad argument");
     mg_status s = c->ensure_renderer();
     if (s) return s;
+    std::lock_guard<std::mutex> g(c->rdr->mu);
     std::string err;
     const size_t bytes = (size_t)width * height * 4;
-    uint8_t *img = c->rdr->frame_image(bytes, err);
-    if (!img) return fail(MG_ERR_OOM, err);
+    uint8_t *img = c->frame_image(bytes);
+    if (!img) return fail(MG_ERR_OOM, "out of device memory (frame)");
     mg::FrameResult r;
-    if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, false, r))) return s;
+    if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, bytes, false, r))) return s;
     *device_rgba = img;
     return MG_OK;
 }
@@ -674,12 +744,13 @@ MG_EXPORT mg_status mg_render_coverage(mg_composition *c, uint32_t width, uint32
     if (cap > 0xFFFFFFFFull) return fail(MG_ERR_INVALID_ARG, "cap too large");
     mg_status s = c->ensure_renderer();
     if (s) return s;
+    std::lock_guard<std::mutex> g(c->rdr->mu);
     std::string err;
     const size_t bytes = (size_t)width * height * 4;
-    uint8_t *img = c->rdr->frame_image(bytes, err);
-    if (!img) return fail(MG_ERR_OOM, err);
+    uint8_t *img = c->frame_image(bytes);
+    if (!img) return fail(MG_ERR_OOM, "out of device memory (frame)");
     mg::FrameResult r;
-    if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, false, r,
+    if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, bytes, false, r,
                           (uint32_t)std::max<size_t>(cap, 1))))
         return s;
     std::vector<int32_t> a;
@@ -699,6 +770,7 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
     if (check_comp(c) || !out || !n) return fail(MG_ERR_INVALID_ARG, "null argument");
     mg_status s = c->ensure_renderer();
     if (s) return s;
+    std::lock_guard<std::mutex> g(c->rdr->mu);
     c->remove_disabled();
     mg::FrameSpec f;
     bool has_clips = false;
@@ -714,7 +786,7 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
     if ((s = c->sync_lines())) return s;
     std::string err;
     mg::FrameResult r;
-    if (!c->rdr->run(f, c->ids.size(), nullptr, nullptr, r, err)) return fail(MG_ERR_DEVICE, err);
+    if (!c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), nullptr, nullptr, r, err)) return fail(MG_ERR_DEVICE, err);
     c->pixsegs.assign(2 * r.line_pixels, mg::kNone);
     if (!c->rdr->download_sorted(c->pixsegs.data(), r.kept, err)) return fail(MG_ERR_DEVICE, err);
     *out = c->pixsegs.data();
@@ -736,6 +808,247 @@ MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *c, mg_frame_stats 
     return MG_OK;
 }
 
+// ------------------------------------------------------------------ batches
+
+}  // extern "C"
+
+struct mg_batch {
+    std::shared_ptr<mg::Renderer> rdr;
+    mg::LineStore dlines;  // lines of all scenes, scene after scene
+    mg::DevBuf frame;      // all frames, 256-byte aligned
+    const uint64_t owner = next_owner_id();
+    // what the resident lines and tables were packed from
+    std::vector<uint64_t> owners, versions;
+    std::vector<mg_batch_frame> packed_frames;
+    uint64_t tables_version = 0;
+    mg::FrameSpec spec;
+    size_t n_lines = 0;
+    std::vector<uint64_t> offsets, bytes;
+    uint32_t radix_bits = 0;  // 0 = autotune
+    // autotuner: the choice holds for batches with the same sort width
+    uint32_t tuned_bits = 0, tuned_for = ~0u;
+    mg_batch_tuning tuning{};
+    mg_frame_stats stats{};
+
+    ~mg_batch() {
+        if (!rdr) return;
+        std::lock_guard<std::mutex> g(rdr->mu);
+        (void)hipSetDevice(rdr->device());
+        (void)hipStreamSynchronize(rdr->stream());
+        dlines.release();
+        frame.release();
+    }
+};
+
+namespace {
+
+bool same_frame(const mg_batch_frame &a, const mg_batch_frame &b) {
+    return std::memcmp(&a, &b, sizeof a) == 0;
+}
+
+// Packs the compositions into one frame: scene descriptors with their line,
+// layer, props, tile-row and output bases, concatenated tables and (when a
+// composition changed) the concatenated resident lines.
+mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_frame *frames, size_t n) {
+    bool same = b->owners.size() == n;
+    for (size_t i = 0; i < n; ++i) {
+        prepare_tables(comps[i], -1);
+        same = same && b->owners[i] == comps[i]->owner && b->versions[i] == comps[i]->version;
+    }
+    bool same_frames = same && b->packed_frames.size() == n;
+    for (size_t i = 0; same_frames && i < n; ++i) same_frames = same_frame(b->packed_frames[i], frames[i]);
+    if (same_frames) return MG_OK;
+    if (!same) {
+        // concatenated lines, resident until a composition changes
+        size_t total = 0;
+        for (size_t i = 0; i < n; ++i) total += comps[i]->ids.size();
+        if (total > 0xFFFFFFFFull) return fail(MG_ERR_CAPACITY, "a batch holds at most 2^32 lines");
+        std::vector<float> pts(4 * total);
+        std::vector<uint16_t> ids(total);
+        size_t at = 0;
+        for (size_t i = 0; i < n; ++i) {
+            const mg_composition *c = comps[i];
+            if (!c->ids.empty()) {
+                std::memcpy(&pts[4 * at], c->pts.data(), c->ids.size() * 16);
+                std::memcpy(&ids[at], c->ids.data(), c->ids.size() * 2);
+            }
+            at += c->ids.size();
+        }
+        std::string err;
+        if (hipSetDevice(b->rdr->device()) != hipSuccess ||
+            !b->dlines.upload(pts.data(), ids.data(), 0, total, b->rdr->stream(), err))
+            return fail(MG_ERR_OOM, err.empty() ? "hipSetDevice failed" : err);
+        b->n_lines = total;
+        b->owners.resize(n);
+        b->versions.resize(n);
+        for (size_t i = 0; i < n; ++i) {
+            b->owners[i] = comps[i]->owner;
+            b->versions[i] = comps[i]->version;
+        }
+    }
+    mg::FrameSpec &f = b->spec;
+    f = mg::FrameSpec{};
+    f.scenes.resize(n);
+    f.scene_line_base.resize(n);
+    b->offsets.resize(n);
+    b->bytes.resize(n);
+    uint64_t line_base = 0, row_base = 0, tile_base = 0, out = 0;
+    for (size_t i = 0; i < n; ++i) {
+        const mg_composition *c = comps[i];
+        const mg::FrameSpec &cs = c->spec;
+        const mg_batch_frame &fr = frames[i];
+        mg::SceneDesc sd = scene_desc(fr.width, fr.width, fr.height, fr.background, fr.has_crop ? &fr.crop : nullptr);
+        sd.line_base = (uint32_t)line_base;
+        sd.n_lines = (uint32_t)c->ids.size();
+        sd.layer_base = (uint32_t)f.layers.size();
+        sd.n_layer_entries = (uint32_t)cs.layers.size();
+        sd.props_base = (uint32_t)f.props.size();
+        sd.n_props = (uint32_t)cs.props.size();
+        sd.row_base = (uint32_t)row_base;
+        sd.tile_base = (uint32_t)tile_base;
+        sd.flags = c->spec_clips ? mg::SF_CLIPS : 0;
+        sd.out_offset = out;
+        const uint32_t stops_base = (uint32_t)f.stops.size();
+        f.layers.insert(f.layers.end(), cs.layers.begin(), cs.layers.end());
+        for (mg::PropsEntry e : cs.props) {
+            e.stops_offset += stops_base;
+            f.props.push_back(e);
+        }
+        f.stops.insert(f.stops.end(), cs.stops.begin(), cs.stops.end());
+        f.max_order = std::max(f.max_order, cs.max_order);
+        f.scenes[i] = sd;
+        f.scene_line_base[i] = (uint32_t)line_base;
+        b->offsets[i] = out;
+        b->bytes[i] = (uint64_t)fr.width * fr.height * 4;
+        line_base += c->ids.size();
+        row_base += sd.tile_rows;
+        tile_base += (uint64_t)sd.tile_rows * sd.row_len;
+        out += (b->bytes[i] + 255) & ~255ull;
+    }
+    if (row_base > 0xFFFFFFFFull || tile_base > 0xFFFFFFFFull || f.layers.size() > 0xFFFFFFFFull ||
+        f.props.size() > 0xFFFFFFFFull)
+        return fail(MG_ERR_CAPACITY, "batch too large");
+    if (!b->frame.reserve(std::max<uint64_t>(out, 256))) return fail(MG_ERR_OOM, "out of device memory (frames)");
+    f.mode = 0;
+    f.image_bytes = out;
+    f.tables_owner = b->owner;
+    f.tables_version = ++b->tables_version;
+    b->packed_frames.assign(frames, frames + n);
+    return MG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+MG_EXPORT mg_status mg_batch_create(mg_batch **out) {
+    if (!out) return fail(MG_ERR_INVALID_ARG, "null argument");
+    std::string err;
+    auto r = mg::Renderer::shared(err);
+    if (!r) return fail(MG_ERR_DEVICE, err);
+    mg_batch *b = new mg_batch();
+    b->rdr = std::move(r);
+    *out = b;
+    return MG_OK;
+}
+
+MG_EXPORT void mg_batch_destroy(mg_batch *b) { delete b; }
+
+MG_EXPORT mg_status mg_batch_set_tuning(mg_batch *b, uint32_t radix_bits) {
+    if (!b) return fail(MG_ERR_INVALID_ARG, "batch is null");
+    if (radix_bits != 0 && radix_bits != 4 && radix_bits != 6 && radix_bits != 8 && radix_bits != 9)
+        return fail(MG_ERR_INVALID_ARG, "radix_bits must be 0, 4, 6, 8 or 9");
+    b->radix_bits = radix_bits;
+    if (!radix_bits) b->tuned_for = ~0u;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_batch_render(mg_batch *b, mg_composition *const *comps, const mg_batch_frame *frames,
+                                    size_t n, void **device_rgba, const uint64_t **offsets) {
+    if (!b || (n && (!comps || !frames)) || !device_rgba || !offsets) return fail(MG_ERR_INVALID_ARG, "null argument");
+    if (n == 0 || n > 8192) return fail(MG_ERR_INVALID_ARG, "a batch holds 1 to 8192 frames");
+    for (size_t i = 0; i < n; ++i) {
+        if (!comps[i]) return fail(MG_ERR_INVALID_ARG, "composition is null");
+        if (frames[i].width == 0 || frames[i].height == 0) return fail(MG_ERR_INVALID_ARG, "empty frame");
+    }
+    std::lock_guard<std::mutex> g(b->rdr->mu);
+    mg_status s = pack_batch(b, comps, frames, n);
+    if (s) return s;
+    mg::FrameSpec &f = b->spec;
+    std::string err;
+    mg::FrameResult r;
+    auto run = [&](uint32_t bits) -> mg_status {
+        f.radix_bits = bits;
+        if (!b->rdr->run(f, b->dlines.points(), b->dlines.layer_ids(), b->n_lines, b->frame.as<uint8_t>(), nullptr,
+                         r, err))
+            return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
+        return MG_OK;
+    };
+    uint32_t bits = b->radix_bits;
+    if (!bits) {
+        // the autotuner: a choice holds for later batches with the same sort
+        // key width and the same power-of-two class of sorted words
+        if ((s = run(0))) return s;
+        uint32_t lg = 0;
+        while (lg < 63 && (r.segments >> lg)) ++lg;
+        const uint32_t key = r.sort_bits * 64 + lg;
+        if (b->tuned_for != key) {
+            static const uint32_t cand[4] = {4, 6, 8, 9};
+            mg_batch_tuning t{};
+            t.n_candidates = 4;
+            double best = 0.0;
+            uint32_t best_bits = 0;
+            for (int k = 0; k < 4; ++k) {
+                double ms = 0.0;
+                for (int rep = 0; rep < 2; ++rep) {  // the first run also sizes scratch buffers
+                    if ((s = run(cand[k]))) return s;
+                    ms = rep == 0 ? r.device_ms : std::min(ms, r.device_ms);
+                }
+                t.radix_bits[k] = cand[k];
+                t.device_ms[k] = ms;
+                if (!best_bits || ms < best) {
+                    best = ms;
+                    best_bits = cand[k];
+                }
+            }
+            t.chosen_radix_bits = best_bits;
+            b->tuning = t;
+            b->tuned_bits = best_bits;
+            b->tuned_for = key;
+        }
+        bits = b->tuned_bits;
+    }
+    if ((s = run(bits))) return s;
+    fill_stats(b->stats, r, b->n_lines);
+    *device_rgba = b->frame.as<uint8_t>();
+    *offsets = b->offsets.data();
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_batch_read(mg_batch *b, size_t i, uint8_t *rgba, size_t len) {
+    if (!b || !rgba) return fail(MG_ERR_INVALID_ARG, "null argument");
+    if (i >= b->offsets.size()) return fail(MG_ERR_INVALID_ARG, "frame index out of range");
+    if (len < b->bytes[i]) return fail(MG_ERR_BUFFER, "buffer shorter than the frame");
+    std::lock_guard<std::mutex> g(b->rdr->mu);
+    if (hipMemcpyAsync(rgba, b->frame.as<uint8_t>() + b->offsets[i], b->bytes[i], hipMemcpyDeviceToHost,
+                       b->rdr->stream()) != hipSuccess ||
+        hipStreamSynchronize(b->rdr->stream()) != hipSuccess)
+        return fail(MG_ERR_DEVICE, "download of the frame failed");
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_batch_last_tuning(const mg_batch *b, mg_batch_tuning *out) {
+    if (!b || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
+    *out = b->tuning;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_batch_stats(const mg_batch *b, mg_frame_stats *out) {
+    if (!b || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
+    *out = b->stats;
+    return MG_OK;
+}
+
 }  // extern "C"
 
 // --------------------------------------------------------------------------
@@ -746,9 +1059,17 @@ namespace {
 
 struct CassiaState {
     uint32_t width = 0, height = 0;
-    std::unique_ptr<mg::Renderer> rdr;
-    mg::DevBuf in;
+    std::shared_ptr<mg::Renderer> rdr;
+    mg::DevBuf in, frame;
     uint8_t *image = nullptr;
+
+    ~CassiaState() {
+        if (!rdr) return;
+        std::lock_guard<std::mutex> g(rdr->mu);
+        (void)hipStreamSynchronize(rdr->stream());
+        in.release();
+        frame.release();
+    }
 };
 
 CassiaState *g_cassia = nullptr;
@@ -762,17 +1083,18 @@ MG_EXPORT void cassia_init(uint32_t width, uint32_t height) {
     g_cassia = new CassiaState();
     g_cassia->width = width;
     g_cassia->height = height;
-    auto r = std::make_unique<mg::Renderer>();
     std::string err;
-    if (!r->init(err)) {
+    auto r = mg::Renderer::shared(err);
+    if (!r) {
         g_error = err;
         return;
     }
-    g_cassia->image = r->frame_image((size_t)width * height * 4 + 4, err);
-    if (!g_cassia->image) {
-        g_error = err;
+    std::lock_guard<std::mutex> g(r->mu);
+    if (!g_cassia->frame.reserve((size_t)width * height * 4 + 4)) {
+        g_error = "out of device memory (frame)";
         return;
     }
+    g_cassia->image = g_cassia->frame.as<uint8_t>();
     (void)hipMemsetAsync(g_cassia->image, 0, (size_t)width * height * 4, r->stream());
     (void)hipStreamSynchronize(r->stream());
     g_cassia->rdr = std::move(r);
@@ -784,6 +1106,7 @@ MG_EXPORT void cassia_render(const uint64_t *psegments, size_t n, const CassiaSt
         g_error = "cassia_render before a successful cassia_init";
         return;
     }
+    std::lock_guard<std::mutex> g(st->rdr->mu);
     std::string err;
     if (!st->in.reserve(std::max<size_t>(n, 1) * 8) ||
         (n && hipMemcpyAsync(st->in.ptr, psegments, n * 8, hipMemcpyHostToDevice, st->rdr->stream()) != hipSuccess)) {
@@ -819,8 +1142,9 @@ MG_EXPORT void cassia_render(const uint64_t *psegments, size_t n, const CassiaSt
     for (int q = 0; q < 4; ++q) sd.clear[q] = 1.0f;
     f.scenes.push_back(sd);
     f.scene_line_base.push_back(0);
+    f.image_bytes = (size_t)st->width * st->height * 4;
     mg::FrameResult r;
-    if (!st->rdr->run(f, 0, st->image, nullptr, r, err)) g_error = err;
+    if (!st->rdr->run(f, nullptr, nullptr, 0, st->image, nullptr, r, err)) g_error = err;
 }
 
 MG_EXPORT void cassia_shutdown(void) {

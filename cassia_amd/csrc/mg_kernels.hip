@@ -1307,7 +1307,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
             }
         });
         if (write_rec && tid == 0) {
-            const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + (colp - 1);
+            const uint64_t tile = (uint64_t)S.tile_base + (uint64_t)j * S.row_len + (colp - 1);
             pp.rec_hdr[tile] = make_uint2(rec_off, n_union);
             atomicAdd(pp.n_records, n_union);
         }
@@ -1540,7 +1540,7 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     const PropsEntry *props = pp.props + S.props_base;
     const uint32_t px = lane >> 3, py = lane & 7u;  // pixel of this lane (x*8+y)
     const bool clips = S.flags & SF_CLIPS;
-    const uint64_t tile = (uint64_t)S.cache_base + (uint64_t)j * S.row_len + c;
+    const uint64_t tile = (uint64_t)S.tile_base + (uint64_t)j * S.row_len + c;
     const uint2 hdr = pp.rec_hdr[tile];
     const uint32_t n = hdr.y;
     const uint4 *rec = pp.rec + hdr.x;

@@ -6,6 +6,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -23,6 +25,28 @@ struct DevBuf {
     T *as() const { return static_cast<T *>(ptr); }
 };
 
+// Device-resident lines of one composition or batch: float4 points and u16
+// layer ids, appended as the caller inserts them.
+struct LineStore {
+    DevBuf pts, ids;
+    size_t cap = 0;
+    bool upload(const float *pts4, const uint16_t *ids, size_t first, size_t n, hipStream_t s, std::string &err);
+    void release();
+    const float4 *points() const { return pts.as<float4>(); }
+    const uint16_t *layer_ids() const { return ids.as<uint16_t>(); }
+};
+
+// Per-tile previous layer counts of the BufferLayerCaches of one
+// composition (-1 = None).
+struct LayerCaches {
+    std::vector<DevBuf> bufs;
+    std::vector<size_t> tiles;
+    int32_t *array(int id, size_t n_tiles, hipStream_t s, bool &resized, std::string &err);
+    void clear(int id, hipStream_t s);
+    void release(int id);
+    void release_all();
+};
+
 // Host-side description of one frame (a batch of scenes).
 struct FrameSpec {
     std::vector<SceneDesc> scenes;
@@ -38,12 +62,15 @@ struct FrameSpec {
     bool want_tile_ops = false;
     bool want_coverage = false;
     uint32_t coverage_cap = 0;
-    uint64_t tables_version = 0;        // layers/props/stops unchanged when equal to the last upload
+    // layers/props/stops/scenes are those of the last upload when both match
+    uint64_t tables_owner = 0;
+    uint64_t tables_version = 0;
+    uint64_t image_bytes = 0;           // size of the output image (every scene's frame must fit)
 };
 
 struct FrameResult {
     uint64_t line_pixels = 0, segments = 0, kept = 0;
-    uint32_t passes = 0, radix_bits = 0;
+    uint32_t passes = 0, radix_bits = 0, sort_bits = 0;
     uint64_t tiles = 0;
     double device_ms = 0.0;
     // setup+scan, emit, histogram+bases, sort passes, tile runs, row walk, paint
@@ -52,32 +79,29 @@ struct FrameResult {
     const uint64_t *sorted = nullptr;   // device pointer to the sorted words (mode 1)
 };
 
+// The frame engine of one device: its stream, the scratch buffers of a frame
+// and the uploaded frame tables. Compositions and batches on a device share
+// one engine (Renderer::shared); `mu` serialises their frames, and scratch
+// contents (sorted words, tile ops, coverage) belong to the last frame run.
 class Renderer {
 public:
     Renderer();
     ~Renderer();
     bool init(std::string &err);
+    static std::shared_ptr<Renderer> shared(std::string &err);
 
-    // Resident line store (float4 points + u16 layer ids).
-    bool upload_lines(const float *pts4, const uint16_t *ids, size_t first, size_t n, std::string &err);
-    size_t lines_capacity() const { return lines_cap_; }
+    std::mutex mu;
 
-    // Per-tile previous layer counts of a BufferLayerCache (-1 = None).
-    int32_t *cache_array(int id, size_t tiles, bool &resized, std::string &err);
-    bool cache_clear(int id, std::string &err);
-    void cache_release(int id);
-
-    // The frame image: the caller's pixels are uploaded first so that tiles
-    // the painter leaves untouched keep their contents.
-    uint8_t *frame_image(size_t bytes, std::string &err);
-
-    bool run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_t *prev_layers,
-             FrameResult &res, std::string &err);
+    // Frame over `n_lines` device-resident lines (float4 points, u16 layer
+    // ids; unused by ingest frames) into the device image `image`.
+    bool run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
+             int32_t *prev_layers, FrameResult &res, std::string &err);
     bool download_sorted(uint64_t *dst, size_t n, std::string &err);
     bool download_tile_ops(int8_t *dst, size_t n, std::string &err);
     bool download_coverage(std::vector<int32_t> &a, std::vector<uint32_t> &key, uint32_t &count,
                            std::string &err);
     hipStream_t stream() const { return stream_; }
+    int device() const { return device_; }
     int cu_count() const { return cus_; }
 
 private:
@@ -90,15 +114,13 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     hipEvent_t evs_[8] = {};
-    uint64_t uploaded_version_ = 0;
+    uint64_t uploaded_owner_ = 0, uploaded_version_ = 0;
     std::vector<SceneDesc> uploaded_scenes_;
-    DevBuf pts_, ids_;
-    size_t lines_cap_ = 0;
     DevBuf scenes_, scene_line_base_, layers_, props_, stops_;
     DevBuf setup_, lengths_, offsets_, scan_status_, counters_, err_;
     DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
     DevBuf row_count_, row_start_, row_scene_, row_status_;
-    DevBuf image_, overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
+    DevBuf overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
     DevBuf cnt0_, cnt_acc_, cnt_status_;
     DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
@@ -106,8 +128,6 @@ private:
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
     uint32_t n_work_ = 0;
     uint64_t n_records_ = 0, n_runs_ = 0;
-    std::vector<DevBuf> caches_;
-    std::vector<size_t> cache_tiles_;
     const uint64_t *last_sorted_ = nullptr;
 };
 

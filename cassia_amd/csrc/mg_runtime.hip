@@ -58,14 +58,13 @@ Renderer::~Renderer() {
         (void)hipSetDevice(device_);
         if (stream_) (void)hipStreamSynchronize(stream_);
     }
-    DevBuf *all[] = {&pts_, &ids_, &scenes_, &scene_line_base_, &layers_, &props_, &stops_,
+    DevBuf *all[] = {&scenes_, &scene_line_base_, &layers_, &props_, &stops_,
                      &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
-                     &row_count_, &row_start_, &row_scene_, &row_status_, &image_, &overflow_,
+                     &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
                      &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_nr_,
                      &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_};
     for (DevBuf *b : all) b->release();
-    for (DevBuf &b : caches_) b.release();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evs_)
@@ -101,37 +100,52 @@ bool Renderer::init(std::string &err) {
     return true;
 }
 
-bool Renderer::upload_lines(const float *pts4, const uint16_t *ids, size_t first, size_t n, std::string &err) {
-    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+std::shared_ptr<Renderer> Renderer::shared(std::string &err) {
+    static std::mutex m;
+    static std::weak_ptr<Renderer> w;
+    std::lock_guard<std::mutex> g(m);
+    if (auto r = w.lock()) return r;
+    auto r = std::make_shared<Renderer>();
+    if (!r->init(err)) return nullptr;
+    w = r;
+    return r;
+}
+
+bool LineStore::upload(const float *pts4, const uint16_t *ids_h, size_t first, size_t n, hipStream_t s,
+                       std::string &err) {
     size_t need = first + n;
-    if (need > lines_cap_) {
-        size_t cap = std::max<size_t>(need, lines_cap_ * 2);
+    if (need > cap) {
+        size_t ncap = std::max<size_t>(need, cap * 2);
         DevBuf np, ni;
-        if (!np.reserve(cap * 16) || !ni.reserve(cap * 2)) {
+        if (!np.reserve(ncap * 16) || !ni.reserve(ncap * 2)) {
             np.release();
             ni.release();
             err = "out of device memory (lines)";
             return false;
         }
-        if (first > 0 && pts_.ptr) {
-            MG_HIP(hipMemcpyAsync(np.ptr, pts_.ptr, first * 16, hipMemcpyDeviceToDevice, stream_), "copy lines");
-            MG_HIP(hipMemcpyAsync(ni.ptr, ids_.ptr, first * 2, hipMemcpyDeviceToDevice, stream_), "copy ids");
-            MG_HIP(hipStreamSynchronize(stream_), "sync");
+        if (first > 0 && pts.ptr) {
+            MG_HIP(hipMemcpyAsync(np.ptr, pts.ptr, first * 16, hipMemcpyDeviceToDevice, s), "copy lines");
+            MG_HIP(hipMemcpyAsync(ni.ptr, ids.ptr, first * 2, hipMemcpyDeviceToDevice, s), "copy ids");
         }
-        pts_.release();
-        ids_.release();
-        pts_ = np;
-        ids_ = ni;
-        lines_cap_ = cap;
+        MG_HIP(hipStreamSynchronize(s), "sync");
+        pts.release();
+        ids.release();
+        pts = np;
+        ids = ni;
+        cap = ncap;
     }
     if (n) {
-        MG_HIP(hipMemcpyAsync(pts_.as<char>() + first * 16, pts4, n * 16, hipMemcpyHostToDevice, stream_),
-               "upload lines");
-        MG_HIP(hipMemcpyAsync(ids_.as<char>() + first * 2, ids, n * 2, hipMemcpyHostToDevice, stream_),
-               "upload ids");
-        MG_HIP(hipStreamSynchronize(stream_), "sync");
+        MG_HIP(hipMemcpyAsync(pts.as<char>() + first * 16, pts4, n * 16, hipMemcpyHostToDevice, s), "upload lines");
+        MG_HIP(hipMemcpyAsync(ids.as<char>() + first * 2, ids_h, n * 2, hipMemcpyHostToDevice, s), "upload ids");
+        MG_HIP(hipStreamSynchronize(s), "sync");
     }
     return true;
+}
+
+void LineStore::release() {
+    pts.release();
+    ids.release();
+    cap = 0;
 }
 
 static void fill32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
@@ -146,45 +160,40 @@ static void fill64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s) {
     k_fill_u64<<<blocks, 256, 0, s>>>(p, v, n);
 }
 
-int32_t *Renderer::cache_array(int id, size_t tiles, bool &resized, std::string &err) {
+int32_t *LayerCaches::array(int id, size_t n_tiles, hipStream_t s, bool &resized, std::string &err) {
     resized = false;
     if (id < 0) return nullptr;
-    if ((size_t)id >= caches_.size()) {
-        caches_.resize(id + 1);
-        cache_tiles_.resize(id + 1, 0);
+    if ((size_t)id >= bufs.size()) {
+        bufs.resize(id + 1);
+        tiles.resize(id + 1, 0);
     }
-    if (cache_tiles_[id] != tiles) {
-        if (!caches_[id].reserve(std::max<size_t>(tiles, 1) * 4)) {
+    if (tiles[id] != n_tiles) {
+        if (!bufs[id].reserve(std::max<size_t>(n_tiles, 1) * 4)) {
             err = "out of device memory (cache)";
             return nullptr;
         }
-        fill32(caches_[id].as<uint32_t>(), 0xFFFFFFFFu, tiles, stream_);
-        cache_tiles_[id] = tiles;
+        fill32(bufs[id].as<uint32_t>(), 0xFFFFFFFFu, n_tiles, s);
+        tiles[id] = n_tiles;
         resized = true;
     }
-    return caches_[id].as<int32_t>();
+    return bufs[id].as<int32_t>();
 }
 
-bool Renderer::cache_clear(int id, std::string &err) {
-    if (id >= 0 && (size_t)id < caches_.size() && caches_[id].ptr)
-        fill32(caches_[id].as<uint32_t>(), 0xFFFFFFFFu, cache_tiles_[id], stream_);
-    MG_HIP(hipGetLastError(), "cache clear");
-    return true;
+void LayerCaches::clear(int id, hipStream_t s) {
+    if (id >= 0 && (size_t)id < bufs.size() && bufs[id].ptr) fill32(bufs[id].as<uint32_t>(), 0xFFFFFFFFu, tiles[id], s);
 }
 
-void Renderer::cache_release(int id) {
-    if (id >= 0 && (size_t)id < caches_.size()) {
-        caches_[id].release();
-        cache_tiles_[id] = 0;
+void LayerCaches::release(int id) {
+    if (id >= 0 && (size_t)id < bufs.size()) {
+        bufs[id].release();
+        tiles[id] = 0;
     }
 }
 
-uint8_t *Renderer::frame_image(size_t bytes, std::string &err) {
-    if (!image_.reserve(bytes)) {
-        err = "out of device memory (image)";
-        return nullptr;
-    }
-    return image_.as<uint8_t>();
+void LayerCaches::release_all() {
+    for (DevBuf &b : bufs) b.release();
+    bufs.clear();
+    tiles.clear();
 }
 
 static uint32_t bits_for(uint64_t v) {  // bits to represent 0..v
@@ -206,10 +215,56 @@ static bool upload_vec(DevBuf &b, const std::vector<T> &v, hipStream_t s, std::s
     return true;
 }
 
-bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_t *prev_layers,
-                   FrameResult &res, std::string &err) {
+// Host check of the scene descriptors the kernels index with: scenes own
+// disjoint tile rows and per-tile ranges (in scene order) inside the frame's
+// totals, their lines and tables lie inside what is uploaded, their crops
+// inside their canvas and their pixels inside the output image.
+static bool scenes_valid(const FrameSpec &spec, size_t n_lines, const uint8_t *image, uint64_t tiles,
+                         std::string &err) {
+    uint64_t row_end = 0, tile_end = 0;
+    for (size_t i = 0; i < spec.scenes.size(); ++i) {
+        const SceneDesc &s = spec.scenes[i];
+        bool ok = s.row_base >= row_end && s.tile_base >= tile_end;
+        row_end = (uint64_t)s.row_base + s.tile_rows;
+        tile_end = (uint64_t)s.tile_base + (uint64_t)s.tile_rows * s.row_len;
+        ok = ok && (uint64_t)s.layer_base + s.n_layer_entries <= spec.layers.size() &&
+             (uint64_t)s.props_base + s.n_props <= spec.props.size();
+        if (spec.mode != 2)
+            ok = ok && (uint64_t)s.line_base + s.n_lines <= n_lines && i < spec.scene_line_base.size() &&
+                 spec.scene_line_base[i] == s.line_base;
+        ok = ok && s.row_len == (s.width + 7) / 8 && s.tile_rows == (s.rows_px + 7) / 8 && s.stride >= s.width &&
+             s.crop_h0 <= s.crop_h1 && s.crop_h1 <= s.row_len && s.crop_v0 <= s.crop_v1 && s.crop_v1 <= s.tile_rows;
+        if (image && s.rows_px && s.width)
+            ok = ok && s.out_offset + ((uint64_t)(s.rows_px - 1) * s.stride + s.width) * 4 <= spec.image_bytes;
+        if (!ok) {
+            err = "invalid scene descriptor " + std::to_string(i);
+            return false;
+        }
+    }
+    if (tile_end > tiles) {
+        err = "scene tile ranges exceed the frame's tiles";
+        return false;
+    }
+    return true;
+}
+
+static bool props_valid(const FrameSpec &spec, std::string &err) {
+    for (const PropsEntry &p : spec.props)
+        if ((p.flags & PF_GRADIENT) && (uint64_t)p.stops_offset + 5ull * p.n_stops > spec.stops.size()) {
+            err = "gradient stops outside the stops table";
+            return false;
+        }
+    return true;
+}
+
+bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
+                   int32_t *prev_layers, FrameResult &res, std::string &err) {
     MG_HIP(hipSetDevice(device_), "hipSetDevice");
     const uint32_t n_scenes = (uint32_t)spec.scenes.size();
+    if (n_scenes == 0 || n_scenes > 8192) {  // LineSetup::packed holds a 13-bit scene index
+        err = "a frame holds 1 to 8192 scenes";
+        return false;
+    }
     uint32_t global_rows = 0;
     uint32_t max_cols = 1;
     uint64_t tiles = 0;
@@ -218,6 +273,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
         max_cols = std::max(max_cols, std::min(s.crop_h1, s.row_len));
         tiles += (uint64_t)s.tile_rows * s.row_len;
     }
+    if (!scenes_valid(spec, n_lines, image, tiles, err)) return false;
     FrameDims dims{};
     dims.n_scenes = n_scenes;
     dims.n_lines = (uint32_t)n_lines;
@@ -247,18 +303,22 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     res = FrameResult{};
     res.passes = dims.passes;
     res.radix_bits = R;
+    res.sort_bits = sort_bits;
     res.tiles = tiles;
 
-    const bool same_scenes = spec.tables_version != 0 && spec.tables_version == uploaded_version_ &&
+    const bool same_scenes = spec.tables_version != 0 && spec.tables_owner == uploaded_owner_ &&
+                             spec.tables_version == uploaded_version_ &&
                              uploaded_scenes_.size() == spec.scenes.size() &&
                              std::memcmp(uploaded_scenes_.data(), spec.scenes.data(),
                                          spec.scenes.size() * sizeof(SceneDesc)) == 0;
     if (!same_scenes) {
+        if (!props_valid(spec, err)) return false;
         if (!upload_vec(scenes_, spec.scenes, stream_, err) ||
             !upload_vec(scene_line_base_, spec.scene_line_base, stream_, err) ||
             !upload_vec(layers_, spec.layers, stream_, err) || !upload_vec(props_, spec.props, stream_, err) ||
             !upload_vec(stops_, spec.stops, stream_, err))
             return false;
+        uploaded_owner_ = spec.tables_owner;
         uploaded_version_ = spec.tables_version;
         uploaded_scenes_ = spec.scenes;
     }
@@ -287,7 +347,7 @@ bool Renderer::run(const FrameSpec &spec, size_t n_lines, uint8_t *image, int32_
     const bool ingest = spec.mode == 2;
     if (L && !ingest) {
         k_setup<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
-            pts_.as<float4>(), ids_.as<uint16_t>(), layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(),
+            pts, ids, layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(),
             scene_line_base_.as<uint32_t>(), dims, setup_.as<LineSetup>(), lengths_.as<uint32_t>(),
             err_.as<uint32_t>());
         const uint64_t nparts = (L + kScanTileHost - 1) / kScanTileHost;

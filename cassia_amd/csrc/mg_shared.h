@@ -60,7 +60,7 @@ struct SceneDesc {
     uint32_t row_len, tile_rows;
     uint32_t row_base;                // first global tile row of the scene
     uint32_t crop_h0, crop_h1, crop_v0, crop_v1;  // tiles
-    uint32_t cache_base;              // into previous_layers (SF_CACHE)
+    uint32_t tile_base;               // first tile of the scene in per-tile arrays (record headers, previous_layers)
     uint32_t flags;
     uint64_t out_offset;              // byte offset of pixel (0, 0) in the output
     float clear[4];

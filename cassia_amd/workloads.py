@@ -36,26 +36,46 @@ def affine_path_transform(pts, t9):
 
 
 class Scene:
-    """Lines with per-line layer ids plus per-layer props as plain data."""
+    """Lines with per-line layer ids plus per-layer solid props as plain data.
+
+    `props` is a list of dicts (color, rule, blend) per layer id, or one dict
+    of arrays with those keys ((n, 4) colors, (n,) fill rules, (n,) blend modes).
+    """
 
     def __init__(self, width, height, lines, layer_ids, props, background, orders=None, name=""):
         self.width, self.height = width, height
         self.lines = np.ascontiguousarray(lines, dtype=np.float32)
         self.layer_ids = np.ascontiguousarray(layer_ids, dtype=np.uint16)
-        self.props = props  # list of dicts per layer id
+        if isinstance(props, dict):
+            self.colors = np.asarray(props["color"], np.float32).reshape(-1, 4)
+            self.rules = np.asarray(props["rule"], np.uint32).reshape(-1)
+            self.blends = np.asarray(props["blend"], np.uint32).reshape(-1)
+        else:
+            self.colors = np.array([p["color"] for p in props], np.float32).reshape(-1, 4)
+            self.rules = np.array([p["rule"] for p in props], np.uint32)
+            self.blends = np.array([p["blend"] for p in props], np.uint32)
         self.background = list(background)
         self.orders = orders
         self.name = name
 
     @property
     def n_layers(self):
-        return len(self.props)
+        return len(self.colors)
+
+    @property
+    def props(self):
+        return [dict(color=[float(v) for v in self.colors[i]], rule=int(self.rules[i]), blend=int(self.blends[i]))
+                for i in range(self.n_layers)]
 
     def build(self, m):
         """Composition of module `m` (mold-API implementation) holding the scene."""
         comp = m.Composition()
-        for _ in range(self.n_layers):
-            comp.create_layer()
+        n = self.n_layers
+        if hasattr(comp, "create_layers"):
+            comp.create_layers(n)
+        else:
+            for _ in range(n):
+                comp.create_layer()
         if hasattr(comp, "insert_lines_multi"):
             comp.insert_lines_multi(self.layer_ids, self.lines)
         else:
@@ -64,13 +84,21 @@ class Scene:
             bounds = np.flatnonzero(np.diff(ids.astype(np.int64))) + 1
             for chunk in np.split(order, bounds):
                 comp.insert_lines(int(self.layer_ids[chunk[0]]), self.lines[chunk])
-        for lid, p in enumerate(self.props):
-            layer = comp.get_mut(lid)
-            if layer is None:
-                continue
-            layer.set_props(props_of(m, p))
-            if self.orders is not None:
-                layer.set_order(int(self.orders[lid]))
+        present = np.zeros(n, bool)
+        present[np.unique(self.layer_ids)] = True
+        if hasattr(comp, "set_solid_props"):
+            # layers without lines do not exist in mold (get_mut gives None)
+            ids = np.flatnonzero(present).astype(np.uint16)
+            comp.set_solid_props(ids, self.colors[ids], self.rules[ids], self.blends[ids])
+        else:
+            for lid, p in enumerate(self.props):
+                if present[lid]:
+                    comp.get_mut(lid).set_props(props_of(m, p))
+        if self.orders is not None:
+            for lid in range(n):
+                layer = comp.get_mut(lid)
+                if layer is not None:
+                    layer.set_order(int(self.orders[lid]))
         return comp
 
 
@@ -142,3 +170,76 @@ def polygons(seed=2, width=3840, height=2160, count=200_000, kmin=5, kmax=50, sm
                   blend=int(blends[i])) for i in range(n_layers)]
     return Scene(width, height, lines, ids, props, [1.0, 1.0, 1.0, 1.0],
                  name=f"polygons-{width}x{height}-{count}")
+
+
+def quad_flatten_points(p0, c, p1, n):
+    """Points B(t_j), j = 0..n, of quadratic Béziers (p0, c, p1) at t_j = j / n.
+
+    float32, left to right, no FMA: mt = 1 - t; x = ((mt*mt)*x0 + (2*mt)*t*cx) + (t*t)*x1.
+    `p0`, `c`, `p1` are (k, 2) arrays, `n` a (k,) array of line counts; returns the
+    (sum(n + 1), 2) points curve after curve. B(0) = p0 and B(1) = p1 exactly.
+    """
+    n = np.asarray(n, dtype=np.int64)
+    curve = np.repeat(np.arange(len(n)), n + 1)
+    start = np.concatenate([[0], np.cumsum(n + 1)[:-1]])
+    j = np.arange(curve.size) - start[curve]
+    t = (j.astype(np.float32) / n[curve].astype(np.float32)).astype(np.float32)
+    mt = (np.float32(1.0) - t).astype(np.float32)
+    a = (mt * mt).astype(np.float32)
+    b = ((np.float32(2.0) * mt).astype(np.float32) * t).astype(np.float32)
+    cc = (t * t).astype(np.float32)
+    out = np.empty((curve.size, 2), np.float32)
+    for k in range(2):
+        x0, cx, x1 = p0[curve, k], c[curve, k], p1[curve, k]
+        out[:, k] = ((a * x0).astype(np.float32) + (b * cx).astype(np.float32)).astype(np.float32) + \
+            (cc * x1).astype(np.float32)
+    return out
+
+
+def glyph_batch(seed=3, n_scenes=4096, gamma=3.0, min_side=32, max_side=512, paths=(1, 200), curves=(2, 6),
+                lines_per_curve=(8, 32)):
+    """Many independent small scenes: canvas side round(32 * 16^(u^gamma)),
+    u ~ U[0, 1) (32..512 px, square), each holding U{1..200} glyph-like paths.
+    A path is a closed loop of U{2..6} quadratic Béziers whose anchor and
+    control points are uniform over the canvas, each flattened to U{8..32}
+    lines (quad_flatten_points). One layer per path (order = index), EvenOdd,
+    opaque solid sRGB colour, Over, white background. Returns a list of Scene."""
+    rng = np.random.default_rng(seed)
+    u = rng.random(n_scenes)
+    side = np.clip(np.round(min_side * (max_side / min_side) ** (u ** gamma)), min_side, max_side).astype(np.int64)
+    n_paths = rng.integers(paths[0], paths[1] + 1, n_scenes)
+    P = int(n_paths.sum())
+    scene_of_path = np.repeat(np.arange(n_scenes), n_paths)
+    m = rng.integers(curves[0], curves[1] + 1, P)
+    Cn = int(m.sum())
+    path_of_curve = np.repeat(np.arange(P), m)
+    sc_side = side[scene_of_path[path_of_curve]].astype(np.float32)[:, None]
+    anchors = (rng.random((Cn, 2)).astype(np.float32) * sc_side).astype(np.float32)
+    ctrl = (rng.random((Cn, 2)).astype(np.float32) * sc_side).astype(np.float32)
+    first = np.concatenate([[0], np.cumsum(m)[:-1]])
+    nxt = np.arange(Cn) + 1
+    nxt[first + m - 1] = first
+    nl = rng.integers(lines_per_curve[0], lines_per_curve[1] + 1, Cn)
+    pts = quad_flatten_points(anchors, ctrl, anchors[nxt], nl)
+    # lines j -> j+1 inside each curve
+    cstart = np.concatenate([[0], np.cumsum(nl + 1)[:-1]])
+    li = np.repeat(cstart, nl) + (np.arange(int(nl.sum())) - np.repeat(np.concatenate([[0], np.cumsum(nl)[:-1]]), nl))
+    lines = np.concatenate([pts[li], pts[li + 1]], axis=1).astype(np.float32)
+    curve_of_line = np.repeat(np.arange(Cn), nl)
+    path_of_line = path_of_curve[curve_of_line]
+    path_first = np.concatenate([[0], np.cumsum(n_paths)[:-1]])
+    scene_of_line = scene_of_path[path_of_line]
+    local_id = (path_of_line - path_first[scene_of_line]).astype(np.uint16)
+    line_bounds = np.concatenate([[0], np.cumsum(np.bincount(scene_of_line, minlength=n_scenes))])
+    crng = np.random.default_rng(seed + 1000)
+    cols = srgb_u8_to_linear(crng.integers(0, 256, (P, 3)))
+    scenes = []
+    for s in range(n_scenes):
+        a, b = line_bounds[s], line_bounds[s + 1]
+        p0 = path_first[s]
+        k = int(n_paths[s])
+        props = dict(color=np.concatenate([cols[p0:p0 + k], np.ones((k, 1), np.float32)], axis=1),
+                     rule=np.ones(k, np.uint32), blend=np.zeros(k, np.uint32))
+        w = int(side[s])
+        scenes.append(Scene(w, w, lines[a:b], local_id[a:b], props, [1.0, 1.0, 1.0, 1.0], name=f"glyphs-{s}"))
+    return scenes

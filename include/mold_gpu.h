@@ -152,6 +152,12 @@ MG_EXPORT mg_status mg_layer_order(const mg_composition *comp, mg_layer_id id, u
 MG_EXPORT mg_status mg_layer_set_order(mg_composition *comp, mg_layer_id id, uint16_t order);
 MG_EXPORT mg_status mg_layer_props(const mg_composition *comp, mg_layer_id id, mg_props *out);
 MG_EXPORT mg_status mg_layer_set_props(mg_composition *comp, mg_layer_id id, const mg_props *props);
+/* Bulk forms for scenes with many layers: n calls of create_layer (ids out,
+   stops at the first failure) and of Layer::set_props (layer.rs:231-238) on
+   ids[i] with props[i]. */
+MG_EXPORT mg_status mg_create_layers(mg_composition *comp, size_t n, mg_layer_id *out);
+MG_EXPORT mg_status mg_layers_set_props(mg_composition *comp, const mg_layer_id *ids, const mg_props *props,
+                                        size_t n);
 /* Composition::remove_disabled (composition.rs:123-142) */
 MG_EXPORT mg_status mg_remove_disabled(mg_composition *comp);
 /* Lines held by the builder / lines of enabled layers (composition.rs:116-121) */
@@ -244,6 +250,61 @@ typedef struct mg_frame_stats {
     uint64_t runs, records;    /* tile layer runs, tile id records       */
 } mg_frame_stats;
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
+
+/* ------------------------------------------------------------- batching */
+
+/* A batch renders many compositions, each into its own frame, as ONE frame
+   of the device pipeline: one launch per stage covers every scene, through a
+   per-scene table of line, layer, props, tile-row and output offsets. No
+   counterpart in the reference's API, where each Composition::render
+   (composition.rs:212-328) is one frame; a caller with many small scenes
+   calls mg_batch_render once instead of mg_render per scene. Every frame's
+   bytes equal what mg_render_device gives for that composition alone. The
+   batch keeps its lines resident while the compositions' contents are
+   unchanged; compositions stay owned by the caller. */
+typedef struct mg_batch mg_batch;
+
+/* One frame of a batch: width x height pixels (stride = width), cleared to
+   `background`, optionally cropped (Rect::new, buffer_layout.rs:43-50). */
+typedef struct mg_batch_frame {
+    uint32_t width, height;
+    float background[4];
+    int32_t has_crop;
+    mg_rect crop;
+} mg_batch_frame;
+
+MG_EXPORT mg_status mg_batch_create(mg_batch **out);
+MG_EXPORT void mg_batch_destroy(mg_batch *batch);
+
+/* Renders comps[i] into frames[i] for i < n (n <= 8192; no layer caches, no
+   flusher). Blocking. `*device_rgba` receives the device buffer holding all
+   frames and `*offsets` (n entries, owned by the batch) the byte offset of
+   each frame in it; both stay valid until the next call on `batch`. */
+MG_EXPORT mg_status mg_batch_render(mg_batch *batch, mg_composition *const *comps, const mg_batch_frame *frames,
+                                    size_t n, void **device_rgba, const uint64_t **offsets);
+
+/* Copies frame i of the last mg_batch_render (width * height * 4 bytes) to
+   host memory `rgba` of `len` bytes. */
+MG_EXPORT mg_status mg_batch_read(mg_batch *batch, size_t i, uint8_t *rgba, size_t len);
+
+/* Radix bits of the batch's sort (4, 6, 8 or 9), or 0 to let the autotuner
+   choose: the next render times each candidate on the batch itself and keeps
+   the fastest for batches of the same shape. Result-invariant. */
+MG_EXPORT mg_status mg_batch_set_tuning(mg_batch *batch, uint32_t radix_bits);
+
+/* The autotuner's last decision: candidates tried (radix bits) with their
+   frame times (ms; 0 = not tried) and the choice. */
+typedef struct mg_batch_tuning {
+    uint32_t n_candidates;
+    uint32_t radix_bits[4];
+    double device_ms[4];
+    uint32_t chosen_radix_bits;
+} mg_batch_tuning;
+MG_EXPORT mg_status mg_batch_last_tuning(const mg_batch *batch, mg_batch_tuning *out);
+
+/* Statistics of the batch's last render (as mg_last_frame_stats, summed over
+   its scenes). */
+MG_EXPORT mg_status mg_batch_stats(const mg_batch *batch, mg_frame_stats *out);
 
 /* ------------------------------------------------------ cassia-compatible */
 
