@@ -84,8 +84,9 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes):
-    """Plain per-scene frames vs one batched frame over the same compositions."""
+def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes, plain=True):
+    """Plain per-scene frames vs one batched frame over the same compositions
+    (`plain` False: the batched frame alone, for profiling it)."""
     t = time.perf_counter()
     scenes = workloads.glyph_batch(n_scenes=n_scenes)
     comps = [s.build(cassia_amd) for s in scenes]
@@ -93,19 +94,21 @@ def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes):
     progress(f"{n_scenes} glyph scenes built in {t_build:.1f} s")
     frames = [(s.width, s.height, s.background) for s in scenes]
     # plain: one frame per scene (a first pass sizes each composition's buffers)
-    for c, (w, h, bg) in zip(comps, frames):
-        c.render_device(w, h, bg)
     plain_dev_ms = 0.0
     plain_segs = 0
-    t = time.perf_counter()
-    for c, (w, h, bg) in zip(comps, frames):
-        c.render_device(w, h, bg)
-    plain_s = time.perf_counter() - t
-    progress(f"plain: {n_scenes} frames in {plain_s:.2f} s")
-    for c in comps:
-        st = c.stats()
-        plain_dev_ms += st.device_ms
-        plain_segs += st.segments
+    plain_s = float("nan")
+    if plain:
+        for c, (w, h, bg) in zip(comps, frames):
+            c.render_device(w, h, bg)
+        t = time.perf_counter()
+        for c, (w, h, bg) in zip(comps, frames):
+            c.render_device(w, h, bg)
+        plain_s = time.perf_counter() - t
+        progress(f"plain: {n_scenes} frames in {plain_s:.2f} s")
+        for c in comps:
+            st = c.stats()
+            plain_dev_ms += st.device_ms
+            plain_segs += st.segments
     # batched: the first render runs the autotuner
     b = cassia_amd.Batch()
     t = time.perf_counter()
@@ -129,6 +132,8 @@ def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes):
     # bit-identity: every batched frame against its plain render
     same = 0
     for i, (c, (w, h, bg)) in enumerate(zip(comps, frames)):
+        if not plain:
+            break
         hb = np.zeros((w * h, 4), np.uint8)
         c.render(cassia_amd.Buffer(hb, w), bg, None)
         same += int(np.array_equal(hb, b.read(i)))
@@ -152,7 +157,7 @@ def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes):
         "batched_p99_ms": float(np.percentile(lat, 99)),
         "speedup_frames_per_s": plain_s / batch_s,
         "bit_identical_frames": same,
-        "bit_identical": same == n,
+        "bit_identical": same == n if plain else None,
         "radix_bits": int(st.radix_bits),
         "sort_passes": int(st.sort_passes),
         "autotune": {"candidates": {int(tn.radix_bits[k]): round(tn.device_ms[k], 3) for k in range(tn.n_candidates)},
@@ -201,6 +206,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scenes measurement")
     ap.add_argument("--batch-scenes", type=int, default=4096)
+    ap.add_argument("--no-frame", action="store_true",
+                    help="measure only the batched many-scenes workload (prints its object alone)")
+    ap.add_argument("--no-plain", action="store_true",
+                    help="with --no-frame: skip the plain per-scene frames and the identity check")
     ap.add_argument("--radix", type=int, default=0, help="sort digit width (0 = automatic)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only copy a device buffer of this many bytes (HBM counter calibration)")
@@ -217,6 +226,10 @@ def main():
     from cassia_amd import workloads
     if cassia_amd.device_count() < 1:
         raise SystemExit("no gfx950 device")
+    if args.no_frame:
+        print(json.dumps({"batched": bench_batched(cassia_amd, workloads, args.steps, args.warmup,
+                                                   args.batch_scenes, plain=not args.no_plain)}))
+        return
     if args.workload == "polygons":
         scene = workloads.polygons(count=args.paths)
         workload = (f"single large frame, plain GPU path: {scene.width}x{scene.height}, "

@@ -987,11 +987,10 @@ MG_EXPORT mg_status mg_batch_render(mg_batch *b, mg_composition *const *comps, c
     uint32_t bits = b->radix_bits;
     if (!bits) {
         // the autotuner: a choice holds for later batches with the same sort
-        // key width and the same power-of-two class of sorted words
-        if ((s = run(0))) return s;
+        // key width and the same power-of-two class of line counts
         uint32_t lg = 0;
-        while (lg < 63 && (r.segments >> lg)) ++lg;
-        const uint32_t key = r.sort_bits * 64 + lg;
+        while (lg < 63 && (b->n_lines >> lg)) ++lg;
+        const uint32_t key = mg::key_layout(f).sort_bits * 64 + lg;
         if (b->tuned_for != key) {
             static const uint32_t cand[4] = {4, 6, 8, 9};
             mg_batch_tuning t{};

@@ -19,6 +19,7 @@ struct PaintParams {
     uint8_t *out;                // RGBA8 frame(s)
     uint32_t col_bits, layer_bits;
     uint32_t cap_ids, cap_runs;  // row-walk LDS capacities: queue + runs, runs of one tile
+    uint32_t rec_chunk;          // id records a row reserves at a time
     uint32_t *overflow;          // bit 0: LDS capacity, bit 1: record capacity
     // debug: coverage records (A values) and per-tile write ops
     int32_t *cov_a;
@@ -50,7 +51,7 @@ __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n);
 __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n);
 __global__ void k_setup(const float4 *pts, const uint16_t *ids, const LayerEntry *layers,
                         const SceneDesc *scenes, const uint32_t *scene_line_base, FrameDims dims,
-                        LineSetup *out, uint32_t *lengths, uint32_t *err);
+                        LineSetup *out, uint32_t *lengths, uint32_t *err, uint32_t *bucket_hist);
 __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *status,
                            uint32_t *stall, uint64_t *total);
 // Per-partition slot counts written by the emission: pass 0's digit counts
@@ -63,9 +64,8 @@ constexpr uint32_t kCountSlicesHost = 256;
 __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
                        FrameDims dims, const FrameCounters *counters, uint64_t *out, const uint32_t *part_line,
                        SlotCounts sc);
-__global__ void k_order_hist(const LineSetup *setup, uint64_t n_lines, uint32_t order_mask, uint32_t *hist);
 __global__ void k_order_scatter(const LineSetup *setup, const uint32_t *lengths, uint64_t n_lines,
-                                uint32_t order_mask, uint32_t *base, LineSetup *setup_perm,
+                                const SceneDesc *scenes, uint32_t *base, LineSetup *setup_perm,
                                 uint32_t *lengths_perm);
 __global__ void k_part_lines(const uint32_t *offsets, const uint32_t *lengths, uint64_t n_lines,
                              uint32_t part_size, uint32_t *part_line);
@@ -81,8 +81,9 @@ __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounte
                             uint32_t shift, const uint32_t *digit_base, uint32_t *status,
                             uint32_t *stall, const uint32_t *first_off);
 __global__ void k_tile_starts(PaintParams pp, uint32_t global_rows);
-__global__ void k_tile_runs(PaintParams pp);
-__global__ void k_row_walk(PaintParams pp);
+__global__ void k_tile_runs(PaintParams pp, const uint2 *work, uint32_t n_work);
+template <uint32_t T>
+__global__ void k_row_walk(PaintParams pp);  // T = 256 or 1024 threads per tile row
 __global__ void k_paint(PaintParams pp);
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride);
 size_t paint_lds_bytes_host(uint32_t cap, bool clips);
@@ -90,6 +91,5 @@ size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 constexpr uint32_t kEmitTileHost = 1024;
 constexpr uint32_t kSortTileHost = 2048;
 constexpr uint32_t kScanTileHost = 2048;
-constexpr uint32_t kWalkThreadsHost = 1024;
 
 }  // namespace mg

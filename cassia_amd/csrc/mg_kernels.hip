@@ -168,15 +168,52 @@ __device__ __forceinline__ uint32_t scene_of_line(const uint32_t *scene_line_bas
     return lo;
 }
 
+// Render frames order their lines by (scene, layer order) with a counting
+// sort, so slot order inside every tile (one scene's) is layer order. Bucket
+// of a line: the scene's props range, where every layer order has an entry.
+__device__ __forceinline__ uint32_t line_bucket(const SceneDesc &S, uint32_t packed) {
+    const uint32_t order = packed & 0xFFFFu, last = S.n_props ? S.n_props - 1 : 0u;
+    return S.props_base + (order < last ? order : last);
+}
+
+// Head lane of each run of equal keys among a wave's valid lanes (a prefix
+// of the wave); returns the head of this lane's run.
+__device__ __forceinline__ uint32_t run_head_lane(bool valid, uint32_t key, uint64_t *heads_out) {
+    const uint32_t lane = lane_id();
+    uint32_t prev = __shfl_up(key, 1);
+    const bool head = valid && (lane == 0 || prev != key);
+    const uint64_t heads = __ballot(head);
+    *heads_out = heads;
+    const uint64_t le = heads & ((2ull << lane) - 1ull);
+    return le ? 63u - (uint32_t)__clzll((long long)le) : 0u;
+}
+
+// hist[key] += 1 for every valid lane: one atomic per run of equal keys.
+__device__ __forceinline__ void bucket_count(bool valid, uint32_t key, uint32_t *hist) {
+    uint64_t heads;
+    run_head_lane(valid, key, &heads);
+    const uint32_t lane = lane_id();
+    const uint32_t vn = (uint32_t)__popcll(__ballot(valid));
+    if ((heads >> lane) & 1ull) {
+        const uint64_t after = heads & ~((2ull << lane) - 1ull);
+        const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
+        const uint32_t end = next < vn ? next : vn;
+        atomicAdd(&hist[key], end - lane);
+    }
+}
+
 // LinesBuilder::build for one line (segment.rs:107-219).
 __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, const uint16_t *__restrict__ ids,
                                                const LayerEntry *__restrict__ layers,
                                                const SceneDesc *__restrict__ scenes,
                                                const uint32_t *__restrict__ scene_line_base,
                                                FrameDims dims, LineSetup *__restrict__ out,
-                                               uint32_t *__restrict__ lengths, uint32_t *__restrict__ err) {
+                                               uint32_t *__restrict__ lengths, uint32_t *__restrict__ err,
+                                               uint32_t *__restrict__ bucket_hist) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= dims.n_lines) return;
+    const bool valid = i < dims.n_lines;
+    uint32_t bucket = 0xFFFFFFFFu;
+    if (valid) {
     uint32_t sc = dims.n_scenes == 1 ? 0 : scene_of_line(scene_line_base, dims.n_scenes, i);
     const SceneDesc &S = scenes[sc];
     uint32_t id = ids[i];
@@ -235,6 +272,9 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
     }
     out[i] = r;
     lengths[i] = length;
+    bucket = line_bucket(S, r.packed);
+    }
+    if (bucket_hist) bucket_count(valid, bucket, bucket_hist);
 }
 
 // ------------------------------------------------------------------- scan
@@ -541,43 +581,19 @@ __global__ __launch_bounds__(256) void k_count_reduce(const uint32_t *__restrict
 // line setup records and lengths. Consecutive lines mostly share an order
 // (the lines of one path), so each run of equal orders inside a wave takes
 // one atomic, issued by the run's first lane.
-__device__ __forceinline__ uint32_t run_head_lane(bool valid, uint32_t order, uint64_t *heads_out) {
-    const uint32_t lane = lane_id();
-    uint32_t prev = __shfl_up(order, 1);
-    const bool head = valid && (lane == 0 || prev != order);
-    const uint64_t heads = __ballot(head);
-    *heads_out = heads;
-    const uint64_t le = heads & ((2ull << lane) - 1ull);
-    return le ? 63u - (uint32_t)__clzll((long long)le) : 0u;
-}
-
-__global__ __launch_bounds__(256) void k_order_hist(const LineSetup *__restrict__ setup, uint64_t n_lines,
-                                                    uint32_t order_mask, uint32_t *__restrict__ hist) {
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    const bool valid = i < n_lines;
-    const uint32_t order = valid ? setup[i].packed & order_mask : 0xFFFFFFFFu;
-    uint64_t heads;
-    run_head_lane(valid, order, &heads);
-    const uint32_t lane = lane_id();
-    const uint32_t vn = (uint32_t)__popcll(__ballot(valid));  // valid lanes are a prefix
-    if ((heads >> lane) & 1ull) {
-        const uint64_t after = heads & ~((2ull << lane) - 1ull);
-        const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
-        const uint32_t end = next < vn ? next : vn;
-        atomicAdd(&hist[order], end - lane);
-    }
-}
-
+// Stable scatter of the lines to their bucket's range (bases = exclusive
+// scan of the bucket counts), one atomic per run of equal buckets.
 __global__ __launch_bounds__(256) void k_order_scatter(const LineSetup *__restrict__ setup,
                                                        const uint32_t *__restrict__ lengths, uint64_t n_lines,
-                                                       uint32_t order_mask, uint32_t *__restrict__ base,
+                                                       const SceneDesc *__restrict__ scenes,
+                                                       uint32_t *__restrict__ base,
                                                        LineSetup *__restrict__ setup_perm,
                                                        uint32_t *__restrict__ lengths_perm) {
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const bool valid = i < n_lines;
     LineSetup L;
     if (valid) L = setup[i];
-    const uint32_t order = valid ? L.packed & order_mask : 0xFFFFFFFFu;
+    const uint32_t order = valid ? line_bucket(scenes[L.packed >> 19], L.packed) : 0xFFFFFFFFu;
     uint64_t heads;
     const uint32_t head = run_head_lane(valid, order, &heads);
     const uint32_t lane = lane_id();
@@ -921,8 +937,6 @@ constexpr uint32_t kMaskBit = 1u << 30, kSkipClipBit = 1u << 31;
 constexpr uint32_t kRunEvenOdd = 1u << 16;                    // run key: layer | even_odd << 16
 constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flags
 constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
-constexpr uint32_t kRecChunk = 16384;                         // records reserved per row at a time
-constexpr uint32_t kWalkThreads = 1024;                       // threads per tile row in the row walk
 
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride) {
     return (size_t)cap * 24 + (size_t)rcap * 16 + (size_t)tstride * 8 + 64;
@@ -1045,14 +1059,11 @@ __global__ __launch_bounds__(256) void k_tile_starts(PaintParams pp, uint32_t gl
     pp.tile_start[i] = lo;
 }
 
-// Runs of tile column colp (col + 1; 0 gathers every col < 0) of row grow.
-// Four columns per work group, one wave each; waves never synchronise.
-__global__ __launch_bounds__(256) void k_tile_runs(PaintParams pp) {
+// Runs of tile column colp (col + 1; 0 gathers every col < 0) of row grow,
+// by one wave. Returns the number of runs.
+__device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_t grow, uint32_t colp) {
     const uint32_t lane = lane_id();
-    const uint32_t grow = blockIdx.x;
-    const uint32_t colp = blockIdx.y * 4 + (threadIdx.x >> 6);
     const SceneDesc &S = pp.scenes[pp.row_scene[grow]];
-    if (colp > scene_h1(S)) return;
     const uint32_t *ts_row = pp.tile_start + (uint64_t)grow * pp.tstride;
     const uint32_t ts = ts_row[colp], te = ts_row[colp + 1];
     const PropsEntry *props = pp.props + S.props_base;
@@ -1112,11 +1123,21 @@ __global__ __launch_bounds__(256) void k_tile_runs(PaintParams pp) {
     }
     if (lane == 0) {
         pp.tile_nr[(uint64_t)grow * pp.tstride + colp] = nr;
-        if (nr) {
-            pp.run_cov[ts + nr - 1] = open;
-            atomicAdd(pp.n_runs, nr);
-        }
+        if (nr) pp.run_cov[ts + nr - 1] = open;
     }
+    return nr;
+}
+
+// Tile runs of every (row, column) of the work list: each wave takes entries
+// wave, wave + waves, ...; waves never synchronise.
+__global__ __launch_bounds__(256) void k_tile_runs(PaintParams pp, const uint2 *__restrict__ work, uint32_t n_work) {
+    const uint32_t waves = gridDim.x * 4;
+    uint32_t runs = 0;
+    for (uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < n_work; t += waves) {
+        const uint2 wk = work[t];
+        runs += tile_runs_one(pp, wk.x, wk.y);
+    }
+    if (lane_id() == 0 && runs) atomicAdd(pp.n_runs, runs);
 }
 
 // Merge-path split of two layer-sorted lists at diagonal d: q elements first
@@ -1135,9 +1156,10 @@ __device__ __forceinline__ uint32_t merge_split(const uint32_t *qk, uint32_t nq,
 // tiles left of the crop and tile -1 are folded without dropping empty
 // covers; inside the crop, covers that become empty are dropped,
 // layer_workbench.rs:194,475).
-__global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ uint32_t s_wsum[kWalkThreads / 64];
+    __shared__ uint32_t s_wsum[T / 64];
     __shared__ uint32_t s_rec[3];
     const uint32_t tid = threadIdx.x;
     const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs, tstride = pp.tstride;
@@ -1160,7 +1182,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
     if (j < S.crop_v0 || j >= S.crop_v1) return;
     const uint32_t h0 = S.crop_h0, h1 = scene_h1(S);
     if (h0 >= h1) return;
-    for (uint32_t k = tid; k < tstride; k += kWalkThreads) {
+    for (uint32_t k = tid; k < tstride; k += T) {
         ts_l[k] = pp.tile_start[(uint64_t)grow * tstride + k];
         nr_l[k] = k + 1 < tstride ? pp.tile_nr[(uint64_t)grow * tstride + k] : 0u;
     }
@@ -1168,6 +1190,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
     uint32_t nq = 0, cur = 0;
     uint32_t rec_next = 0, rec_end = 0;
     bool counting = false;  // record store full: only count what the frame needs
+    uint32_t row_records = 0;  // id records of the row (thread 0)
     // the next column to walk: fold columns without runs change nothing
     auto next_colp = [&](uint32_t c) {
         uint32_t x = c + 1;
@@ -1176,7 +1199,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
     };
     // Runs are prefetched into registers two columns ahead (one run per
     // thread, two register sets taking turns), so their loads overlap a
-    // whole column's merge; runs beyond kWalkThreads are loaded when used.
+    // whole column's merge; runs beyond T are loaded when used.
     struct Prefetch {
         uint32_t key, beg, n;
         uint64_t cov;
@@ -1185,7 +1208,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
         f.n = 0;
         if (col > h1) return;
         const uint32_t nr2 = nr_l[col], ts2 = ts_l[col];
-        f.n = nr2 < kWalkThreads ? nr2 : kWalkThreads;
+        f.n = nr2 < T ? nr2 : T;
         if (f.n > rcap) f.n = rcap;
         if (tid < f.n) {
             f.key = pp.run_key[ts2 + tid];
@@ -1208,7 +1231,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
             rb[tid] = f.beg;
             rc[tid] = f.cov;
         }
-        for (uint32_t k = f.n + tid; k < nr; k += kWalkThreads) {
+        for (uint32_t k = f.n + tid; k < nr; k += T) {
             rk[k] = pp.run_key[ts + k];
             rb[k] = pp.run_beg[ts + k];
             rc[k] = pp.run_cov[ts + k];
@@ -1218,7 +1241,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
         const uint32_t *qkc = qk[cur];
         const uint64_t *qcc = qc[cur];
         const uint32_t total = nq + nr;
-        const uint32_t per = (total + kWalkThreads - 1) / kWalkThreads;
+        const uint32_t per = (total + T - 1) / T;
         const uint32_t d0 = tid * per < total ? tid * per : total;
         const uint32_t d1 = d0 + per < total ? d0 + per : total;
         const uint32_t i0 = merge_split(qkc, nq, rk, nr, d0);
@@ -1255,7 +1278,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
             cq += keep ? 1u : 0u;
         });
         uint32_t all;
-        const uint32_t ex = block_excl_scan<kWalkThreads / 64>(cu | (cq << 16), s_wsum, &all);
+        const uint32_t ex = block_excl_scan<T / 64>(cu | (cq << 16), s_wsum, &all);
         const uint32_t n_union = all & 0xFFFFu, nq_new = all >> 16;
         uint32_t rec_off = 0;
         if (!fold && counting) {
@@ -1263,7 +1286,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
         } else if (!fold) {
             if (rec_next + n_union > rec_end) {
                 if (tid == 0) {
-                    const uint32_t sz = n_union > kRecChunk ? n_union : kRecChunk;
+                    const uint32_t sz = n_union > pp.rec_chunk ? n_union : pp.rec_chunk;
                     const uint32_t base = atomicAdd(pp.rec_used, sz);
                     s_rec[0] = base;
                     s_rec[1] = base + sz;
@@ -1309,7 +1332,7 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
         if (write_rec && tid == 0) {
             const uint64_t tile = (uint64_t)S.tile_base + (uint64_t)j * S.row_len + (colp - 1);
             pp.rec_hdr[tile] = make_uint2(rec_off, n_union);
-            atomicAdd(pp.n_records, n_union);
+            row_records += n_union;
         }
         __syncthreads();
         nq = nq_new;
@@ -1329,7 +1352,10 @@ __global__ __launch_bounds__(kWalkThreads) void k_row_walk(PaintParams pp) {
         c1 = c3;
         c2 = next_colp(c3);
     }
+    if (tid == 0 && row_records) atomicAdd(pp.n_records, row_records);
 }
+template __global__ void k_row_walk<256>(PaintParams);
+template __global__ void k_row_walk<1024>(PaintParams);
 
 // Staged window of sorted words for the paint wave.
 struct Stage {

@@ -79,6 +79,13 @@ struct FrameResult {
     const uint64_t *sorted = nullptr;   // device pointer to the sorted words (mode 1)
 };
 
+// Bit layout of a frame's render key: (row, column + 1, layer order) above
+// the 22-bit payload, and the bits the LSD sort covers.
+struct KeyLayout {
+    uint32_t col_bits, layer_bits, key_bits, sort_shift, sort_bits;
+};
+KeyLayout key_layout(const FrameSpec &spec);
+
 // The frame engine of one device: its stream, the scratch buffers of a frame
 // and the uploaded frame tables. Compositions and batches on a device share
 // one engine (Renderer::shared); `mu` serialises their frames, and scratch
@@ -123,10 +130,10 @@ private:
     DevBuf overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
     DevBuf cnt0_, cnt_acc_, cnt_status_;
-    DevBuf work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
+    DevBuf work_, runs_work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
-    uint32_t n_work_ = 0;
+    uint32_t n_work_ = 0, n_runs_work_ = 0;
     uint64_t n_records_ = 0, n_runs_ = 0;
     const uint64_t *last_sorted_ = nullptr;
 };

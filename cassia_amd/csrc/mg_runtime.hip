@@ -63,7 +63,7 @@ Renderer::~Renderer() {
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
                      &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_nr_,
-                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_};
+                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_work_};
     for (DevBuf *b : all) b->release();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -215,6 +215,27 @@ static bool upload_vec(DevBuf &b, const std::vector<T> &v, hipStream_t s, std::s
     return true;
 }
 
+KeyLayout key_layout(const FrameSpec &spec) {
+    uint32_t global_rows = 0, max_cols = 1;
+    for (const SceneDesc &s : spec.scenes) {
+        global_rows = std::max(global_rows, s.row_base + s.tile_rows);
+        max_cols = std::max(max_cols, std::min(s.crop_h1, s.row_len));
+    }
+    KeyLayout k;
+    k.col_bits = std::max(1u, bits_for(max_cols));
+    k.layer_bits = std::max(1u, bits_for(spec.max_order));
+    const uint32_t row_bits = global_rows > 1 ? bits_for(global_rows - 1) : 0;
+    k.key_bits = spec.mode != 1 ? row_bits + k.col_bits + k.layer_bits : 41;
+    // Render frames emit their lines grouped by layer order, so slot order is
+    // layer order inside every tile and a stable sort on (row, column) alone
+    // gives (row, column, layer). The canonical stream (generation order) and
+    // ingested streams sort on the whole key.
+    const bool order_lines = spec.mode == 0;
+    k.sort_shift = kPayloadBits + (order_lines ? k.layer_bits : 0);
+    k.sort_bits = order_lines ? std::max(1u, row_bits + k.col_bits) : k.key_bits;
+    return k;
+}
+
 // Host check of the scene descriptors the kernels index with: scenes own
 // disjoint tile rows and per-tile ranges (in scene order) inside the frame's
 // totals, their lines and tables lie inside what is uploaded, their crops
@@ -278,17 +299,13 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     dims.n_scenes = n_scenes;
     dims.n_lines = (uint32_t)n_lines;
     dims.global_rows = global_rows;
-    dims.col_bits = std::max(1u, bits_for(max_cols));
-    dims.layer_bits = std::max(1u, bits_for(spec.max_order));
-    const uint32_t row_bits = global_rows > 1 ? bits_for(global_rows - 1) : 0;
-    dims.key_bits = spec.mode != 1 ? row_bits + dims.col_bits + dims.layer_bits : 41;
-    // Render frames emit their lines grouped by layer order, so slot order is
-    // layer order inside every tile and a stable sort on (row, column) alone
-    // gives (row, column, layer). The canonical stream (generation order) and
-    // ingested streams sort on the whole key.
+    const KeyLayout kl = key_layout(spec);
+    dims.col_bits = kl.col_bits;
+    dims.layer_bits = kl.layer_bits;
+    dims.key_bits = kl.key_bits;
     const bool order_lines = spec.mode == 0;
-    dims.sort_shift = kPayloadBits + (order_lines ? dims.layer_bits : 0);
-    const uint32_t sort_bits = order_lines ? std::max(1u, row_bits + dims.col_bits) : dims.key_bits;
+    dims.sort_shift = kl.sort_shift;
+    const uint32_t sort_bits = kl.sort_bits;
     uint32_t R = spec.radix_bits;
     if (R == 0)  // measured per-pass cost on MI355X: 6-bit 2.3 ms, 8-bit 2.6 ms, 9-bit 4.0 ms (3e8 words)
         R = (sort_bits + 7) / 8 < (sort_bits + 5) / 6 ? 8 : 6;
@@ -346,33 +363,33 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     fill32(pc, 0, 64, stream_);
     const bool ingest = spec.mode == 2;
     if (L && !ingest) {
+        // render frames: counting sort of the lines by (scene, layer order),
+        // the bucket counts taken by the setup kernel
+        const uint64_t n_buckets = std::max<size_t>(spec.props.size(), 1);
+        const uint64_t oparts = (n_buckets + kScanTileHost - 1) / kScanTileHost;
+        if (order_lines) {
+            if (!setup_perm_.reserve(L * sizeof(LineSetup)) || !lengths_perm_.reserve(L * 4) ||
+                !order_hist_.reserve(n_buckets * 4) || !order_status_.reserve(oparts * 8)) {
+                err = "out of device memory (line order)";
+                return false;
+            }
+            fill32(order_hist_.as<uint32_t>(), 0, n_buckets, stream_);
+            fill64(order_status_.as<uint64_t>(), 0, oparts, stream_);
+        }
         k_setup<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
-            pts, ids, layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(),
-            scene_line_base_.as<uint32_t>(), dims, setup_.as<LineSetup>(), lengths_.as<uint32_t>(),
-            err_.as<uint32_t>());
+            pts, ids, layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(), scene_line_base_.as<uint32_t>(), dims,
+            setup_.as<LineSetup>(), lengths_.as<uint32_t>(), err_.as<uint32_t>(),
+            order_lines ? order_hist_.as<uint32_t>() : nullptr);
         const uint64_t nparts = (L + kScanTileHost - 1) / kScanTileHost;
         if (!scan_status_.reserve(nparts * 8)) { err = "out of device memory"; return false; }
         const uint32_t *scan_in = lengths_.as<uint32_t>();
         if (order_lines) {
-            // counting sort of the lines by layer order
-            const uint32_t n_orders = 1u << dims.layer_bits;
-            const uint64_t oparts = (n_orders + kScanTileHost - 1) / kScanTileHost;
-            if (!setup_perm_.reserve(L * sizeof(LineSetup)) || !lengths_perm_.reserve(L * 4) ||
-                !order_hist_.reserve(n_orders * 4) ||
-                !order_status_.reserve(oparts * 8)) {
-                err = "out of device memory (line order)";
-                return false;
-            }
-            fill32(order_hist_.as<uint32_t>(), 0, n_orders, stream_);
-            fill64(order_status_.as<uint64_t>(), 0, oparts, stream_);
-            k_order_hist<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(setup_.as<LineSetup>(), L, n_orders - 1,
-                                                                         order_hist_.as<uint32_t>());
             k_scan_u32<<<(uint32_t)oparts, 256, 0, stream_>>>(order_hist_.as<uint32_t>(), order_hist_.as<uint32_t>(),
-                                                            n_orders, order_status_.as<uint64_t>(), stall,
+                                                            n_buckets, order_status_.as<uint64_t>(), stall,
                                                             &ctr->order_total);
             k_order_scatter<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
-                setup_.as<LineSetup>(), lengths_.as<uint32_t>(), L, n_orders - 1, order_hist_.as<uint32_t>(),
-                setup_perm_.as<LineSetup>(), lengths_perm_.as<uint32_t>());
+                setup_.as<LineSetup>(), lengths_.as<uint32_t>(), L, scenes_.as<SceneDesc>(),
+                order_hist_.as<uint32_t>(), setup_perm_.as<LineSetup>(), lengths_perm_.as<uint32_t>());
             scan_in = lengths_perm_.as<uint32_t>();
         }
         fill64(scan_status_.as<uint64_t>(), 0, nparts, stream_);
@@ -538,18 +555,28 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     }
     if (work_scenes_.size() != spec.scenes.size() ||
         std::memcmp(work_scenes_.data(), spec.scenes.data(), spec.scenes.size() * sizeof(SceneDesc)) != 0) {
-        std::vector<uint32_t> work;
+        // paint: the tiles inside each crop; tile runs: columns 0..h1 (colp)
+        // of the same rows
+        std::vector<uint32_t> work, rwork;
         for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
             const SceneDesc &s = spec.scenes[sc];
             const uint32_t h1 = std::min(s.crop_h1, s.row_len);
-            for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j)
+            if (s.crop_h0 >= h1) continue;
+            for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j) {
                 for (uint32_t c = s.crop_h0; c < h1; ++c) {
                     work.push_back(s.row_base + j);
                     work.push_back(c);
                 }
+                for (uint32_t c = 0; c <= h1; ++c) {
+                    rwork.push_back(s.row_base + j);
+                    rwork.push_back(c);
+                }
+            }
         }
         n_work_ = (uint32_t)(work.size() / 2);
-        if (n_work_ && !upload_vec(work_, work, stream_, err)) return false;
+        n_runs_work_ = (uint32_t)(rwork.size() / 2);
+        if (n_work_ && (!upload_vec(work_, work, stream_, err) || !upload_vec(runs_work_, rwork, stream_, err)))
+            return false;
         work_scenes_ = spec.scenes;
     }
     const uint32_t n_work = n_work_;
@@ -594,7 +621,10 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.n_records = overflow_.as<uint32_t>() + 2;
     fill32(overflow_.as<uint32_t>() + 3, 0, 1, stream_);
     k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows);
-    k_tile_runs<<<dim3(global_rows, (tstride - 1 + 3) / 4), 256, 0, stream_>>>(pp);
+    {
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((n_runs_work_ + 3) / 4, (uint64_t)cus_ * 16);
+        k_tile_runs<<<blocks, 256, 0, stream_>>>(pp, runs_work_.as<uint2>(), n_runs_work_);
+    }
     MG_HIP(hipGetLastError(), "tile runs launch");
     MG_HIP(hipEventRecord(evs_[5], stream_), "event");
 
@@ -605,18 +635,42 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         hipDeviceProp_t p;
         if (hipGetDeviceProperties(&p, device_) == hipSuccess) lds_max = p.sharedMemPerBlock;
     }
-    const uint32_t rcap = 2048;
-    const size_t fixed = walk_lds_bytes_host(0, rcap, tstride) + 256;
-    if (fixed + 64 * 24 > lds_max) {
+    // A tile holds at most one run per layer order and the cover queue one
+    // entry per order, so the walk never needs more than runs = orders and
+    // queue + runs = 2 orders. Frames with few orders walk each row with a
+    // 256-thread work group in a small LDS share (several rows per CU);
+    // others with 1024 threads and the whole LDS.
+    const uint64_t orders = (uint64_t)spec.max_order + 1;
+    const bool small_walk = orders <= 512;
+    uint32_t cap, rcap;
+    if (small_walk) {
+        rcap = (uint32_t)((orders + 63) & ~63ull);
+        cap = 2 * rcap;
+    } else {
+        rcap = 2048;
+        const size_t fixed = walk_lds_bytes_host(0, rcap, tstride) + 256;
+        if (fixed + 64 * 24 > lds_max) {
+            err = "canvas too wide for the row walk";
+            return false;
+        }
+        cap = (uint32_t)std::min<size_t>(((lds_max - fixed) / 24) & ~size_t(63), 0xFFFFu - rcap);
+    }
+    const size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride), lds_p = paint_lds_bytes_host(cap, clips);
+    if (lds_w > lds_max) {
         err = "canvas too wide for the row walk";
         return false;
     }
-    const uint32_t cap = (uint32_t)std::min<size_t>(((lds_max - fixed) / 24) & ~size_t(63), 0xFFFFu - rcap);
-    const size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride), lds_p = paint_lds_bytes_host(cap, clips);
     pp.cap_ids = cap;
     pp.cap_runs = rcap;
-    MG_HIP(hipFuncSetAttribute((const void *)k_row_walk, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_w),
-           "walk lds");
+    // id records a row reserves at a time: about a row's worth of tiles
+    {
+        uint32_t chunk = 1024;
+        while (chunk < 16384 && chunk < tstride * 16) chunk <<= 1;
+        pp.rec_chunk = chunk;
+    }
+    const void *walk_fn = small_walk ? (const void *)k_row_walk<256> : (const void *)k_row_walk<1024>;
+    const uint32_t walk_threads = small_walk ? 256 : 1024;
+    MG_HIP(hipFuncSetAttribute(walk_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_w), "walk lds");
     MG_HIP(hipFuncSetAttribute((const void *)k_paint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p),
            "paint lds");
     if (rec_cap_ < tiles * 64) rec_cap_ = tiles * 64;
@@ -632,7 +686,10 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
         MG_HIP(hipEventRecord(evs_[5], stream_), "event");
-        k_row_walk<<<global_rows, kWalkThreadsHost, lds_w, stream_>>>(pp);
+        if (small_walk)
+            k_row_walk<256><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
+        else
+            k_row_walk<1024><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
         MG_HIP(hipEventRecord(evs_[6], stream_), "event");
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
         MG_HIP(hipEventRecord(evs_[7], stream_), "event");
@@ -649,7 +706,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         if (rec_cap_ >= 0xFFFF0000ull) break;
         // the walk kept counting after the store filled up: ov[1] covers the
         // frame's records plus at most one partly used chunk per row
-        rec_cap_ = std::max<uint64_t>(rec_cap_ * 2, (uint64_t)ov[1] + (uint64_t)global_rows * 16384);
+        rec_cap_ = std::max<uint64_t>(rec_cap_ * 2, (uint64_t)ov[1] + (uint64_t)global_rows * pp.rec_chunk);
     }
     err = (rec_cap_ >= 0xFFFF0000ull) ? "too many tile id records" : "a tile holds more layers than the row walk's capacity";
     return false;
