@@ -168,6 +168,101 @@ def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes, plain=True):
     }
 
 
+def bench_serving(cassia_amd, workloads, n_requests=160, fractions=(0.25, 0.5, 0.8), seed=5):
+    """Poisson request streams against the render server: a pool of 32
+    request scenes (serve_pool), each request one of them drawn at random.
+    The capacity comes from a closed burst of every scene four times; each
+    stream then arrives at a fraction of it. Latency = submission -> RGBA8 in
+    the caller's buffer; every result is compared with the scene's render
+    alone."""
+    import threading
+    t = time.perf_counter()
+    scenes = workloads.serve_pool(seed=seed, n=32)
+    comps = [s.build(cassia_amd) for s in scenes]
+    progress(f"serving pool: {len(scenes)} scenes, {sum(len(s.lines) for s in scenes)} lines "
+             f"built in {time.perf_counter() - t:.1f} s")
+    refs, alone = [], []
+    for s, c in zip(scenes, comps):
+        buf = np.zeros((s.width * s.height, 4), np.uint8)
+        c.render(cassia_amd.Buffer(buf, s.width), s.background, None)
+        refs.append(buf)
+        c.render_device(s.width, s.height, s.background)
+        st = c.stats()
+        alone.append({"side": s.width, "lines": int(st.lines), "line_pixels": int(st.line_pixels),
+                      "segments": int(st.segments), "painted": int(st.segments_painted),
+                      "device_ms": round(st.device_ms, 3)})
+    progress("pool alone: " + "; ".join(f"{a['side']}px {a['lines']}L {a['device_ms']}ms" for a in alone))
+    srv = cassia_amd.Server(engines=2)
+
+    def run_stream(arrivals, picks):
+        done = [None] * len(picks)
+        tickets = [None] * len(picks)
+        ready = threading.Semaphore(0)
+
+        def waiter():
+            for k in range(len(picks)):
+                ready.acquire()
+                try:
+                    frame, res = srv.wait(tickets[k])
+                    done[k] = (np.array_equal(frame, refs[picks[k]]), res.latency_ms, res.batch_size)
+                except cassia_amd.MoldError as e:
+                    progress(f"request {k} (scene {picks[k]}) failed: {e}")
+                    done[k] = (False, float("nan"), 0)
+
+        th = threading.Thread(target=waiter)
+        th.start()
+        t0 = time.perf_counter()
+        for k, (a, i) in enumerate(zip(arrivals, picks)):
+            dt = t0 + a - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            s = scenes[i]
+            tickets[k] = srv.submit(comps[i], s.width, s.height, s.background)
+            ready.release()
+        th.join()
+        return done, time.perf_counter() - t0
+
+    rng = np.random.default_rng(seed)
+    # capacity: everything submitted at once
+    picks = [i for _ in range(4) for i in range(len(scenes))]
+    srv.stats(reset=True)
+    done, el = run_stream([0.0] * len(picks), picks)
+    capacity = len(picks) / el
+    st = srv.stats(reset=True)
+    progress(f"serving capacity: {capacity:.0f} requests/s (burst of {len(picks)})")
+    burst = {"requests": len(picks), "requests_per_s": capacity, "p50_ms": st.p50_ms, "p99_ms": st.p99_ms,
+             "mean_batch": st.mean_batch, "identical": all(d[0] for d in done)}
+    rates = []
+    for frac in fractions:
+        lam = frac * capacity
+        gaps = rng.exponential(1.0 / lam, n_requests)
+        arrivals = np.cumsum(gaps) - gaps[0]
+        picks = [int(i) for i in rng.integers(0, len(scenes), n_requests)]
+        done, el = run_stream(list(arrivals), picks)
+        st = srv.stats(reset=True)
+        lat = np.array([d[1] for d in done])
+        rates.append({"arrival_rate": lam, "requests": n_requests, "requests_per_s": n_requests / el,
+                      "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                      "max_ms": float(lat.max()), "mean_batch": st.mean_batch,
+                      "max_concurrent_batches": int(st.max_concurrent_batches),
+                      "identical": all(d[0] for d in done)})
+        progress(f"serving at {lam:.0f}/s: p50 {rates[-1]['p50_ms']:.2f} ms, p99 {rates[-1]['p99_ms']:.2f} ms")
+    srv.close()
+    sides = [s.width for s in scenes]
+    return {
+        "workload": ("Poisson request streams; request scene = canvas {64,256,1024,2048}^2 w.p. .5/.3/.15/.05, "
+                     "log-uniform 1e2..1e6 lines generated as the large frame's polygons; pool of 32 scenes"),
+        "pool_lines": int(sum(len(s.lines) for s in scenes)),
+        "pool_canvases": {str(c): sides.count(c) for c in sorted(set(sides))},
+        "pool_alone_device_ms": round(sum(a["device_ms"] for a in alone), 3),
+        "pool_alone": alone,
+        "engines": 2,
+        "burst": burst,
+        "rates": rates,
+        "identical": burst["identical"] and all(r["identical"] for r in rates),
+    }
+
+
 def measured_traffic(workload):
     """HBM bytes per launch per kernel from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
@@ -208,6 +303,8 @@ def main():
     ap.add_argument("--batch-scenes", type=int, default=4096)
     ap.add_argument("--no-frame", action="store_true",
                     help="measure only the batched many-scenes workload (prints its object alone)")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--only-serve", action="store_true", help="measure only the serving workload")
     ap.add_argument("--no-plain", action="store_true",
                     help="with --no-frame: skip the plain per-scene frames and the identity check")
     ap.add_argument("--radix", type=int, default=0, help="sort digit width (0 = automatic)")
@@ -226,6 +323,9 @@ def main():
     from cassia_amd import workloads
     if cassia_amd.device_count() < 1:
         raise SystemExit("no gfx950 device")
+    if args.only_serve:
+        print(json.dumps({"serving": bench_serving(cassia_amd, workloads)}))
+        return
     if args.no_frame:
         print(json.dumps({"batched": bench_batched(cassia_amd, workloads, args.steps, args.warmup,
                                                    args.batch_scenes, plain=not args.no_plain)}))
@@ -326,6 +426,8 @@ def main():
     if not args.no_batch:
         del comp
         out["batched"] = bench_batched(cassia_amd, workloads, args.steps, args.warmup, args.batch_scenes)
+    if not args.no_serve:
+        out["serving"] = bench_serving(cassia_amd, workloads)
     if not args.no_cpu:
         cb = cpu_baseline(scene, args.cpu_fraction, args.cpu_threads)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

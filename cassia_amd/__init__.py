@@ -102,6 +102,23 @@ class BatchTuning(C.Structure):
                 ("chosen_radix_bits", C.c_uint32)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("engines", C.c_uint32), ("max_batch", C.c_uint32)]
+
+
+class RequestResult(C.Structure):
+    _fields_ = [("ticket", C.c_uint64), ("status", C.c_int32), ("batch_size", C.c_uint32), ("engine", C.c_uint32),
+                ("_pad", C.c_uint32), ("latency_ms", C.c_double), ("queue_ms", C.c_double),
+                ("footprint", C.c_uint64)]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("completed", C.c_uint64), ("failed", C.c_uint64), ("batches", C.c_uint64),
+                ("mean_batch", C.c_double), ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("max_ms", C.c_double),
+                ("max_concurrent_batches", C.c_uint32), ("_pad", C.c_uint32), ("queued", C.c_uint64),
+                ("inflight_bytes", C.c_uint64)]
+
+
 class CassiaStyling(C.Structure):
     _fields_ = [("fill", C.c_float * 4), ("fillRule", C.c_uint32), ("blendMode", C.c_uint32),
                 ("_padding", C.c_uint32 * 2)]
@@ -164,6 +181,13 @@ _sig("mg_batch_read", C.c_int, _vp, C.c_size_t, C.POINTER(C.c_uint8), C.c_size_t
 _sig("mg_batch_set_tuning", C.c_int, _vp, C.c_uint32)
 _sig("mg_batch_last_tuning", C.c_int, _vp, C.POINTER(BatchTuning))
 _sig("mg_batch_stats", C.c_int, _vp, C.POINTER(FrameStats))
+_sig("mg_server_create", C.c_int, C.POINTER(ServerConfig), C.POINTER(_vp))
+_sig("mg_server_destroy", None, _vp)
+_sig("mg_server_config_of", C.c_int, _vp, C.POINTER(ServerConfig))
+_sig("mg_server_submit", C.c_int, _vp, _vp, C.POINTER(_BatchFrame), C.POINTER(C.c_uint8), C.c_size_t,
+     C.POINTER(C.c_uint64))
+_sig("mg_server_wait", C.c_int, _vp, C.c_uint64, C.POINTER(RequestResult))
+_sig("mg_server_stats", C.c_int, _vp, C.c_int, C.POINTER(ServerCounters))
 _sig("cassia_init", None, C.c_uint32, C.c_uint32)
 _sig("cassia_render", None, C.POINTER(C.c_uint64), C.c_size_t, C.POINTER(CassiaStyling), C.c_size_t)
 _sig("cassia_shutdown", None)
@@ -732,6 +756,62 @@ class Batch:
         s = FrameStats()
         _check(_lib.mg_batch_stats(self._h, C.byref(s)))
         return s
+
+
+class Server:
+    """Render server (mg_server_*): requests admitted by a device-memory
+    budget and rendered as batched frames by `engines` workers, each on its
+    own HIP stream (continuous batching)."""
+
+    def __init__(self, hbm_budget=0, engines=2, max_batch=8192):
+        cfg = ServerConfig(hbm_budget, engines, max_batch)
+        h = _vp()
+        _check(_lib.mg_server_create(C.byref(cfg), C.byref(h)))
+        self._h = h
+        self._live = {}
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.mg_server_destroy(self._h)
+            self._h = None
+            self._live.clear()
+
+    __del__ = close
+
+    def config(self):
+        c = ServerConfig()
+        _check(_lib.mg_server_config_of(self._h, C.byref(c)))
+        return c
+
+    def submit(self, comp, width, height, background, crop: Optional[Rect] = None, out=None):
+        """Queue a render of `comp` (kept unchanged until wait); returns the ticket."""
+        fr = _BatchFrame()
+        fr.width, fr.height = width, height
+        for k in range(4):
+            fr.background[k] = float(background[k])
+        if crop is not None:
+            fr.has_crop = 1
+            fr.crop = _Rect(*crop.px[:2], *crop.px[2:])
+        if out is None:
+            out = np.zeros((width * height, 4), np.uint8)
+        t = C.c_uint64(0)
+        _check(_lib.mg_server_submit(self._h, comp._h, C.byref(fr), out.ctypes.data_as(C.POINTER(C.c_uint8)),
+                                     out.nbytes, C.byref(t)))
+        self._live[t.value] = (comp, out)
+        return t.value
+
+    def wait(self, ticket):
+        """Blocks until the request is done: (frame (width * height, 4) uint8, RequestResult)."""
+        r = RequestResult()
+        st = _lib.mg_server_wait(self._h, ticket, C.byref(r))
+        comp, out = self._live.pop(ticket, (None, None))
+        _check(st)
+        return out, r
+
+    def stats(self, reset=False):
+        c = ServerCounters()
+        _check(_lib.mg_server_stats(self._h, 1 if reset else 0, C.byref(c)))
+        return c
 
 
 # -------------------------------------------------------- cassia-compatible

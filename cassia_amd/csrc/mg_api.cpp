@@ -14,6 +14,7 @@
 #include <vector>
 
 #include "../../include/mold_gpu.h"
+#include "mg_internal.h"
 #include "mg_path.h"
 #include "mg_runtime.h"
 #include "mg_shared.h"
@@ -817,7 +818,7 @@ struct mg_batch {
     mg::LineStore dlines;  // lines of all scenes, scene after scene
     mg::DevBuf frame;      // all frames, 256-byte aligned
     const uint64_t owner = next_owner_id();
-    // what the resident lines and tables were packed from
+    // what the scene descriptors and tables were packed from
     std::vector<uint64_t> owners, versions;
     std::vector<mg_batch_frame> packed_frames;
     uint64_t tables_version = 0;
@@ -835,7 +836,6 @@ struct mg_batch {
         std::lock_guard<std::mutex> g(rdr->mu);
         (void)hipSetDevice(rdr->device());
         (void)hipStreamSynchronize(rdr->stream());
-        dlines.release();
         frame.release();
     }
 };
@@ -849,42 +849,44 @@ bool same_frame(const mg_batch_frame &a, const mg_batch_frame &b) {
 // Packs the compositions into one frame: scene descriptors with their line,
 // layer, props, tile-row and output bases, concatenated tables and (when a
 // composition changed) the concatenated resident lines.
-mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_frame *frames, size_t n) {
+// Tables and resident lines of a composition for batched frames. The
+// composition's lines live in its own device buffer (uploaded as they are
+// inserted); a batch reads them in place through each scene's line address.
+// `held`: the engine whose mutex the caller holds.
+mg_status prepare_for_batch(mg_composition *c, const mg::Renderer *held) {
+    mg_status s = c->ensure_renderer();
+    if (s) return s;
+    prepare_tables(c, -1);
+    if (c->rdr.get() == held) return c->sync_lines();
+    std::lock_guard<std::mutex> g(c->rdr->mu);
+    return c->sync_lines();
+}
+
+// Packs the compositions into one frame: scene descriptors with their line,
+// layer, props, tile-row and output bases and line addresses, and the
+// concatenated tables.
+mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_frame *frames, size_t n,
+                     bool prepare) {
     bool same = b->owners.size() == n;
     for (size_t i = 0; i < n; ++i) {
-        prepare_tables(comps[i], -1);
+        if (prepare) {
+            mg_status s = prepare_for_batch(comps[i], b->rdr.get());
+            if (s) return s;
+        }
         same = same && b->owners[i] == comps[i]->owner && b->versions[i] == comps[i]->version;
     }
     bool same_frames = same && b->packed_frames.size() == n;
     for (size_t i = 0; same_frames && i < n; ++i) same_frames = same_frame(b->packed_frames[i], frames[i]);
     if (same_frames) return MG_OK;
-    if (!same) {
-        // concatenated lines, resident until a composition changes
-        size_t total = 0;
-        for (size_t i = 0; i < n; ++i) total += comps[i]->ids.size();
-        if (total > 0xFFFFFFFFull) return fail(MG_ERR_CAPACITY, "a batch holds at most 2^32 lines");
-        std::vector<float> pts(4 * total);
-        std::vector<uint16_t> ids(total);
-        size_t at = 0;
-        for (size_t i = 0; i < n; ++i) {
-            const mg_composition *c = comps[i];
-            if (!c->ids.empty()) {
-                std::memcpy(&pts[4 * at], c->pts.data(), c->ids.size() * 16);
-                std::memcpy(&ids[at], c->ids.data(), c->ids.size() * 2);
-            }
-            at += c->ids.size();
-        }
-        std::string err;
-        if (hipSetDevice(b->rdr->device()) != hipSuccess ||
-            !b->dlines.upload(pts.data(), ids.data(), 0, total, b->rdr->stream(), err))
-            return fail(MG_ERR_OOM, err.empty() ? "hipSetDevice failed" : err);
-        b->n_lines = total;
-        b->owners.resize(n);
-        b->versions.resize(n);
-        for (size_t i = 0; i < n; ++i) {
-            b->owners[i] = comps[i]->owner;
-            b->versions[i] = comps[i]->version;
-        }
+    size_t total = 0;
+    for (size_t i = 0; i < n; ++i) total += comps[i]->ids.size();
+    if (total > 0xFFFFFFFFull) return fail(MG_ERR_CAPACITY, "a batch holds at most 2^32 lines");
+    b->n_lines = total;
+    b->owners.resize(n);
+    b->versions.resize(n);
+    for (size_t i = 0; i < n; ++i) {
+        b->owners[i] = comps[i]->owner;
+        b->versions[i] = comps[i]->version;
     }
     mg::FrameSpec &f = b->spec;
     f = mg::FrameSpec{};
@@ -906,6 +908,8 @@ mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_f
         sd.n_props = (uint32_t)cs.props.size();
         sd.row_base = (uint32_t)row_base;
         sd.tile_base = (uint32_t)tile_base;
+        sd.lines_pts = (uint64_t)(uintptr_t)c->dlines.points();
+        sd.lines_ids = (uint64_t)(uintptr_t)c->dlines.layer_ids();
         sd.flags = c->spec_clips ? mg::SF_CLIPS : 0;
         sd.out_offset = out;
         const uint32_t stops_base = (uint32_t)f.stops.size();
@@ -971,16 +975,50 @@ MG_EXPORT mg_status mg_batch_render(mg_batch *b, mg_composition *const *comps, c
         if (!comps[i]) return fail(MG_ERR_INVALID_ARG, "composition is null");
         if (frames[i].width == 0 || frames[i].height == 0) return fail(MG_ERR_INVALID_ARG, "empty frame");
     }
+    return mgi::batch_render(b, comps, frames, n, true, device_rgba, offsets);
+}
+
+}  // extern "C"
+
+namespace mgi {
+
+mg_status set_error(mg_status s, const std::string &msg) { return fail(s, msg); }
+const char *last_error() { return g_error.c_str(); }
+
+mg_batch *batch_new(std::shared_ptr<mg::Renderer> engine) {
+    mg_batch *b = new mg_batch();
+    b->rdr = std::move(engine);
+    return b;
+}
+
+mg_status prepare(mg_composition *c) { return prepare_for_batch(c, nullptr); }
+
+uint64_t footprint(const mg_composition *c, const mg_batch_frame &fr, uint64_t *line_pixels) {
+    // 18 B per resident line, 40 + 40 B of line setup (and its ordered
+    // copy), two 8-byte sort buffers of 2 slots per line-pixel, the frame and
+    // per-tile paint state; line-pixels from each line's major extent
+    double lp = 0.0;
+    const size_t L = c->ids.size();
+    for (size_t i = 0; i < L; ++i) {
+        const float *q = &c->pts[4 * i];
+        lp += std::max(std::fabs(q[2] - q[0]), std::fabs(q[3] - q[1])) + 2.0;
+    }
+    if (line_pixels) *line_pixels = (uint64_t)lp;
+    const uint64_t tiles = (uint64_t)((fr.width + 7) / 8) * ((fr.height + 7) / 8);
+    return 98ull * L + 32ull * (uint64_t)lp + 4ull * fr.width * fr.height + 1024ull * tiles + (1ull << 20);
+}
+
+mg_status batch_render(mg_batch *b, mg_composition *const *comps, const mg_batch_frame *frames, size_t n,
+                       bool prepare, void **device_rgba, const uint64_t **offsets) {
     std::lock_guard<std::mutex> g(b->rdr->mu);
-    mg_status s = pack_batch(b, comps, frames, n);
+    mg_status s = pack_batch(b, comps, frames, n, prepare);
     if (s) return s;
     mg::FrameSpec &f = b->spec;
     std::string err;
     mg::FrameResult r;
     auto run = [&](uint32_t bits) -> mg_status {
         f.radix_bits = bits;
-        if (!b->rdr->run(f, b->dlines.points(), b->dlines.layer_ids(), b->n_lines, b->frame.as<uint8_t>(), nullptr,
-                         r, err))
+        if (!b->rdr->run(f, nullptr, nullptr, b->n_lines, b->frame.as<uint8_t>(), nullptr, r, err))
             return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
         return MG_OK;
     };
@@ -1019,13 +1057,12 @@ MG_EXPORT mg_status mg_batch_render(mg_batch *b, mg_composition *const *comps, c
     }
     if ((s = run(bits))) return s;
     fill_stats(b->stats, r, b->n_lines);
-    *device_rgba = b->frame.as<uint8_t>();
-    *offsets = b->offsets.data();
+    if (device_rgba) *device_rgba = b->frame.as<uint8_t>();
+    if (offsets) *offsets = b->offsets.data();
     return MG_OK;
 }
 
-MG_EXPORT mg_status mg_batch_read(mg_batch *b, size_t i, uint8_t *rgba, size_t len) {
-    if (!b || !rgba) return fail(MG_ERR_INVALID_ARG, "null argument");
+mg_status batch_download(mg_batch *b, size_t i, uint8_t *rgba, size_t len) {
     if (i >= b->offsets.size()) return fail(MG_ERR_INVALID_ARG, "frame index out of range");
     if (len < b->bytes[i]) return fail(MG_ERR_BUFFER, "buffer shorter than the frame");
     std::lock_guard<std::mutex> g(b->rdr->mu);
@@ -1034,6 +1071,15 @@ MG_EXPORT mg_status mg_batch_read(mg_batch *b, size_t i, uint8_t *rgba, size_t l
         hipStreamSynchronize(b->rdr->stream()) != hipSuccess)
         return fail(MG_ERR_DEVICE, "download of the frame failed");
     return MG_OK;
+}
+
+}  // namespace mgi
+
+extern "C" {
+
+MG_EXPORT mg_status mg_batch_read(mg_batch *b, size_t i, uint8_t *rgba, size_t len) {
+    if (!b || !rgba) return fail(MG_ERR_INVALID_ARG, "null argument");
+    return mgi::batch_download(b, i, rgba, len);
 }
 
 MG_EXPORT mg_status mg_batch_last_tuning(const mg_batch *b, mg_batch_tuning *out) {

@@ -20,6 +20,8 @@ struct PaintParams {
     uint32_t col_bits, layer_bits;
     uint32_t cap_ids, cap_runs;  // row-walk LDS capacities: queue + runs, runs of one tile
     uint32_t rec_chunk;          // id records a row reserves at a time
+    unsigned char *walk_scratch; // row-walk arrays in global memory (k_row_walk<T, true>)
+    uint64_t walk_stride;        // bytes per row of walk_scratch
     uint32_t *overflow;          // bit 0: LDS capacity, bit 1: record capacity
     // debug: coverage records (A values) and per-tile write ops
     int32_t *cov_a;
@@ -82,8 +84,9 @@ __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounte
                             uint32_t *stall, const uint32_t *first_off);
 __global__ void k_tile_starts(PaintParams pp, uint32_t global_rows);
 __global__ void k_tile_runs(PaintParams pp, const uint2 *work, uint32_t n_work);
-template <uint32_t T>
-__global__ void k_row_walk(PaintParams pp);  // T = 256 or 1024 threads per tile row
+// T = 256 or 1024 threads per tile row; G: queue in global scratch
+template <uint32_t T, bool G>
+__global__ void k_row_walk(PaintParams pp);
 __global__ void k_paint(PaintParams pp);
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride);
 size_t paint_lds_bytes_host(uint32_t cap, bool clips);

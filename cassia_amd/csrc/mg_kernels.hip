@@ -120,20 +120,20 @@ __device__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *wsum /* 4 */, uint
     return base + x - v;
 }
 
-// Block-wide exclusive scan of one u32 per thread over W waves.
-template <uint32_t W>
-__device__ uint32_t block_excl_scan(uint32_t v, uint32_t *wsum /* W */, uint32_t *total) {
+// Block-wide exclusive scan of one value (u32 or u64) per thread over W waves.
+template <uint32_t W, class V>
+__device__ V block_excl_scan(V v, V *wsum /* W */, V *total) {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    uint32_t x = v;
+    V x = v;
     for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off);
+        V y = __shfl_up(x, off);
         if (lane >= (uint32_t)off) x += y;
     }
     if (lane == 63) wsum[w] = x;
     __syncthreads();
-    uint32_t base = 0, all = 0;
+    V base = 0, all = 0;
     for (uint32_t k = 0; k < W; ++k) {
-        uint32_t s = wsum[k];
+        V s = wsum[k];
         if (k < w) base += s;
         all += s;
     }
@@ -216,7 +216,11 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
     if (valid) {
     uint32_t sc = dims.n_scenes == 1 ? 0 : scene_of_line(scene_line_base, dims.n_scenes, i);
     const SceneDesc &S = scenes[sc];
-    uint32_t id = ids[i];
+    // the scene's own resident lines, or the frame's line arrays
+    const float4 *P = S.lines_pts ? (const float4 *)S.lines_pts : pts;
+    const uint16_t *I = S.lines_ids ? (const uint16_t *)S.lines_ids : ids;
+    const uint32_t li = S.lines_pts ? i - S.line_base : i;
+    uint32_t id = I[li];
     LayerEntry le;
     le.flags = 0;
     if (id < S.n_layer_entries) le = layers[S.layer_base + id];
@@ -228,7 +232,7 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
     bool exists = le.flags & LF_EXISTS;
     if (!(exists && !(le.flags & LF_ENABLED))) {
         uint32_t order = (exists && (le.flags & LF_ORDER)) ? le.order : id;
-        float4 p = pts[i];
+        float4 p = P[li];
         float p0x = p.x, p0y = p.y, p1x = p.z, p1y = p.w;
         if (exists && (le.flags & LF_AFFINE)) {
             const float *t = le.affine;
@@ -1156,14 +1160,16 @@ __device__ __forceinline__ uint32_t merge_split(const uint32_t *qk, uint32_t nq,
 // tiles left of the crop and tile -1 are folded without dropping empty
 // covers; inside the crop, covers that become empty are dropped,
 // layer_workbench.rs:194,475).
-template <uint32_t T>
+// G: the queue and run arrays live in the row's slice of a global scratch
+// buffer instead of LDS (tiles with more layers than LDS holds).
+template <uint32_t T, bool G>
 __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
-    __shared__ uint32_t s_wsum[T / 64];
+    __shared__ uint64_t s_wsum[T / 64];
     __shared__ uint32_t s_rec[3];
     const uint32_t tid = threadIdx.x;
     const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs, tstride = pp.tstride;
-    unsigned char *p = lds_raw;
+    unsigned char *p = G ? pp.walk_scratch + (uint64_t)blockIdx.x * pp.walk_stride : lds_raw;
     uint64_t *qc[2];
     qc[0] = (uint64_t *)p; p += (size_t)cap * 8;
     qc[1] = (uint64_t *)p; p += (size_t)cap * 8;
@@ -1277,9 +1283,10 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
             ++cu;
             cq += keep ? 1u : 0u;
         });
-        uint32_t all;
-        const uint32_t ex = block_excl_scan<T / 64>(cu | (cq << 16), s_wsum, &all);
-        const uint32_t n_union = all & 0xFFFFu, nq_new = all >> 16;
+        uint64_t all;
+        // (union entries, kept queue entries) in the low and high halves
+        const uint64_t ex = block_excl_scan<T / 64>((uint64_t)cu | ((uint64_t)cq << 32), s_wsum, &all);
+        const uint32_t n_union = (uint32_t)all, nq_new = (uint32_t)(all >> 32);
         uint32_t rec_off = 0;
         if (!fold && counting) {
             if (tid == 0) atomicAdd(pp.rec_used, n_union);
@@ -1303,7 +1310,7 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
         }
         const bool write_rec = !fold && !counting;
         const uint32_t nxt = cur ^ 1u;
-        uint32_t ou = rec_off + (ex & 0xFFFFu), oq = ex >> 16;
+        uint32_t ou = rec_off + (uint32_t)ex, oq = (uint32_t)(ex >> 32);
         walk([&](uint32_t lay, int32_t qi, int32_t ri) {
             bool keep;
             const uint64_t v = next_cover(qi, ri, &keep);
@@ -1354,8 +1361,9 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     }
     if (tid == 0 && row_records) atomicAdd(pp.n_records, row_records);
 }
-template __global__ void k_row_walk<256>(PaintParams);
-template __global__ void k_row_walk<1024>(PaintParams);
+template __global__ void k_row_walk<256, false>(PaintParams);
+template __global__ void k_row_walk<1024, false>(PaintParams);
+template __global__ void k_row_walk<1024, true>(PaintParams);
 
 // Staged window of sorted words for the paint wave.
 struct Stage {

@@ -118,6 +118,7 @@ private:
 
     int device_ = -1;
     int cus_ = 0;
+    size_t lds_max_ = 65536;
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     hipEvent_t evs_[8] = {};
@@ -130,7 +131,7 @@ private:
     DevBuf overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
     DevBuf cnt0_, cnt_acc_, cnt_status_;
-    DevBuf work_, runs_work_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
+    DevBuf work_, runs_work_, walk_scratch_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
     uint32_t n_work_ = 0, n_runs_work_ = 0;

@@ -63,7 +63,7 @@ Renderer::~Renderer() {
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
                      &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_nr_,
-                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_work_};
+                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_work_, &walk_scratch_};
     for (DevBuf *b : all) b->release();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -88,6 +88,17 @@ bool Renderer::init(std::string &err) {
     hipDeviceProp_t p;
     MG_HIP(hipGetDeviceProperties(&p, device_), "hipGetDeviceProperties");
     cus_ = p.multiProcessorCount;
+    lds_max_ = p.sharedMemPerBlock;
+    // dynamic LDS limits of the kernels that size their LDS per frame, set
+    // once to the device maximum (the attribute is per kernel, shared by
+    // every engine of the process)
+    for (const void *fn : {(const void *)k_row_walk<256, false>, (const void *)k_row_walk<1024, false>,
+                           (const void *)k_paint}) {
+        hipFuncAttributes a;
+        MG_HIP(hipFuncGetAttributes(&a, fn), "kernel attributes");
+        MG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max_ - a.sharedSizeBytes)),
+               "lds limit");
+    }
     MG_HIP(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
     MG_HIP(hipEventCreate(&ev0_), "hipEventCreate");
     MG_HIP(hipEventCreate(&ev1_), "hipEventCreate");
@@ -630,11 +641,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
 
     // Row-walk LDS: the whole work group's share (one work group per CU is
     // enough for the tile rows); runs of one tile up to rcap.
-    size_t lds_max = 65536;
-    {
-        hipDeviceProp_t p;
-        if (hipGetDeviceProperties(&p, device_) == hipSuccess) lds_max = p.sharedMemPerBlock;
-    }
+    const size_t lds_max = lds_max_;
     // A tile holds at most one run per layer order and the cover queue one
     // entry per order, so the walk never needs more than runs = orders and
     // queue + runs = 2 orders. Frames with few orders walk each row with a
@@ -655,11 +662,12 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         }
         cap = (uint32_t)std::min<size_t>(((lds_max - fixed) / 24) & ~size_t(63), 0xFFFFu - rcap);
     }
-    const size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride), lds_p = paint_lds_bytes_host(cap, clips);
+    size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride), lds_p = paint_lds_bytes_host(cap, clips);
     if (lds_w > lds_max) {
         err = "canvas too wide for the row walk";
         return false;
     }
+    bool global_walk = false;
     pp.cap_ids = cap;
     pp.cap_runs = rcap;
     // id records a row reserves at a time: about a row's worth of tiles
@@ -668,11 +676,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         while (chunk < 16384 && chunk < tstride * 16) chunk <<= 1;
         pp.rec_chunk = chunk;
     }
-    const void *walk_fn = small_walk ? (const void *)k_row_walk<256> : (const void *)k_row_walk<1024>;
     const uint32_t walk_threads = small_walk ? 256 : 1024;
-    MG_HIP(hipFuncSetAttribute(walk_fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_w), "walk lds");
-    MG_HIP(hipFuncSetAttribute((const void *)k_paint, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_p),
-           "paint lds");
     if (rec_cap_ < tiles * 64) rec_cap_ = tiles * 64;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (rec_cap_ > 0xFFFF0000ull) rec_cap_ = 0xFFFF0000ull;
@@ -686,10 +690,12 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
         MG_HIP(hipEventRecord(evs_[5], stream_), "event");
-        if (small_walk)
-            k_row_walk<256><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
+        if (global_walk)
+            k_row_walk<1024, true><<<global_rows, 1024, 0, stream_>>>(pp);
+        else if (small_walk)
+            k_row_walk<256, false><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
         else
-            k_row_walk<1024><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
+            k_row_walk<1024, false><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
         MG_HIP(hipEventRecord(evs_[6], stream_), "event");
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
         MG_HIP(hipEventRecord(evs_[7], stream_), "event");
@@ -702,7 +708,30 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
             n_runs_ = ov[3];
             return true;
         }
-        if (ov[0] & 1u) break;
+        if (ov[0] & 1u) {
+            // a tile holds more layers than the LDS arrays: walk again with
+            // them in global memory, sized so that no tile can exceed them
+            if (global_walk) break;
+            const uint64_t orders = (uint64_t)spec.max_order + 1;
+            rcap = (uint32_t)((orders + 63) & ~63ull);
+            cap = 2 * rcap;
+            lds_p = paint_lds_bytes_host(cap, clips);
+            if (lds_p > lds_max) {
+                err = "a tile holds more clipped layers than the painter's LDS capacity";
+                return false;
+            }
+            const uint64_t stride = (walk_lds_bytes_host(cap, rcap, tstride) + 255) & ~255ull;
+            if (!walk_scratch_.reserve(stride * global_rows)) {
+                err = "out of device memory (row walk)";
+                return false;
+            }
+            pp.walk_scratch = walk_scratch_.as<unsigned char>();
+            pp.walk_stride = stride;
+            pp.cap_ids = cap;
+            pp.cap_runs = rcap;
+            global_walk = true;
+            continue;
+        }
         if (rec_cap_ >= 0xFFFF0000ull) break;
         // the walk kept counting after the store filled up: ov[1] covers the
         // frame's records plus at most one partly used chunk per row

@@ -64,6 +64,10 @@ struct SceneDesc {
     uint32_t flags;
     uint64_t out_offset;              // byte offset of pixel (0, 0) in the output
     float clear[4];
+    // device addresses of the scene's own resident lines (float4 points, u16
+    // layer ids), line i of the frame being their line i - line_base; 0: the
+    // frame's line arrays hold them at line_base
+    uint64_t lines_pts, lines_ids;
 };
 
 // Per-line setup record written by the setup kernel (segment.rs:144-219).

@@ -243,3 +243,23 @@ def glyph_batch(seed=3, n_scenes=4096, gamma=3.0, min_side=32, max_side=512, pat
         w = int(side[s])
         scenes.append(Scene(w, w, lines[a:b], local_id[a:b], props, [1.0, 1.0, 1.0, 1.0], name=f"glyphs-{s}"))
     return scenes
+
+
+SERVE_CANVASES = ((64, 0.5), (256, 0.3), (1024, 0.15), (2048, 0.05))
+
+
+def serve_pool(seed=5, n=32, lines=(1e2, 1e6), canvases=SERVE_CANVASES):
+    """Request scenes of the serving workload: square canvas drawn from
+    `canvases` (side, probability), line count log-uniform in `lines`,
+    generated as the single large frame's polygons (5-50 edges, box 4-512 px,
+    affine, mixed fill rules, blend modes and alpha) on that canvas."""
+    rng = np.random.default_rng(seed)
+    sides = np.array([c for c, _ in canvases])
+    probs = np.array([p for _, p in canvases], dtype=np.float64)
+    out = []
+    for i in range(n):
+        side = int(rng.choice(sides, p=probs / probs.sum()))
+        n_lines = float(np.exp(rng.uniform(np.log(lines[0]), np.log(lines[1]))))
+        count = max(1, int(round(n_lines / 27.5)))  # 27.5 edges per path on average
+        out.append(polygons(seed=seed * 1000 + i, width=side, height=side, count=count))
+    return out

@@ -306,6 +306,59 @@ MG_EXPORT mg_status mg_batch_last_tuning(const mg_batch *batch, mg_batch_tuning 
    its scenes). */
 MG_EXPORT mg_status mg_batch_stats(const mg_batch *batch, mg_frame_stats *out);
 
+/* -------------------------------------------------------------- serving */
+
+/* A render server: requests (a composition and a frame) are admitted by a
+   device-memory budget and rendered as batched frames (continuous batching)
+   by one worker thread per engine, each engine with its own HIP stream, so a
+   batch admitted while another is on the GPU runs beside it. No counterpart
+   in the reference, whose Composition::render (composition.rs:212-328) is a
+   blocking call on the caller's thread; for callers that render many
+   independent scenes as requests arrive. Each request's bytes equal what
+   mg_render_device gives for that composition alone. */
+typedef struct mg_server mg_server;
+
+typedef struct mg_server_config {
+    uint64_t hbm_budget;  /* bytes of device memory in-flight requests may take (0 = half of free) */
+    uint32_t engines;     /* batches in flight at once, 1..4 (0 = 2) */
+    uint32_t max_batch;   /* requests per batch, 1..8192 (0 = 8192) */
+} mg_server_config;
+
+typedef struct mg_request_result {
+    uint64_t ticket;
+    int32_t status;       /* MG_OK or the render's error */
+    uint32_t batch_size;  /* requests in the batch that rendered it */
+    uint32_t engine;
+    uint32_t _pad;
+    double latency_ms;    /* submission -> RGBA8 in the caller's buffer */
+    double queue_ms;      /* submission -> start of its batch */
+    uint64_t footprint;   /* device-memory estimate used for admission */
+} mg_request_result;
+
+typedef struct mg_server_counters {
+    uint64_t completed, failed, batches;
+    double mean_batch;              /* requests per batch */
+    double p50_ms, p99_ms, max_ms;  /* latency of completed requests */
+    uint32_t max_concurrent_batches;
+    uint32_t _pad;
+    uint64_t queued, inflight_bytes;
+} mg_server_counters;
+
+MG_EXPORT mg_status mg_server_create(const mg_server_config *cfg, mg_server **out);
+/* Finishes every submitted request, then stops the workers. */
+MG_EXPORT void mg_server_destroy(mg_server *server);
+MG_EXPORT mg_status mg_server_config_of(const mg_server *server, mg_server_config *out);
+/* Queues a render of `comp` into `frame`; `rgba_out` (len >= width * height
+   * 4 bytes, or NULL) receives the frame. `comp` and `rgba_out` must stay
+   alive, and `comp` unchanged, until mg_server_wait returns for the ticket.
+   MG_ERR_CAPACITY when the request alone exceeds the budget. */
+MG_EXPORT mg_status mg_server_submit(mg_server *server, mg_composition *comp, const mg_batch_frame *frame,
+                                     uint8_t *rgba_out, size_t len, uint64_t *ticket);
+/* Blocks until the request completed; returns its status. */
+MG_EXPORT mg_status mg_server_wait(mg_server *server, uint64_t ticket, mg_request_result *out);
+/* Counts and latency percentiles since the last reset (reset != 0 clears them). */
+MG_EXPORT mg_status mg_server_stats(mg_server *server, int reset, mg_server_counters *out);
+
 /* ------------------------------------------------------ cassia-compatible */
 
 /* cassia_init / cassia_render / cassia_shutdown (cassia/src/Cassia.h:32-41).

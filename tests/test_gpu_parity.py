@@ -85,6 +85,36 @@ def test_many_layers_deep_tile(gpu, cpu):
     assert_same_image(a, b, w)
 
 
+def deep_scene(m, n_layers, w, h, seed):
+    """n_layers quads that each cover most of a small canvas: every tile holds
+    about n_layers layer runs (more than the row walk keeps in LDS)."""
+    rng = np.random.default_rng(seed)
+    comp = m.Composition()
+    for i in range(n_layers):
+        lid = comp.create_layer()
+        x0, y0 = rng.uniform(-6, 3, 2)
+        x1, y1 = rng.uniform(w - 3, w + 6), rng.uniform(h - 3, h + 6)
+        q = np.array([[x0, y0], [x1, y0 + rng.uniform(-2, 2)], [x1, y1], [x0 + rng.uniform(-2, 2), y1]], np.float32)
+        comp.insert_lines(lid, np.concatenate([q, np.roll(q, -1, 0)], 1))
+        style = m.Style(fill=m.Fill.Solid([float(np.float32(v)) for v in rng.uniform(0, 1, 3)] + [0.02]),
+                        blend_mode=m.BlendMode(int(rng.integers(0, 12))))
+        comp.get_mut(lid).set_props(m.Props(fill_rule=m.FillRule(int(rng.integers(0, 2))), func=m.Func.Draw(style)))
+    return comp
+
+
+@pytest.mark.parametrize("n_layers", [3000, 9000])
+def test_tiles_deeper_than_lds(gpu, cpu, n_layers):
+    # 3000 runs per tile exceed the LDS walk's run capacity (2048) and 9000
+    # its queue capacity: the walk reruns with its arrays in global memory
+    w, h = 24, 20
+    outs = []
+    for m in (cpu, gpu):
+        buf = np.full((w * h, 4), 7, np.uint8)
+        deep_scene(m, n_layers, w, h, 77).render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
+        outs.append(buf)
+    assert_same_image(outs[0], outs[1], w)
+
+
 def test_pixel_segments_canonical_stream(gpu, cpu):
     for seed in range(3):
         s_cpu = scenegen.build_scene(cpu, 500 + seed, 90, 70, 25, transforms=True, curves=True).pixel_segments()
