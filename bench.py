@@ -263,6 +263,49 @@ def bench_serving(cassia_amd, workloads, n_requests=160, fractions=(0.25, 0.5, 0
     }
 
 
+def bench_animation(cassia_amd, workloads, frames=240, sample_every=10):
+    """The keyframed animation (workloads.Animation: 1920x1080, 2,000 paths of
+    16-256 lines, per-frame transforms of every layer, 5% of orders swapped,
+    2% restyled) played twice from the same seed: through the captured
+    hipGraph (mg_render_animated) and eagerly (mg_render_device). A frame's
+    time covers its host-side layer updates and the render; every
+    `sample_every`-th frame of both runs is compared byte for byte."""
+    out = {}
+    samples = {}
+    for mode in ("graph", "eager"):
+        A = workloads.Animation(frames=frames)
+        comp = A.scene().build(cassia_amd)
+        lat, dev, replayed = [], [], 0
+        t0 = time.perf_counter()
+        for f in range(frames):
+            a = time.perf_counter()
+            A.apply(cassia_amd, comp, f)
+            if mode == "graph":
+                addr, rep = comp.render_animated(A.width, A.height, [1.0, 1.0, 1.0, 1.0])
+                replayed += int(rep)
+            else:
+                addr = comp.render_device(A.width, A.height, [1.0, 1.0, 1.0, 1.0])
+            lat.append((time.perf_counter() - a) * 1e3)
+            dev.append(comp.stats().device_ms)
+            if f % sample_every == 0:
+                c0 = time.perf_counter()
+                samples.setdefault(f, []).append(cassia_amd.device_frame(addr, A.width, A.height))
+                t0 += time.perf_counter() - c0  # the sample copy is not frame time
+        el = time.perf_counter() - t0
+        st = comp.stats()
+        out[mode] = {"frames_per_s": frames / el, "ms_per_frame": el * 1e3 / frames,
+                     "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
+                     "device_ms_per_frame": float(np.mean(dev)), "replayed_frames": replayed,
+                     "pixel_segments_last_frame": int(st.segments), "lines": int(st.lines)}
+        progress(f"animation {mode}: {out[mode]['ms_per_frame']:.2f} ms/frame")
+    same = all(np.array_equal(v[0], v[1]) for v in samples.values())
+    return {"workload": ("keyframed animation, 1920x1080, 2000 random polygons of 16-256 lines, per-frame affine of "
+                         "every layer, 5% order swaps and 2% restyles per frame"),
+            "frames": frames, **out,
+            "speedup_graph_vs_eager": out["eager"]["ms_per_frame"] / out["graph"]["ms_per_frame"],
+            "sampled_frames_identical": same, "sampled_frames": len(samples)}
+
+
 def measured_traffic(workload):
     """HBM bytes per launch per kernel from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
@@ -304,6 +347,8 @@ def main():
     ap.add_argument("--no-frame", action="store_true",
                     help="measure only the batched many-scenes workload (prints its object alone)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
+    ap.add_argument("--no-anim", action="store_true", help="skip the animation measurement")
+    ap.add_argument("--only-anim", action="store_true", help="measure only the animation workload")
     ap.add_argument("--only-serve", action="store_true", help="measure only the serving workload")
     ap.add_argument("--no-plain", action="store_true",
                     help="with --no-frame: skip the plain per-scene frames and the identity check")
@@ -323,6 +368,9 @@ def main():
     from cassia_amd import workloads
     if cassia_amd.device_count() < 1:
         raise SystemExit("no gfx950 device")
+    if args.only_anim:
+        print(json.dumps({"animation": bench_animation(cassia_amd, workloads)}))
+        return
     if args.only_serve:
         print(json.dumps({"serving": bench_serving(cassia_amd, workloads)}))
         return
@@ -426,6 +474,8 @@ def main():
     if not args.no_batch:
         del comp
         out["batched"] = bench_batched(cassia_amd, workloads, args.steps, args.warmup, args.batch_scenes)
+    if not args.no_anim:
+        out["animation"] = bench_animation(cassia_amd, workloads)
     if not args.no_serve:
         out["serving"] = bench_serving(cassia_amd, workloads)
     if not args.no_cpu:

@@ -161,6 +161,11 @@ _sig("mg_layer_props", C.c_int, _vp, C.c_uint16, C.POINTER(_Props))
 _sig("mg_layer_set_props", C.c_int, _vp, C.c_uint16, C.POINTER(_Props))
 _sig("mg_create_layers", C.c_int, _vp, C.c_size_t, C.POINTER(C.c_uint16))
 _sig("mg_layers_set_props", C.c_int, _vp, C.POINTER(C.c_uint16), _vp, C.c_size_t)
+_sig("mg_layers_set_transform", C.c_int, _vp, C.POINTER(C.c_uint16), _f, C.c_size_t)
+_sig("mg_layers_set_order", C.c_int, _vp, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), C.c_size_t)
+_sig("mg_render_animated", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(_vp),
+     C.POINTER(C.c_int))
+_sig("mg_copy_to_host", C.c_int, _vp, _vp, C.c_size_t)
 _sig("mg_remove_disabled", C.c_int, _vp)
 _sig("mg_lines_len", C.c_int, _vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t))
 _sig("mg_create_buffer_layer_cache", C.c_int, _vp, C.POINTER(C.c_int32))
@@ -678,6 +683,30 @@ class Composition:
             return self.render_coverage(width, height, background_color, crop, cap=int(n.value))
         return a[:n.value], k[:n.value]
 
+    def render_animated(self, width, height, background_color, crop: Optional[Rect] = None):
+        """render_device through a captured hipGraph (mg_render_animated):
+        returns (device address of the frame, whether the graph was replayed)."""
+        bg = _f32(background_color, 4)
+        r = _Rect(*crop.px[:2], *crop.px[2:]) if crop is not None else None
+        out = _vp()
+        rep = C.c_int(0)
+        _check(_lib.mg_render_animated(self._h, width, height, _fp(bg), C.byref(r) if r is not None else None,
+                                       C.byref(out), C.byref(rep)))
+        return out.value, bool(rep.value)
+
+    def set_transforms(self, layer_ids, transforms):
+        """Layer::set_transform on many layers: transforms is (n, 6) float32."""
+        ids = np.ascontiguousarray(np.asarray(layer_ids, dtype=np.uint16).reshape(-1))
+        t = np.ascontiguousarray(np.asarray(transforms, dtype=np.float32).reshape(len(ids), 6))
+        _check(_lib.mg_layers_set_transform(self._h, ids.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(t), len(ids)))
+
+    def set_orders(self, layer_ids, orders):
+        """Layer::set_order on many layers."""
+        ids = np.ascontiguousarray(np.asarray(layer_ids, dtype=np.uint16).reshape(-1))
+        o = np.ascontiguousarray(np.asarray(orders, dtype=np.uint16).reshape(len(ids)))
+        _check(_lib.mg_layers_set_order(self._h, ids.ctypes.data_as(C.POINTER(C.c_uint16)),
+                                        o.ctypes.data_as(C.POINTER(C.c_uint16)), len(ids)))
+
     def set_tuning(self, radix_bits=0):
         """Result-invariant knob: the sort's digit width in bits (4, 6, 8; 0 = automatic)."""
         _check(_lib.mg_set_tuning(self._h, radix_bits))
@@ -686,6 +715,13 @@ class Composition:
         s = FrameStats()
         _check(_lib.mg_last_frame_stats(self._h, C.byref(s)))
         return s
+
+
+def device_frame(address, width, height):
+    """Copy of a device-resident RGBA8 frame as a (height * width, 4) uint8 array."""
+    out = np.zeros((width * height, 4), np.uint8)
+    _check(_lib.mg_copy_to_host(out.ctypes.data, address, out.nbytes))
+    return out
 
 
 class Batch:

@@ -495,6 +495,24 @@ MG_EXPORT mg_status mg_create_layers(mg_composition *c, size_t n, mg_layer_id *o
     return MG_OK;
 }
 
+MG_EXPORT mg_status mg_layers_set_transform(mg_composition *c, const mg_layer_id *ids, const float *t6, size_t n) {
+    if (check_comp(c) || (n && (!ids || !t6))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    for (size_t i = 0; i < n; ++i) {
+        mg_status s = mg_layer_set_transform(c, ids[i], &t6[6 * i]);
+        if (s) return s;
+    }
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_layers_set_order(mg_composition *c, const mg_layer_id *ids, const uint16_t *orders, size_t n) {
+    if (check_comp(c) || (n && (!ids || !orders))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    for (size_t i = 0; i < n; ++i) {
+        mg_status s = mg_layer_set_order(c, ids[i], orders[i]);
+        if (s) return s;
+    }
+    return MG_OK;
+}
+
 MG_EXPORT mg_status mg_layers_set_props(mg_composition *c, const mg_layer_id *ids, const mg_props *props, size_t n) {
     if (check_comp(c) || (n && (!ids || !props))) return fail(MG_ERR_INVALID_ARG, "null argument");
     for (size_t i = 0; i < n; ++i) {
@@ -615,7 +633,8 @@ static mg::SceneDesc scene_desc(size_t width, size_t stride, size_t rows_px, con
 // `stride` into the device image `img` (composition.rs:212-328).
 static mg_status render_frame(mg_composition *c, size_t width, size_t stride, size_t rows_px, const float bg[4],
                               const mg_rect *crop, int32_t cache_id, int32_t *prev, uint8_t *img, size_t img_bytes,
-                              bool want_tile_ops, mg::FrameResult &r, uint32_t coverage_cap = 0) {
+                              bool want_tile_ops, mg::FrameResult &r, uint32_t coverage_cap = 0,
+                              bool *graph_replayed = nullptr) {
     std::string err;
     prepare_tables(c, cache_id);
     mg::FrameSpec &f = c->spec;
@@ -645,8 +664,11 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.radix_bits = c->radix_bits;
     mg_status s;
     if ((s = c->sync_lines())) return s;
-    if (!c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), img, prev, r, err))
-        return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
+    const bool ok = graph_replayed
+                        ? c->rdr->run_graph(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), img, r,
+                                            *graph_replayed, err)
+                        : c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), img, prev, r, err);
+    if (!ok) return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
     if (cache_id >= 0) {
         bool changed = false;
         for (auto &kv : c->layers) {
@@ -734,6 +756,31 @@ MG_EXPORT mg_status mg_render_device(mg_composition *c, uint32_t width, uint32_t
     mg::FrameResult r;
     if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, bytes, false, r))) return s;
     *device_rgba = img;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_render_animated(mg_composition *c, uint32_t width, uint32_t height, const float bg[4],
+                                       const mg_rect *crop, void **device_rgba, int *replayed) {
+    if (check_comp(c) || !bg || !device_rgba || width == 0 || height == 0)
+        return fail(MG_ERR_INVALID_ARG, "bad argument");
+    mg_status s = c->ensure_renderer();
+    if (s) return s;
+    std::lock_guard<std::mutex> g(c->rdr->mu);
+    const size_t bytes = (size_t)width * height * 4;
+    uint8_t *img = c->frame_image(bytes);
+    if (!img) return fail(MG_ERR_OOM, "out of device memory (frame)");
+    mg::FrameResult r;
+    bool rep = false;
+    if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, bytes, false, r, 0, &rep))) return s;
+    *device_rgba = img;
+    if (replayed) *replayed = rep ? 1 : 0;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_copy_to_host(void *dst, const void *device_src, size_t bytes) {
+    if ((!dst || !device_src) && bytes) return fail(MG_ERR_INVALID_ARG, "null argument");
+    if (bytes && hipMemcpy(dst, device_src, bytes, hipMemcpyDeviceToHost) != hipSuccess)
+        return fail(MG_ERR_DEVICE, "device-to-host copy failed");
     return MG_OK;
 }
 

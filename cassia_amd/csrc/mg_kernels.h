@@ -70,7 +70,8 @@ __global__ void k_order_scatter(const LineSetup *setup, const uint32_t *lengths,
                                 const SceneDesc *scenes, uint32_t *base, LineSetup *setup_perm,
                                 uint32_t *lengths_perm);
 __global__ void k_part_lines(const uint32_t *offsets, const uint32_t *lengths, uint64_t n_lines,
-                             uint32_t part_size, uint32_t *part_line);
+                             uint32_t part_size, uint32_t *part_line, uint64_t n_parts);
+__global__ void k_clamp_line_pixels(FrameCounters *counters, uint64_t cap, uint32_t *flag);
 __global__ void k_histogram(const uint64_t *words, FrameCounters *counters, FrameDims dims, uint32_t *hist);
 __global__ void k_row_starts(const uint64_t *sorted, const FrameCounters *counters, uint32_t rows,
                              uint32_t row_shift, uint32_t *row_start);

@@ -617,16 +617,31 @@ __global__ __launch_bounds__(256) void k_order_scatter(const LineSetup *__restri
 }
 
 // part_line[p] = the line whose line-pixels contain p * part_size.
+// (Partitions from n_parts on are not emitted: a frame replayed with a
+// line-pixel capacity stops there.)
 __global__ __launch_bounds__(256) void k_part_lines(const uint32_t *__restrict__ offsets,
                                                     const uint32_t *__restrict__ lengths, uint64_t n_lines,
-                                                    uint32_t part_size, uint32_t *__restrict__ part_line) {
+                                                    uint32_t part_size, uint32_t *__restrict__ part_line,
+                                                    uint64_t n_parts) {
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (i >= n_lines) return;
     const uint32_t len = lengths[i];
     if (!len) return;
     const uint64_t off = offsets[i];
     const uint64_t p0 = (off + part_size - 1) / part_size, p1 = (off + len - 1) / part_size;
-    for (uint64_t p = p0; p <= p1; ++p) part_line[p] = (uint32_t)i;
+    for (uint64_t p = p0; p <= p1 && p < n_parts; ++p) part_line[p] = (uint32_t)i;
+}
+
+// Frames replayed from a captured graph have buffers and grids sized for a
+// line-pixel capacity: a frame beyond it is cut to the capacity (every later
+// kernel reads the count from the counters) and flagged, and the host
+// renders it again without the graph.
+__global__ void k_clamp_line_pixels(FrameCounters *__restrict__ counters, uint64_t cap,
+                                    uint32_t *__restrict__ flag) {
+    if (threadIdx.x == 0 && blockIdx.x == 0 && counters->line_pixels > cap) {
+        counters->line_pixels = cap;
+        *flag = 1u;
+    }
 }
 
 // Digit histograms of every sort pass in one read of the stream (persistent

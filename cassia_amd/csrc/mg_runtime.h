@@ -6,6 +6,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <cstring>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -103,6 +104,13 @@ public:
     // ids; unused by ingest frames) into the device image `image`.
     bool run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
              int32_t *prev_layers, FrameResult &res, std::string &err);
+    // A render frame through a captured hipGraph: replayed while everything
+    // its kernels take as arguments (lines, image, canvas, key layout) is
+    // unchanged, with the frame's tables uploaded beside it. The first frame
+    // of a shape, and any frame beyond the captured capacities, runs eagerly
+    // and captures anew (`replayed` false).
+    bool run_graph(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
+                   FrameResult &res, bool &replayed, std::string &err);
     bool download_sorted(uint64_t *dst, size_t n, std::string &err);
     bool download_tile_ops(int8_t *dst, size_t n, std::string &err);
     bool download_coverage(std::vector<int32_t> &a, std::vector<uint32_t> &key, uint32_t &count,
@@ -112,6 +120,25 @@ public:
     int cu_count() const { return cus_; }
 
 private:
+    struct GraphKey {
+        const void *pts = nullptr, *ids = nullptr, *image = nullptr;
+        size_t n_lines = 0;
+        uint64_t image_bytes = 0;
+        uint32_t radix_bits = 0, max_order = 0, mode = 0;
+        size_t n_props = 0;
+        std::vector<SceneDesc> scenes;
+        bool operator==(const GraphKey &o) const {
+            return pts == o.pts && ids == o.ids && image == o.image && n_lines == o.n_lines &&
+                   image_bytes == o.image_bytes && radix_bits == o.radix_bits && max_order == o.max_order &&
+                   mode == o.mode && n_props == o.n_props && scenes.size() == o.scenes.size() &&
+                   std::memcmp(scenes.data(), o.scenes.data(), scenes.size() * sizeof(SceneDesc)) == 0;
+        }
+    };
+    hipError_t mark(hipEvent_t e) { return capture_P_ ? hipSuccess : hipEventRecord(e, stream_); }
+    bool upload_tables(const FrameSpec &spec, uint32_t global_rows, std::string &err);
+    bool upload_work(const FrameSpec &spec, std::string &err);
+    std::vector<const void *> frame_buffers() const;
+    void drop_graph();
     bool paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
                const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
                std::string &err);
@@ -137,6 +164,12 @@ private:
     uint32_t n_work_ = 0, n_runs_work_ = 0;
     uint64_t n_records_ = 0, n_runs_ = 0;
     const uint64_t *last_sorted_ = nullptr;
+    uint64_t capture_P_ = 0;  // nonzero while a frame is enqueued for capture: its line-pixel capacity
+    hipGraph_t graph_ = nullptr;
+    hipGraphExec_t graph_exec_ = nullptr;
+    GraphKey graph_key_;
+    std::vector<const void *> graph_bufs_;
+    FrameResult graph_res_;
 };
 
 // Number of usable gfx950 devices.

@@ -58,6 +58,7 @@ Renderer::~Renderer() {
         (void)hipSetDevice(device_);
         if (stream_) (void)hipStreamSynchronize(stream_);
     }
+    drop_graph();
     DevBuf *all[] = {&scenes_, &scene_line_base_, &layers_, &props_, &stops_,
                      &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
@@ -289,6 +290,31 @@ static bool props_valid(const FrameSpec &spec, std::string &err) {
     return true;
 }
 
+// Scene descriptors, layer / props / stops tables and the row -> scene map of
+// a frame, uploaded unless the engine holds them already.
+bool Renderer::upload_tables(const FrameSpec &spec, uint32_t global_rows, std::string &err) {
+    const bool same_scenes = spec.tables_version != 0 && spec.tables_owner == uploaded_owner_ &&
+                             spec.tables_version == uploaded_version_ &&
+                             uploaded_scenes_.size() == spec.scenes.size() &&
+                             std::memcmp(uploaded_scenes_.data(), spec.scenes.data(),
+                                         spec.scenes.size() * sizeof(SceneDesc)) == 0;
+    if (same_scenes) return true;
+    if (!props_valid(spec, err)) return false;
+    if (!upload_vec(scenes_, spec.scenes, stream_, err) ||
+        !upload_vec(scene_line_base_, spec.scene_line_base, stream_, err) ||
+        !upload_vec(layers_, spec.layers, stream_, err) || !upload_vec(props_, spec.props, stream_, err) ||
+        !upload_vec(stops_, spec.stops, stream_, err))
+        return false;
+    std::vector<uint32_t> rs(std::max<uint32_t>(global_rows, 1), 0);
+    for (uint32_t s = 0; s < spec.scenes.size(); ++s)
+        for (uint32_t r = 0; r < spec.scenes[s].tile_rows; ++r) rs[spec.scenes[s].row_base + r] = s;
+    if (!upload_vec(row_scene_, rs, stream_, err)) return false;
+    uploaded_owner_ = spec.tables_owner;
+    uploaded_version_ = spec.tables_version;
+    uploaded_scenes_ = spec.scenes;
+    return true;
+}
+
 bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
                    int32_t *prev_layers, FrameResult &res, std::string &err) {
     MG_HIP(hipSetDevice(device_), "hipSetDevice");
@@ -334,29 +360,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     res.sort_bits = sort_bits;
     res.tiles = tiles;
 
-    const bool same_scenes = spec.tables_version != 0 && spec.tables_owner == uploaded_owner_ &&
-                             spec.tables_version == uploaded_version_ &&
-                             uploaded_scenes_.size() == spec.scenes.size() &&
-                             std::memcmp(uploaded_scenes_.data(), spec.scenes.data(),
-                                         spec.scenes.size() * sizeof(SceneDesc)) == 0;
-    if (!same_scenes) {
-        if (!props_valid(spec, err)) return false;
-        if (!upload_vec(scenes_, spec.scenes, stream_, err) ||
-            !upload_vec(scene_line_base_, spec.scene_line_base, stream_, err) ||
-            !upload_vec(layers_, spec.layers, stream_, err) || !upload_vec(props_, spec.props, stream_, err) ||
-            !upload_vec(stops_, spec.stops, stream_, err))
-            return false;
-        uploaded_owner_ = spec.tables_owner;
-        uploaded_version_ = spec.tables_version;
-        uploaded_scenes_ = spec.scenes;
-    }
-    // global row -> scene
-    if (!same_scenes) {
-        std::vector<uint32_t> rs(std::max<uint32_t>(global_rows, 1), 0);
-        for (uint32_t s = 0; s < n_scenes; ++s)
-            for (uint32_t r = 0; r < spec.scenes[s].tile_rows; ++r) rs[spec.scenes[s].row_base + r] = s;
-        if (!upload_vec(row_scene_, rs, stream_, err)) return false;
-    }
+    if (!upload_tables(spec, global_rows, err)) return false;
     const size_t L = n_lines;
     if (!setup_.reserve(std::max<size_t>(L, 1) * sizeof(LineSetup)) || !lengths_.reserve(std::max<size_t>(L, 1) * 4) ||
         !offsets_.reserve(std::max<size_t>(L, 1) * 4)) {
@@ -365,11 +369,12 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     }
     FrameCounters *ctr = counters_.as<FrameCounters>();
     uint32_t *pc = part_counters_.as<uint32_t>();
-    MG_HIP(hipEventRecord(ev0_, stream_), "event");
-    MG_HIP(hipEventRecord(evs_[0], stream_), "event");
+    MG_HIP(mark(ev0_), "event");
+    MG_HIP(mark(evs_[0]), "event");
     fill64((uint64_t *)ctr, 0, sizeof(FrameCounters) / 8, stream_);
     // err_[0]: a line longer than i32 pixels; err_[1]: a look-back wait gave up
-    fill32(err_.as<uint32_t>(), 0, 2, stream_);
+    // err_[2]: the frame exceeded a replayed graph's line-pixel capacity
+    fill32(err_.as<uint32_t>(), 0, 3, stream_);
     uint32_t *stall = err_.as<uint32_t>() + 1;
     fill32(pc, 0, 64, stream_);
     const bool ingest = spec.mode == 2;
@@ -411,7 +416,11 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     FrameCounters hc{};
     uint32_t herr = 0;
     uint64_t P = 0;
-    if (!ingest) {
+    if (!ingest && capture_P_) {
+        // frame of a graph: buffers and grids for the capacity, no host read
+        k_clamp_line_pixels<<<1, 64, 0, stream_>>>(ctr, capture_P_, err_.as<uint32_t>() + 2);
+        P = capture_P_;
+    } else if (!ingest) {
         MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read P");
         MG_HIP(hipMemcpyAsync(&herr, err_.ptr, 4, hipMemcpyDeviceToHost, stream_), "read err");
         MG_HIP(hipStreamSynchronize(stream_), "setup/scan");
@@ -428,7 +437,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         return false;
     }
     res.line_pixels = ingest ? 0 : P;
-    MG_HIP(hipEventRecord(evs_[1], stream_), "event");
+    MG_HIP(mark(evs_[1]), "event");
     const uint64_t cap_segs = std::max<uint64_t>(2 * P, 1);
     const uint64_t nparts_emit = (P + kEmitTileHost - 1) / kEmitTileHost;
     const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
@@ -450,6 +459,8 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         return false;
     }
     fill32(cnt_acc_.as<uint32_t>(), 0, (uint64_t)kCountSlicesHost * rest_rows, stream_);
+    // a capacity frame has partitions beyond the frame's: their counts are zero
+    if (capture_P_) fill32(cnt0_.as<uint32_t>(), 0, n_cnt0, stream_);
     SlotCounts counts{cnt0_.as<uint32_t>(), cnt_acc_.as<uint32_t>()};
     const size_t count_lds = (size_t)count_rows * 4;
     // every line-pixel owns two word slots; the ingest path uses the same layout
@@ -461,7 +472,8 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         if (!part_line_.reserve(nparts_emit * 4)) { err = "out of device memory"; return false; }
         const uint32_t *lens = order_lines ? lengths_perm_.as<uint32_t>() : lengths_.as<uint32_t>();
         k_part_lines<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(offsets_.as<uint32_t>(), lens, L,
-                                                                      kEmitTileHost, part_line_.as<uint32_t>());
+                                                                      kEmitTileHost, part_line_.as<uint32_t>(),
+                                                                      nparts_emit);
         const LineSetup *setup = order_lines ? setup_perm_.as<LineSetup>() : setup_.as<LineSetup>();
         k_emit<<<(uint32_t)nparts_emit, 256, count_lds, stream_>>>(setup, offsets_.as<uint32_t>(),
                                                                  scenes_.as<SceneDesc>(), dims, ctr,
@@ -469,7 +481,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
                                                                  counts);
     }
     MG_HIP(hipGetLastError(), "emit launch");
-    MG_HIP(hipEventRecord(evs_[2], stream_), "event");
+    MG_HIP(mark(evs_[2]), "event");
     if (P) {
         k_count_reduce<<<rest_rows, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), dims, hist_.as<uint32_t>(), ctr);
         fill64(cnt_status_.as<uint64_t>(), 0, cnt_parts, stream_);
@@ -480,7 +492,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     // look-back status words of every digit pass, cleared outside the sort's timing
     fill32(sort_status_.as<uint32_t>(), 0, nparts_sort * radix * dims.passes, stream_);
     MG_HIP(hipGetLastError(), "histogram launch");
-    MG_HIP(hipEventRecord(evs_[3], stream_), "event");
+    MG_HIP(mark(evs_[3]), "event");
     uint64_t *in = segs_a_.as<uint64_t>(), *out = segs_b_.as<uint64_t>();
     const uint32_t sort_grid = (uint32_t)std::max<uint64_t>(nparts_sort, 1);
     for (uint32_t p = 0; p < dims.passes; ++p) {
@@ -502,10 +514,10 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         std::swap(in, out);
     }
     MG_HIP(hipGetLastError(), "sort launch");
-    MG_HIP(hipEventRecord(evs_[4], stream_), "event");
+    MG_HIP(mark(evs_[4]), "event");
     n_runs_ = n_records_ = 0;
     // paint re-records these at its kernel boundaries
-    for (int k = 5; k < 8; ++k) MG_HIP(hipEventRecord(evs_[k], stream_), "event");
+    for (int k = 5; k < 8; ++k) MG_HIP(mark(evs_[k]), "event");
     last_sorted_ = in;
     res.sorted = in;
     if (spec.mode != 1) {
@@ -528,7 +540,8 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, cap_segs, image, prev_layers, tiles, err))
             return false;
     }
-    MG_HIP(hipEventRecord(ev1_, stream_), "event");
+    MG_HIP(mark(ev1_), "event");
+    if (capture_P_) return true;  // the replay reads the results
     MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
     uint32_t hstall = 0;
     MG_HIP(hipMemcpyAsync(&hstall, stall, 4, hipMemcpyDeviceToHost, stream_), "read stall");
@@ -564,32 +577,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         tstride = std::max(tstride, std::min(s.crop_h1, s.row_len) + 2);
         clips |= (s.flags & SF_CLIPS) != 0;
     }
-    if (work_scenes_.size() != spec.scenes.size() ||
-        std::memcmp(work_scenes_.data(), spec.scenes.data(), spec.scenes.size() * sizeof(SceneDesc)) != 0) {
-        // paint: the tiles inside each crop; tile runs: columns 0..h1 (colp)
-        // of the same rows
-        std::vector<uint32_t> work, rwork;
-        for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
-            const SceneDesc &s = spec.scenes[sc];
-            const uint32_t h1 = std::min(s.crop_h1, s.row_len);
-            if (s.crop_h0 >= h1) continue;
-            for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j) {
-                for (uint32_t c = s.crop_h0; c < h1; ++c) {
-                    work.push_back(s.row_base + j);
-                    work.push_back(c);
-                }
-                for (uint32_t c = 0; c <= h1; ++c) {
-                    rwork.push_back(s.row_base + j);
-                    rwork.push_back(c);
-                }
-            }
-        }
-        n_work_ = (uint32_t)(work.size() / 2);
-        n_runs_work_ = (uint32_t)(rwork.size() / 2);
-        if (n_work_ && (!upload_vec(work_, work, stream_, err) || !upload_vec(runs_work_, rwork, stream_, err)))
-            return false;
-        work_scenes_ = spec.scenes;
-    }
+    if (!upload_work(spec, err)) return false;
     const uint32_t n_work = n_work_;
     if (n_work == 0) return true;
     const size_t n_cols = (size_t)global_rows * tstride;
@@ -637,7 +625,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         k_tile_runs<<<blocks, 256, 0, stream_>>>(pp, runs_work_.as<uint2>(), n_runs_work_);
     }
     MG_HIP(hipGetLastError(), "tile runs launch");
-    MG_HIP(hipEventRecord(evs_[5], stream_), "event");
+    MG_HIP(mark(evs_[5]), "event");
 
     // Row-walk LDS: the whole work group's share (one work group per CU is
     // enough for the tile rows); runs of one tile up to rcap.
@@ -689,17 +677,18 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.rec_cap = rec_cap_;
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
-        MG_HIP(hipEventRecord(evs_[5], stream_), "event");
+        MG_HIP(mark(evs_[5]), "event");
         if (global_walk)
             k_row_walk<1024, true><<<global_rows, 1024, 0, stream_>>>(pp);
         else if (small_walk)
             k_row_walk<256, false><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
         else
             k_row_walk<1024, false><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
-        MG_HIP(hipEventRecord(evs_[6], stream_), "event");
+        MG_HIP(mark(evs_[6]), "event");
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
-        MG_HIP(hipEventRecord(evs_[7], stream_), "event");
+        MG_HIP(mark(evs_[7]), "event");
         MG_HIP(hipGetLastError(), "paint launch");
+        if (capture_P_) return true;  // one attempt; the replay reads the flags
         uint32_t ov[4] = {0, 0, 0, 0};
         MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 16, hipMemcpyDeviceToHost, stream_), "read overflow");
         MG_HIP(hipStreamSynchronize(stream_), "paint");
@@ -739,6 +728,159 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     }
     err = (rec_cap_ >= 0xFFFF0000ull) ? "too many tile id records" : "a tile holds more layers than the row walk's capacity";
     return false;
+}
+
+// Paint work list (the tiles inside each crop) and tile-runs work list
+// (columns 0..h1 of the same rows), rebuilt when the scenes change.
+bool Renderer::upload_work(const FrameSpec &spec, std::string &err) {
+    if (work_scenes_.size() != spec.scenes.size() ||
+        std::memcmp(work_scenes_.data(), spec.scenes.data(), spec.scenes.size() * sizeof(SceneDesc)) != 0) {
+        // paint: the tiles inside each crop; tile runs: columns 0..h1 (colp)
+        // of the same rows
+        std::vector<uint32_t> work, rwork;
+        for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
+            const SceneDesc &s = spec.scenes[sc];
+            const uint32_t h1 = std::min(s.crop_h1, s.row_len);
+            if (s.crop_h0 >= h1) continue;
+            for (uint32_t j = s.crop_v0; j < std::min(s.crop_v1, s.tile_rows); ++j) {
+                for (uint32_t c = s.crop_h0; c < h1; ++c) {
+                    work.push_back(s.row_base + j);
+                    work.push_back(c);
+                }
+                for (uint32_t c = 0; c <= h1; ++c) {
+                    rwork.push_back(s.row_base + j);
+                    rwork.push_back(c);
+                }
+            }
+        }
+        n_work_ = (uint32_t)(work.size() / 2);
+        n_runs_work_ = (uint32_t)(rwork.size() / 2);
+        if (n_work_ && (!upload_vec(work_, work, stream_, err) || !upload_vec(runs_work_, rwork, stream_, err)))
+            return false;
+        work_scenes_ = spec.scenes;
+    }
+    return true;
+}
+
+// Every device buffer a frame's kernels address: a captured graph stays valid
+// while none of them moved.
+std::vector<const void *> Renderer::frame_buffers() const {
+    const DevBuf *all[] = {&scenes_, &scene_line_base_, &layers_, &props_, &stops_, &row_scene_, &setup_,
+                           &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_, &segs_b_, &hist_,
+                           &bases_, &sort_status_, &part_counters_, &row_start_, &overflow_, &part_line_,
+                           &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_,
+                           &cnt_status_, &work_, &runs_work_, &tile_start_, &tile_nr_, &run_key_, &run_beg_,
+                           &run_cov_, &rec_, &rec_len_, &rec_hdr_};
+    std::vector<const void *> v;
+    for (const DevBuf *b : all) v.push_back(b->ptr);
+    return v;
+}
+
+void Renderer::drop_graph() {
+    if (graph_exec_) (void)hipGraphExecDestroy(graph_exec_);
+    if (graph_) (void)hipGraphDestroy(graph_);
+    graph_exec_ = nullptr;
+    graph_ = nullptr;
+}
+
+bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines,
+                         uint8_t *image, FrameResult &res, bool &replayed, std::string &err) {
+    replayed = false;
+    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+    // everything the captured kernels take as arguments or grid sizes
+    GraphKey key;
+    key.pts = pts;
+    key.ids = ids;
+    key.image = image;
+    key.n_lines = n_lines;
+    key.image_bytes = spec.image_bytes;
+    key.radix_bits = spec.radix_bits;
+    key.max_order = spec.max_order;
+    key.n_props = spec.props.size();
+    key.mode = spec.mode;
+    key.scenes = spec.scenes;
+    if (graph_exec_ && key == graph_key_ && spec.mode == 0 && !spec.want_tile_ops && !spec.want_coverage) {
+        uint32_t global_rows = 0;
+        for (const SceneDesc &s : spec.scenes) global_rows = std::max(global_rows, s.row_base + s.tile_rows);
+        // this frame's tables and work lists (outside the graph) into the
+        // buffers it reads
+        if (!upload_tables(spec, global_rows, err) || !upload_work(spec, err)) return false;
+        if (frame_buffers() == graph_bufs_) {
+            MG_HIP(hipEventRecord(ev0_, stream_), "event");
+            MG_HIP(hipGraphLaunch(graph_exec_, stream_), "graph launch");
+            MG_HIP(hipEventRecord(ev1_, stream_), "event");
+            FrameCounters hc{};
+            uint32_t herr[3] = {0, 0, 0}, ov[4] = {0, 0, 0, 0};
+            MG_HIP(hipMemcpyAsync(&hc, counters_.ptr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
+            MG_HIP(hipMemcpyAsync(herr, err_.ptr, 12, hipMemcpyDeviceToHost, stream_), "read flags");
+            MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 16, hipMemcpyDeviceToHost, stream_), "read overflow");
+            MG_HIP(hipStreamSynchronize(stream_), "graph frame");
+            if (!herr[0] && !herr[1] && !herr[2] && !ov[0]) {
+                float ms = 0.0f;
+                MG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
+                res = graph_res_;
+                res.line_pixels = hc.line_pixels;
+                res.segments = hc.segments;
+                res.kept = hc.kept;
+                res.device_ms = ms;
+                res.records = ov[2];
+                res.runs = ov[3];
+                for (double &v : res.stage_ms) v = 0.0;
+                replayed = true;
+                return true;
+            }
+            // beyond the captured capacities: render it again below, and capture anew
+        }
+        drop_graph();
+    }
+    drop_graph();
+    if (!run(spec, pts, ids, n_lines, image, nullptr, res, err)) return false;
+    if (spec.mode != 0 || spec.want_tile_ops || spec.want_coverage) return true;
+    // capture the frame with 25% headroom over this one: a run with the
+    // capacity sizes first (it sizes every buffer, so the capture allocates
+    // nothing), then the captured run
+    capture_P_ = std::max<uint64_t>(res.line_pixels + res.line_pixels / 4, 1 << 16);
+    if (2 * capture_P_ >= (1ull << 30)) {
+        capture_P_ = 0;
+        return true;  // too large to capture with headroom: stays eager
+    }
+    rec_cap_ += rec_cap_ / 4;
+    FrameResult tmp;
+    std::string cerr;
+    bool ok = run(spec, pts, ids, n_lines, image, nullptr, tmp, cerr);
+    {
+        // the capacity run must complete like the eager one (same image)
+        uint32_t herr[3] = {0, 0, 0}, ov[4] = {0, 0, 0, 0};
+        ok = ok && hipMemcpyAsync(herr, err_.ptr, 12, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+             hipMemcpyAsync(ov, overflow_.ptr, 16, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
+             hipStreamSynchronize(stream_) == hipSuccess && !herr[0] && !herr[1] && !herr[2] && !ov[0];
+    }
+    if (!ok) {
+        capture_P_ = 0;
+        (void)hipGetLastError();
+        return run(spec, pts, ids, n_lines, image, nullptr, res, err);  // stays eager
+    }
+    hipGraph_t g = nullptr;
+    if (ok) {
+        ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed) == hipSuccess;
+        if (ok) {
+            const bool enq = run(spec, pts, ids, n_lines, image, nullptr, tmp, cerr);
+            ok = hipStreamEndCapture(stream_, &g) == hipSuccess && enq;
+        }
+    }
+    capture_P_ = 0;
+    if (ok) ok = hipGraphInstantiate(&graph_exec_, g, nullptr, nullptr, 0) == hipSuccess;
+    if (!ok) {
+        if (g) (void)hipGraphDestroy(g);
+        graph_exec_ = nullptr;
+        (void)hipGetLastError();
+        return true;  // rendered eagerly; later frames try to capture again
+    }
+    graph_ = g;
+    graph_key_ = key;
+    graph_bufs_ = frame_buffers();
+    graph_res_ = res;
+    return true;
 }
 
 bool Renderer::download_sorted(uint64_t *dst, size_t n, std::string &err) {

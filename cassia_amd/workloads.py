@@ -263,3 +263,103 @@ def serve_pool(seed=5, n=32, lines=(1e2, 1e6), canvases=SERVE_CANVASES):
         count = max(1, int(round(n_lines / 27.5)))  # 27.5 edges per path on average
         out.append(polygons(seed=seed * 1000 + i, width=side, height=side, count=count))
     return out
+
+
+class Animation:
+    """Keyframed animation over one composition: `paths` random polygons
+    (U{16..256} edges, 50% star / 50% simple, box side log-uniform in
+    [4, 512] px, centred on the origin), one layer per path with random
+    colour, alpha in [0.2, 1], fill rule and blend mode. Every path has an
+    affine transform (angle, scale in [0.5, 1], translation over the canvas)
+    interpolated linearly between keyframes every `key_every` frames and set
+    with Layer::set_transform; every frame also swaps the orders of `swap`
+    random layer pairs (orders stay a permutation) and gives `restyle`
+    random layers new props. All values are float32 and seeded."""
+
+    def __init__(self, seed=4, width=1920, height=1080, paths=2000, edges=(16, 256), frames=240, key_every=30,
+                 swap_frac=0.05, restyle_frac=0.02, box=(4.0, 512.0)):
+        self.width, self.height, self.paths, self.frames = width, height, paths, frames
+        rng = np.random.default_rng(seed)
+        k = rng.integers(edges[0], edges[1] + 1, paths)
+        side = np.exp(rng.uniform(np.log(box[0]), np.log(box[1]), paths)).astype(np.float32)
+        star = rng.random(paths) < 0.5
+        total = int(k.sum())
+        path_of = np.repeat(np.arange(paths), k)
+        first = np.concatenate([[0], np.cumsum(k)[:-1]])
+        local = (rng.random((total, 2)).astype(np.float32) - np.float32(0.5)) * side[path_of, None]
+        ang = np.arctan2(local[:, 1], local[:, 0])
+        order = np.lexsort((np.arange(total), path_of * 8.0 + np.where(star[path_of], 0.0, ang + 4.0)))
+        local = local[order]
+        nxt = np.arange(total) + 1
+        nxt[first + k - 1] = first
+        self.lines = np.concatenate([local, local[nxt]], axis=1).astype(np.float32)
+        self.layer_ids = path_of.astype(np.uint16)
+        self._rng = np.random.default_rng(seed + 1)
+        self.colors, self.rules, self.blends = self._styles(paths)
+        n_keys = frames // key_every + 2
+        self.key_every = key_every
+        self.theta = self._rng.uniform(0, 2 * np.pi, (n_keys, paths)).astype(np.float32)
+        self.scale = self._rng.uniform(0.5, 1.0, (n_keys, paths)).astype(np.float32)
+        self.tx = self._rng.uniform(0, width, (n_keys, paths)).astype(np.float32)
+        self.ty = self._rng.uniform(0, height, (n_keys, paths)).astype(np.float32)
+        self.n_swap = int(round(paths * swap_frac / 2))
+        self.n_restyle = int(round(paths * restyle_frac))
+        self.orders = np.arange(paths, dtype=np.uint16)
+        self.seed = seed
+
+    def _styles(self, n):
+        cols = self._rng.random((n, 4)).astype(np.float32)
+        cols[:, 3] = self._rng.uniform(0.2, 1.0, n).astype(np.float32)
+        return cols, self._rng.integers(0, 2, n).astype(np.uint32), self._rng.integers(0, 12, n).astype(np.uint32)
+
+    def scene(self):
+        props = dict(color=self.colors, rule=self.rules, blend=self.blends)
+        return Scene(self.width, self.height, self.lines, self.layer_ids, props, [1.0, 1.0, 1.0, 1.0],
+                     name=f"animation-{self.width}x{self.height}-{self.paths}")
+
+    def transforms(self, f):
+        """(paths, 6) float32 [a, b, c, d, tx, ty] of frame f: keyframes
+        interpolated as k0 + (k1 - k0) * w in float32."""
+        i = f // self.key_every
+        w = np.float32((f % self.key_every) / self.key_every)
+        lerp = lambda a: (a[i] + (a[i + 1] - a[i]) * w).astype(np.float32)
+        th, sc, tx, ty = lerp(self.theta), lerp(self.scale), lerp(self.tx), lerp(self.ty)
+        c = (np.cos(th).astype(np.float32) * sc).astype(np.float32)
+        s = (np.sin(th).astype(np.float32) * sc).astype(np.float32)
+        return np.stack([c, -s, s, c, tx, ty], axis=1).astype(np.float32)
+
+    def updates(self, f):
+        """Changes of frame f >= 1 (frame 0 only sets transforms): layer order
+        swaps (ids, new orders) and restyled layers (ids, colors, rules, blends).
+        Advances the animation state; call once per frame, in order."""
+        if f == 0:
+            return (np.zeros(0, np.uint16), np.zeros(0, np.uint16)), (np.zeros(0, np.uint16),) + self._styles(0)
+        pick = self._rng.choice(self.paths, 2 * self.n_swap, replace=False)
+        a, b = pick[: self.n_swap], pick[self.n_swap:]
+        oa, ob = self.orders[a].copy(), self.orders[b].copy()
+        self.orders[a], self.orders[b] = ob, oa
+        ids = np.concatenate([a, b]).astype(np.uint16)
+        restyle = self._rng.choice(self.paths, self.n_restyle, replace=False).astype(np.uint16)
+        cols, rules, blends = self._styles(self.n_restyle)
+        self.colors[restyle], self.rules[restyle], self.blends[restyle] = cols, rules, blends
+        return (ids, self.orders[ids].copy()), (restyle, cols, rules, blends)
+
+    def apply(self, m, comp, f):
+        """Frame f's changes on composition `comp` of module `m`."""
+        t = self.transforms(f)
+        (oid, orders), (sid, cols, rules, blends) = self.updates(f)
+        ids = np.arange(self.paths, dtype=np.uint16)
+        if hasattr(comp, "set_transforms"):
+            comp.set_transforms(ids, t)
+            if len(oid):
+                comp.set_orders(oid, orders)
+            if len(sid):
+                comp.set_solid_props(sid, cols, rules, blends)
+        else:
+            for i in range(self.paths):
+                comp.get_mut(i).set_transform([float(v) for v in t[i]])
+            for i, o in zip(oid.tolist(), orders.tolist()):
+                comp.get_mut(i).set_order(int(o))
+            for k, i in enumerate(sid.tolist()):
+                comp.get_mut(i).set_props(props_of(m, dict(color=[float(v) for v in cols[k]], rule=int(rules[k]),
+                                                           blend=int(blends[k]))))

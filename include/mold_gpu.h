@@ -153,10 +153,15 @@ MG_EXPORT mg_status mg_layer_set_order(mg_composition *comp, mg_layer_id id, uin
 MG_EXPORT mg_status mg_layer_props(const mg_composition *comp, mg_layer_id id, mg_props *out);
 MG_EXPORT mg_status mg_layer_set_props(mg_composition *comp, mg_layer_id id, const mg_props *props);
 /* Bulk forms for scenes with many layers: n calls of create_layer (ids out,
-   stops at the first failure) and of Layer::set_props (layer.rs:231-238) on
+   stops at the first failure) and of Layer::set_props (layer.rs:224-231) on
    ids[i] with props[i]. */
 MG_EXPORT mg_status mg_create_layers(mg_composition *comp, size_t n, mg_layer_id *out);
 MG_EXPORT mg_status mg_layers_set_props(mg_composition *comp, const mg_layer_id *ids, const mg_props *props,
+                                        size_t n);
+/* Bulk Layer::set_transform (layer.rs:182-201; t6 = n x 6 floats) and
+   Layer::set_order (layer.rs:209-216). */
+MG_EXPORT mg_status mg_layers_set_transform(mg_composition *comp, const mg_layer_id *ids, const float *t6, size_t n);
+MG_EXPORT mg_status mg_layers_set_order(mg_composition *comp, const mg_layer_id *ids, const uint16_t *orders,
                                         size_t n);
 /* Composition::remove_disabled (composition.rs:123-142) */
 MG_EXPORT mg_status mg_remove_disabled(mg_composition *comp);
@@ -206,6 +211,22 @@ MG_EXPORT mg_status mg_render(mg_composition *comp, const mg_buffer *buffer,
 MG_EXPORT mg_status mg_render_device(mg_composition *comp, uint32_t width, uint32_t height,
                                      const float background[4], const mg_rect *crop,
                                      void **device_rgba);
+
+/* mg_render_device through a captured hipGraph, for animations: while the
+   composition's lines, the canvas and the layer count stay the same, each
+   frame uploads its layer tables (transforms, orders, props) and replays the
+   graph captured from an earlier frame. The first frame of a shape, and a
+   frame whose pixel work exceeds the captured capacities, render eagerly and
+   capture anew; `*replayed` (may be NULL) says which. Same bytes as
+   mg_render_device. No counterpart in the reference (composition.rs:212-328
+   renders every frame from scratch). */
+MG_EXPORT mg_status mg_render_animated(mg_composition *comp, uint32_t width, uint32_t height,
+                                       const float background[4], const mg_rect *crop, void **device_rgba,
+                                       int *replayed);
+
+/* Copies `bytes` of device memory (a frame from mg_render_device or
+   mg_render_animated) to host memory. */
+MG_EXPORT mg_status mg_copy_to_host(void *dst, const void *device_src, size_t bytes);
 
 /* Composition::pixel_segments (composition.rs:153-188): the canonically
    sorted PixelSegment stream (stable by `seg >> 22` over generation order,
