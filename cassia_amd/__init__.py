@@ -69,7 +69,7 @@ class FrameStats(C.Structure):
                 ("radix_bits", C.c_uint32), ("device_ms", C.c_double), ("setup_ms", C.c_double),
                 ("emit_ms", C.c_double), ("hist_ms", C.c_double), ("sort_ms", C.c_double),
                 ("runs_ms", C.c_double), ("walk_ms", C.c_double), ("paint_ms", C.c_double),
-                ("runs", C.c_uint64), ("records", C.c_uint64)]
+                ("runs", C.c_uint64), ("records", C.c_uint64), ("walk_threads", C.c_uint32), ("_pad", C.c_uint32)]
 
 
 def _props_dtype():
@@ -99,7 +99,9 @@ class _BatchFrame(C.Structure):
 
 class BatchTuning(C.Structure):
     _fields_ = [("n_candidates", C.c_uint32), ("radix_bits", C.c_uint32 * 4), ("device_ms", C.c_double * 4),
-                ("chosen_radix_bits", C.c_uint32)]
+                ("chosen_radix_bits", C.c_uint32), ("n_walk_candidates", C.c_uint32), ("walk_threads", C.c_uint32 * 3),
+                ("_pad", C.c_uint32), ("walk_ms", C.c_double * 3), ("chosen_walk_threads", C.c_uint32),
+                ("_pad2", C.c_uint32)]
 
 
 class ServerConfig(C.Structure):
@@ -176,6 +178,8 @@ _sig("mg_render_device", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Re
 _sig("mg_render_coverage", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(C.c_int32),
      C.POINTER(C.c_uint32), C.c_size_t, C.POINTER(C.c_size_t))
 _sig("mg_set_tuning", C.c_int, _vp, C.c_uint32)
+_sig("mg_set_walk_threads", C.c_int, _vp, C.c_uint32)
+_sig("mg_batch_set_walk_threads", C.c_int, _vp, C.c_uint32)
 _sig("mg_pixel_segments", C.c_int, _vp, C.POINTER(C.POINTER(C.c_uint64)), C.POINTER(C.c_size_t))
 _sig("mg_last_frame_stats", C.c_int, _vp, C.POINTER(FrameStats))
 _sig("mg_batch_create", C.c_int, C.POINTER(_vp))
@@ -708,8 +712,12 @@ class Composition:
                                         o.ctypes.data_as(C.POINTER(C.c_uint16)), len(ids)))
 
     def set_tuning(self, radix_bits=0):
-        """Result-invariant knob: the sort's digit width in bits (4, 6, 8; 0 = automatic)."""
+        """Result-invariant knob: the sort's digit width in bits (4, 6, 8, 9; 0 = automatic)."""
         _check(_lib.mg_set_tuning(self._h, radix_bits))
+
+    def set_walk_threads(self, threads=0):
+        """Result-invariant knob: threads per tile row of the row walk (64, 256, 1024; 0 = automatic)."""
+        _check(_lib.mg_set_walk_threads(self._h, threads))
 
     def stats(self):
         s = FrameStats()
@@ -779,9 +787,12 @@ class Batch:
         _check(_lib.mg_batch_read(self._h, i, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.nbytes))
         return out
 
-    def set_tuning(self, radix_bits=0):
-        """Sort digit width (4, 6, 8 or 9), or 0 = autotune on the next render."""
+    def set_tuning(self, radix_bits=0, walk_threads=None):
+        """Sort digit width (4, 6, 8 or 9) and, if given, row-walk threads (64,
+        256, 1024); 0 = autotune on the next render."""
         _check(_lib.mg_batch_set_tuning(self._h, radix_bits))
+        if walk_threads is not None:
+            _check(_lib.mg_batch_set_walk_threads(self._h, walk_threads))
 
     def tuning(self):
         t = BatchTuning()

@@ -133,6 +133,7 @@ struct mg_composition {
     mg_frame_stats stats{};
     uint64_t version = 1;      // bumped by every change of layers, props or lines
     uint32_t radix_bits = 0;
+    uint32_t walk_threads = 0;
     // frame tables (layers by id, props by order) of the last render, valid
     // while `version` and the layer cache id are unchanged
     mg::FrameSpec spec;
@@ -601,6 +602,7 @@ static void fill_stats(mg_frame_stats &st, const mg::FrameResult &r, size_t line
     st.paint_ms = r.stage_ms[6];
     st.runs = r.runs;
     st.records = r.records;
+    st.walk_threads = r.walk_threads;
 }
 
 // Scene descriptor of one frame of `width` x `rows_px` pixels at `stride`
@@ -662,6 +664,7 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.want_coverage = coverage_cap > 0;
     f.coverage_cap = coverage_cap;
     f.radix_bits = c->radix_bits;
+    f.walk_threads = c->walk_threads;
     mg_status s;
     if ((s = c->sync_lines())) return s;
     const bool ok = graph_replayed
@@ -850,6 +853,14 @@ MG_EXPORT mg_status mg_set_tuning(mg_composition *c, uint32_t radix_bits) {
     return MG_OK;
 }
 
+MG_EXPORT mg_status mg_set_walk_threads(mg_composition *c, uint32_t threads) {
+    if (check_comp(c)) return MG_ERR_INVALID_ARG;
+    if (threads != 0 && threads != 64 && threads != 256 && threads != 1024)
+        return fail(MG_ERR_INVALID_ARG, "walk threads must be 0, 64, 256 or 1024");
+    c->walk_threads = threads;
+    return MG_OK;
+}
+
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *c, mg_frame_stats *out) {
     if (check_comp(c) || !out) return fail(MG_ERR_INVALID_ARG, "null argument");
     *out = c->stats;
@@ -872,9 +883,10 @@ struct mg_batch {
     mg::FrameSpec spec;
     size_t n_lines = 0;
     std::vector<uint64_t> offsets, bytes;
-    uint32_t radix_bits = 0;  // 0 = autotune
-    // autotuner: the choice holds for batches with the same sort width
-    uint32_t tuned_bits = 0, tuned_for = ~0u;
+    uint32_t radix_bits = 0;    // 0 = autotune
+    uint32_t walk_threads = 0;  // 0 = autotune
+    // autotuner: the choice holds for batches of the same shape
+    uint32_t tuned_bits = 0, tuned_walk = 0, tuned_for = ~0u;
     mg_batch_tuning tuning{};
     mg_frame_stats stats{};
 
@@ -1010,7 +1022,16 @@ MG_EXPORT mg_status mg_batch_set_tuning(mg_batch *b, uint32_t radix_bits) {
     if (radix_bits != 0 && radix_bits != 4 && radix_bits != 6 && radix_bits != 8 && radix_bits != 9)
         return fail(MG_ERR_INVALID_ARG, "radix_bits must be 0, 4, 6, 8 or 9");
     b->radix_bits = radix_bits;
-    if (!radix_bits) b->tuned_for = ~0u;
+    b->tuned_for = ~0u;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_batch_set_walk_threads(mg_batch *b, uint32_t threads) {
+    if (!b) return fail(MG_ERR_INVALID_ARG, "batch is null");
+    if (threads != 0 && threads != 64 && threads != 256 && threads != 1024)
+        return fail(MG_ERR_INVALID_ARG, "walk threads must be 0, 64, 256 or 1024");
+    b->walk_threads = threads;
+    b->tuned_for = ~0u;
     return MG_OK;
 }
 
@@ -1063,46 +1084,76 @@ mg_status batch_render(mg_batch *b, mg_composition *const *comps, const mg_batch
     mg::FrameSpec &f = b->spec;
     std::string err;
     mg::FrameResult r;
-    auto run = [&](uint32_t bits) -> mg_status {
+    auto run = [&](uint32_t bits, uint32_t walk) -> mg_status {
         f.radix_bits = bits;
+        f.walk_threads = walk;
         if (!b->rdr->run(f, nullptr, nullptr, b->n_lines, b->frame.as<uint8_t>(), nullptr, r, err))
             return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
         return MG_OK;
     };
-    uint32_t bits = b->radix_bits;
-    if (!bits) {
-        // the autotuner: a choice holds for later batches with the same sort
-        // key width and the same power-of-two class of line counts
+    uint32_t bits = b->radix_bits, walk = b->walk_threads;
+    if (!bits || !walk) {
+        // the autotuner: each free knob in turn (radix with the fixed or
+        // automatic walk size, then walk sizes with the chosen radix), each
+        // candidate timed twice on this batch (the first run also sizes
+        // scratch buffers); the choice holds for later batches with the same
+        // sort key width and power-of-two class of line counts
         uint32_t lg = 0;
         while (lg < 63 && (b->n_lines >> lg)) ++lg;
         const uint32_t key = mg::key_layout(f).sort_bits * 64 + lg;
         if (b->tuned_for != key) {
-            static const uint32_t cand[4] = {4, 6, 8, 9};
-            mg_batch_tuning t{};
-            t.n_candidates = 4;
-            double best = 0.0;
-            uint32_t best_bits = 0;
-            for (int k = 0; k < 4; ++k) {
-                double ms = 0.0;
-                for (int rep = 0; rep < 2; ++rep) {  // the first run also sizes scratch buffers
-                    if ((s = run(cand[k]))) return s;
-                    ms = rep == 0 ? r.device_ms : std::min(ms, r.device_ms);
+            auto time_of = [&](uint32_t rb, uint32_t wt, double *ms) -> mg_status {
+                for (int rep = 0; rep < 2; ++rep) {
+                    mg_status e = run(rb, wt);
+                    if (e) return e;
+                    *ms = rep == 0 ? r.device_ms : std::min(*ms, r.device_ms);
                 }
-                t.radix_bits[k] = cand[k];
-                t.device_ms[k] = ms;
-                if (!best_bits || ms < best) {
-                    best = ms;
-                    best_bits = cand[k];
+                return MG_OK;
+            };
+            mg_batch_tuning t{};
+            uint32_t best_bits = bits;
+            if (!bits) {
+                static const uint32_t cand[4] = {4, 6, 8, 9};
+                t.n_candidates = 4;
+                double best = 0.0;
+                for (int k = 0; k < 4; ++k) {
+                    double ms = 0.0;
+                    if ((s = time_of(cand[k], walk, &ms))) return s;
+                    t.radix_bits[k] = cand[k];
+                    t.device_ms[k] = ms;
+                    if (!best_bits || ms < best) {
+                        best = ms;
+                        best_bits = cand[k];
+                    }
+                }
+            }
+            uint32_t best_walk = walk;
+            if (!walk) {
+                static const uint32_t cand[3] = {64, 256, 1024};
+                t.n_walk_candidates = 3;
+                double best = 0.0;
+                for (int k = 0; k < 3; ++k) {
+                    double ms = 0.0;
+                    if ((s = time_of(best_bits, cand[k], &ms))) return s;
+                    t.walk_threads[k] = cand[k];
+                    t.walk_ms[k] = ms;
+                    if (!best_walk || ms < best) {
+                        best = ms;
+                        best_walk = cand[k];
+                    }
                 }
             }
             t.chosen_radix_bits = best_bits;
+            t.chosen_walk_threads = best_walk;
             b->tuning = t;
             b->tuned_bits = best_bits;
+            b->tuned_walk = best_walk;
             b->tuned_for = key;
         }
         bits = b->tuned_bits;
+        walk = b->tuned_walk;
     }
-    if ((s = run(bits))) return s;
+    if ((s = run(bits, walk))) return s;
     fill_stats(b->stats, r, b->n_lines);
     if (device_rgba) *device_rgba = b->frame.as<uint8_t>();
     if (offsets) *offsets = b->offsets.data();

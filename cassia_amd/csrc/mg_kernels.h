@@ -85,7 +85,7 @@ __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounte
                             uint32_t *stall, const uint32_t *first_off);
 __global__ void k_tile_starts(PaintParams pp, uint32_t global_rows);
 __global__ void k_tile_runs(PaintParams pp, const uint2 *work, uint32_t n_work);
-// T = 256 or 1024 threads per tile row; G: queue in global scratch
+// T = 64, 256 or 1024 threads per tile row; G: queue in global scratch
 template <uint32_t T, bool G>
 __global__ void k_row_walk(PaintParams pp);
 __global__ void k_paint(PaintParams pp);

@@ -1376,8 +1376,11 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     }
     if (tid == 0 && row_records) atomicAdd(pp.n_records, row_records);
 }
+template __global__ void k_row_walk<64, false>(PaintParams);
 template __global__ void k_row_walk<256, false>(PaintParams);
 template __global__ void k_row_walk<1024, false>(PaintParams);
+template __global__ void k_row_walk<64, true>(PaintParams);
+template __global__ void k_row_walk<256, true>(PaintParams);
 template __global__ void k_row_walk<1024, true>(PaintParams);
 
 // Staged window of sorted words for the paint wave.

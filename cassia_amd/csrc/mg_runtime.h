@@ -60,6 +60,7 @@ struct FrameSpec {
     const uint64_t *in_segs = nullptr;  // mode 2: device PixelSegment stream
     uint64_t n_in = 0;
     uint32_t radix_bits = 0;            // sort digit width; 0 = automatic
+    uint32_t walk_threads = 0;          // row-walk work-group size (64, 256, 1024); 0 = automatic
     bool want_tile_ops = false;
     bool want_coverage = false;
     uint32_t coverage_cap = 0;
@@ -71,7 +72,7 @@ struct FrameSpec {
 
 struct FrameResult {
     uint64_t line_pixels = 0, segments = 0, kept = 0;
-    uint32_t passes = 0, radix_bits = 0, sort_bits = 0;
+    uint32_t passes = 0, radix_bits = 0, sort_bits = 0, walk_threads = 0;
     uint64_t tiles = 0;
     double device_ms = 0.0;
     // setup+scan, emit, histogram+bases, sort passes, tile runs, row walk, paint
@@ -141,7 +142,7 @@ private:
     void drop_graph();
     bool paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
                const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
-               std::string &err);
+               uint32_t &walk_threads, std::string &err);
 
     int device_ = -1;
     int cus_ = 0;

@@ -93,8 +93,8 @@ bool Renderer::init(std::string &err) {
     // dynamic LDS limits of the kernels that size their LDS per frame, set
     // once to the device maximum (the attribute is per kernel, shared by
     // every engine of the process)
-    for (const void *fn : {(const void *)k_row_walk<256, false>, (const void *)k_row_walk<1024, false>,
-                           (const void *)k_paint}) {
+    for (const void *fn : {(const void *)k_row_walk<64, false>, (const void *)k_row_walk<256, false>,
+                           (const void *)k_row_walk<1024, false>, (const void *)k_paint}) {
         hipFuncAttributes a;
         MG_HIP(hipFuncGetAttributes(&a, fn), "kernel attributes");
         MG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max_ - a.sharedSizeBytes)),
@@ -537,7 +537,8 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
             }
             fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         }
-        if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, cap_segs, image, prev_layers, tiles, err))
+        if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, cap_segs, image, prev_layers, tiles,
+                   res.walk_threads, err))
             return false;
     }
     MG_HIP(mark(ev1_), "event");
@@ -567,7 +568,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
 
 bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
                      const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
-                     std::string &err) {
+                     uint32_t &walk_threads_out, std::string &err) {
     if (global_rows == 0) return true;
     // one paint work group per tile inside the crop; the list is rebuilt
     // only when the scene descriptors change
@@ -632,14 +633,21 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     const size_t lds_max = lds_max_;
     // A tile holds at most one run per layer order and the cover queue one
     // entry per order, so the walk never needs more than runs = orders and
-    // queue + runs = 2 orders. Frames with few orders walk each row with a
-    // 256-thread work group in a small LDS share (several rows per CU);
-    // others with 1024 threads and the whole LDS.
+    // queue + runs = 2 orders. Frames whose arrays sized that way fit in a
+    // LDS share walk each row with 256 threads (several rows per CU when the
+    // share is small); others use 1024 threads and the whole LDS, which holds
+    // the deepest tiles (measured on MI355X: 256 threads are 12% faster for a
+    // 2,000-order frame, 1024 threads 34% faster for a 65,536-order one). The
+    // work-group size is a result-invariant tuning knob (FrameSpec::walk_threads).
     const uint64_t orders = (uint64_t)spec.max_order + 1;
-    const bool small_walk = orders <= 512;
+    const uint32_t rcap_fit = (uint32_t)((orders + 63) & ~63ull);
+    const bool fits = orders <= 2048 && walk_lds_bytes_host(2 * rcap_fit, rcap_fit, tstride) + 256 <= lds_max;
+    uint32_t walk_threads = spec.walk_threads;
+    if (walk_threads != 64 && walk_threads != 256 && walk_threads != 1024) walk_threads = fits ? 256 : 1024;
+    const bool small_walk = fits;
     uint32_t cap, rcap;
     if (small_walk) {
-        rcap = (uint32_t)((orders + 63) & ~63ull);
+        rcap = rcap_fit;
         cap = 2 * rcap;
     } else {
         rcap = 2048;
@@ -650,6 +658,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         }
         cap = (uint32_t)std::min<size_t>(((lds_max - fixed) / 24) & ~size_t(63), 0xFFFFu - rcap);
     }
+    walk_threads_out = walk_threads;
     size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride), lds_p = paint_lds_bytes_host(cap, clips);
     if (lds_w > lds_max) {
         err = "canvas too wide for the row walk";
@@ -664,7 +673,6 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         while (chunk < 16384 && chunk < tstride * 16) chunk <<= 1;
         pp.rec_chunk = chunk;
     }
-    const uint32_t walk_threads = small_walk ? 256 : 1024;
     if (rec_cap_ < tiles * 64) rec_cap_ = tiles * 64;
     for (int attempt = 0; attempt < 8; ++attempt) {
         if (rec_cap_ > 0xFFFF0000ull) rec_cap_ = 0xFFFF0000ull;
@@ -678,12 +686,16 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
         MG_HIP(mark(evs_[5]), "event");
-        if (global_walk)
-            k_row_walk<1024, true><<<global_rows, 1024, 0, stream_>>>(pp);
-        else if (small_walk)
-            k_row_walk<256, false><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
+        const size_t lds = global_walk ? 0 : lds_w;
+        if (walk_threads == 64)
+            global_walk ? k_row_walk<64, true><<<global_rows, 64, 0, stream_>>>(pp)
+                        : k_row_walk<64, false><<<global_rows, 64, lds, stream_>>>(pp);
+        else if (walk_threads == 256)
+            global_walk ? k_row_walk<256, true><<<global_rows, 256, 0, stream_>>>(pp)
+                        : k_row_walk<256, false><<<global_rows, 256, lds, stream_>>>(pp);
         else
-            k_row_walk<1024, false><<<global_rows, walk_threads, lds_w, stream_>>>(pp);
+            global_walk ? k_row_walk<1024, true><<<global_rows, 1024, 0, stream_>>>(pp)
+                        : k_row_walk<1024, false><<<global_rows, 1024, lds, stream_>>>(pp);
         MG_HIP(mark(evs_[6]), "event");
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
         MG_HIP(mark(evs_[7]), "event");

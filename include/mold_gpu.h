@@ -253,6 +253,9 @@ MG_EXPORT mg_status mg_render_coverage(mg_composition *comp, uint32_t width, uin
    digit width in bits (4, 6, 8 or 9). Every setting produces the same bytes;
    only the speed changes. */
 MG_EXPORT mg_status mg_set_tuning(mg_composition *comp, uint32_t radix_bits);
+/* Second result-invariant knob: threads per tile row in the row walk (64,
+   256 or 1024; 0 = automatic). */
+MG_EXPORT mg_status mg_set_walk_threads(mg_composition *comp, uint32_t threads);
 
 /* Statistics of the last render: lines, line-pixels, non-NONE segments,
    segments kept for painting, tiles, digit passes, radix bits. */
@@ -269,6 +272,8 @@ typedef struct mg_frame_stats {
     double walk_ms;            /* row-serial cover carry (id records)    */
     double paint_ms;           /* tile paint: workbench + painter        */
     uint64_t runs, records;    /* tile layer runs, tile id records       */
+    uint32_t walk_threads;     /* row-walk work-group size used          */
+    uint32_t _pad;
 } mg_frame_stats;
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
 
@@ -308,18 +313,27 @@ MG_EXPORT mg_status mg_batch_render(mg_batch *batch, mg_composition *const *comp
    host memory `rgba` of `len` bytes. */
 MG_EXPORT mg_status mg_batch_read(mg_batch *batch, size_t i, uint8_t *rgba, size_t len);
 
-/* Radix bits of the batch's sort (4, 6, 8 or 9), or 0 to let the autotuner
-   choose: the next render times each candidate on the batch itself and keeps
-   the fastest for batches of the same shape. Result-invariant. */
+/* Radix bits of the batch's sort (4, 6, 8 or 9) and threads per row of its
+   row walk (64, 256 or 1024), or 0 to let the autotuner choose: the next
+   render times each radix on the batch itself, then each walk size with the
+   fastest radix, and keeps the fastest pair for batches of the same shape.
+   Result-invariant. */
 MG_EXPORT mg_status mg_batch_set_tuning(mg_batch *batch, uint32_t radix_bits);
+MG_EXPORT mg_status mg_batch_set_walk_threads(mg_batch *batch, uint32_t threads);
 
-/* The autotuner's last decision: candidates tried (radix bits) with their
-   frame times (ms; 0 = not tried) and the choice. */
+/* The autotuner's last decision: candidates tried with their frame times
+   (ms; 0 = not tried) and the choices. */
 typedef struct mg_batch_tuning {
     uint32_t n_candidates;
     uint32_t radix_bits[4];
     double device_ms[4];
     uint32_t chosen_radix_bits;
+    uint32_t n_walk_candidates;
+    uint32_t walk_threads[3];
+    uint32_t _pad;
+    double walk_ms[3];
+    uint32_t chosen_walk_threads;
+    uint32_t _pad2;
 } mg_batch_tuning;
 MG_EXPORT mg_status mg_batch_last_tuning(const mg_batch *batch, mg_batch_tuning *out);
 

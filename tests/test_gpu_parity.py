@@ -140,6 +140,23 @@ def test_tuning_is_result_invariant(gpu, cpu, radix):
     assert np.array_equal(s_cpu, g2.pixel_segments())
 
 
+@pytest.mark.parametrize("walk", [64, 256, 1024])
+@pytest.mark.parametrize("n_layers", [60, 700])
+def test_walk_threads_are_result_invariant(gpu, cpu, walk, n_layers):
+    # 60 layers: the small-LDS walk; 700: more than 512 orders (the automatic
+    # choice would be 1024 threads with the whole LDS)
+    w, h = 150, 70
+    comp = scenegen.build_scene(cpu, 610 + n_layers, w, h, n_layers, clips=True)
+    ref = np.zeros((w * h, 4), np.uint8)
+    comp.render(cpu.Buffer(ref, w), [1.0, 1.0, 1.0, 1.0], None)
+    g = scenegen.build_scene(gpu, 610 + n_layers, w, h, n_layers, clips=True)
+    g.set_walk_threads(walk)
+    out = np.zeros((w * h, 4), np.uint8)
+    g.render(gpu.Buffer(out, w), [1.0, 1.0, 1.0, 1.0], None)
+    assert g.stats().walk_threads == walk
+    assert_same_image(ref, out, w)
+
+
 def test_full_size_circles_frame(gpu, cpu):
     """The CPU-runnable configuration at full size: 1024x1024, 10,000 circles."""
     from cassia_amd import workloads
