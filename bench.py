@@ -306,6 +306,50 @@ def bench_animation(cassia_amd, workloads, frames=240, sample_every=10):
             "sampled_frames_identical": same, "sampled_frames": len(samples)}
 
 
+def bench_flatten(cassia_amd, n_paths=100_000, seed=12):
+    """Curve ingestion: glyph-like closed loops of 2-6 quadratic Beziers over
+    a 512-px box (as the batched scenes' paths, before flattening), inserted
+    into one composition with the flattening on the GPU
+    (mg_insert_paths_device) and on the host (mg_insert_paths, C++); the
+    two line sets are compared bit for bit."""
+    rng = np.random.default_rng(seed)
+    P = cassia_amd.Point
+    paths = []
+    for _ in range(n_paths):
+        p = cassia_amd.Path()
+        pts = rng.uniform(0, 512, (int(rng.integers(2, 7)), 2, 2)).astype(np.float32)
+        start = P(*pts[0, 0])
+        cur = start
+        for k in range(len(pts)):
+            nxt = P(*pts[k + 1, 0]) if k + 1 < len(pts) else start
+            p.quad(cur, P(*pts[k, 1]), nxt)
+            cur = nxt
+        p.close()
+        paths.append(p)
+    ids = (np.arange(n_paths) % 4096).astype(np.uint16)
+    out = {}
+    lines = {}
+    for mode in ("device", "host"):
+        best = None
+        for rep in range(3):
+            comp = cassia_amd.Composition()
+            comp.create_layers(4096)
+            t = time.perf_counter()
+            comp.insert_paths_device(ids, paths, device=(mode == "device"))
+            if mode == "host":
+                comp.render_device(8, 8, [1.0, 1.0, 1.0, 1.0])  # the lines reach the device too
+            dt = time.perf_counter() - t
+            best = dt if best is None else min(best, dt)
+        n = comp.builder_len()
+        out[mode] = {"ms": best * 1e3, "lines": int(n), "lines_per_s": n / best}
+        lines[mode] = comp.lines()
+    same = bool(np.array_equal(lines["device"][0].view(np.uint32), lines["host"][0].view(np.uint32)) and
+                np.array_equal(lines["device"][1], lines["host"][1]))
+    progress(f"flatten: device {out['device']['ms']:.2f} ms, host {out['host']['ms']:.2f} ms")
+    return {"workload": f"{n_paths} closed loops of 2-6 quadratic Beziers in a 512-px box, 4096 layers",
+            **out, "speedup": out["host"]["ms"] / out["device"]["ms"], "identical": same}
+
+
 def measured_traffic(workload):
     """HBM bytes per launch per kernel from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
@@ -348,7 +392,9 @@ def main():
                     help="measure only the batched many-scenes workload (prints its object alone)")
     ap.add_argument("--no-serve", action="store_true", help="skip the serving measurement")
     ap.add_argument("--no-anim", action="store_true", help="skip the animation measurement")
+    ap.add_argument("--no-flatten", action="store_true", help="skip the curve flattening measurement")
     ap.add_argument("--only-anim", action="store_true", help="measure only the animation workload")
+    ap.add_argument("--only-flatten", action="store_true", help="measure only the curve flattening")
     ap.add_argument("--only-serve", action="store_true", help="measure only the serving workload")
     ap.add_argument("--no-plain", action="store_true",
                     help="with --no-frame: skip the plain per-scene frames and the identity check")
@@ -368,6 +414,9 @@ def main():
     from cassia_amd import workloads
     if cassia_amd.device_count() < 1:
         raise SystemExit("no gfx950 device")
+    if args.only_flatten:
+        print(json.dumps({"flatten": bench_flatten(cassia_amd)}))
+        return
     if args.only_anim:
         print(json.dumps({"animation": bench_animation(cassia_amd, workloads)}))
         return
@@ -476,6 +525,8 @@ def main():
         out["batched"] = bench_batched(cassia_amd, workloads, args.steps, args.warmup, args.batch_scenes)
     if not args.no_anim:
         out["animation"] = bench_animation(cassia_amd, workloads)
+    if not args.no_flatten:
+        out["flatten"] = bench_flatten(cassia_amd)
     if not args.no_serve:
         out["serving"] = bench_serving(cassia_amd, workloads)
     if not args.no_cpu:

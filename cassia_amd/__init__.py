@@ -164,6 +164,11 @@ _sig("mg_layer_set_props", C.c_int, _vp, C.c_uint16, C.POINTER(_Props))
 _sig("mg_create_layers", C.c_int, _vp, C.c_size_t, C.POINTER(C.c_uint16))
 _sig("mg_layers_set_props", C.c_int, _vp, C.POINTER(C.c_uint16), _vp, C.c_size_t)
 _sig("mg_layers_set_transform", C.c_int, _vp, C.POINTER(C.c_uint16), _f, C.c_size_t)
+_sig("mg_insert_paths_device", C.c_int, _vp, C.POINTER(_vp), C.POINTER(C.c_uint16), _f, C.POINTER(C.c_uint8),
+     C.c_size_t)
+_sig("mg_insert_paths", C.c_int, _vp, C.POINTER(_vp), C.POINTER(C.c_uint16), _f, C.POINTER(C.c_uint8),
+     C.c_size_t)
+_sig("mg_lines", C.c_int, _vp, _f, C.POINTER(C.c_uint16), C.c_size_t, C.POINTER(C.c_size_t))
 _sig("mg_layers_set_order", C.c_int, _vp, C.POINTER(C.c_uint16), C.POINTER(C.c_uint16), C.c_size_t)
 _sig("mg_render_animated", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(_vp),
      C.POINTER(C.c_int))
@@ -570,6 +575,38 @@ class Composition:
         ids = np.ascontiguousarray(np.asarray(layer_ids, dtype=np.uint16).reshape(-1))
         assert len(ids) == len(a)
         _check(_lib.mg_insert_lines_multi(self._h, ids.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(a), len(a)))
+
+    def insert_paths_device(self, layer_ids, paths, transforms=None, device=True):
+        """insert_in_layer[_transformed] of many paths, flattened on the GPU
+        (mg_insert_paths_device; device=False: mg_insert_paths, on the host);
+        transforms[i] is a 9-float transform or None."""
+        n = len(paths)
+        if n != len(layer_ids):
+            raise ValueError("one layer id per path")
+        if n == 0:
+            return
+        hs = (_vp * n)(*[p._h for p in paths])
+        ids = np.ascontiguousarray(np.asarray(layer_ids, dtype=np.uint16).reshape(-1))
+        t, flags = None, None
+        if transforms is not None:
+            t = np.zeros((n, 9), np.float32)
+            flags = np.zeros(n, np.uint8)
+            for i, tr in enumerate(transforms):
+                if tr is not None:
+                    t[i] = np.asarray(tr, np.float32).reshape(9)
+                    flags[i] = 1
+        fn = _lib.mg_insert_paths_device if device else _lib.mg_insert_paths
+        _check(fn(self._h, hs, ids.ctypes.data_as(C.POINTER(C.c_uint16)), _fp(t) if t is not None else None,
+                  flags.ctypes.data_as(C.POINTER(C.c_uint8)) if flags is not None else None, n))
+
+    def lines(self):
+        """The composition's lines: ((n, 4) float32 x0, y0, x1, y1; (n,) uint16 layer ids)."""
+        n = C.c_size_t(0)
+        _check(_lib.mg_lines(self._h, None, None, 0, C.byref(n)))
+        pts = np.zeros((max(n.value, 1), 4), np.float32)
+        ids = np.zeros(max(n.value, 1), np.uint16)
+        _check(_lib.mg_lines(self._h, _fp(pts), ids.ctypes.data_as(C.POINTER(C.c_uint16)), n.value, C.byref(n)))
+        return pts[:n.value], ids[:n.value]
 
     def create_layers(self, n):
         """n create_layer calls in one: returns the ids (uint16 array)."""

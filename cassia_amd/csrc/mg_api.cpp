@@ -118,6 +118,9 @@ struct mg_composition {
     std::vector<float> pts;      // 4 floats per line (LinesBuilder SoA, interleaved)
     std::vector<uint16_t> ids;
     size_t uploaded = 0;         // lines already resident on the device
+    // lines appended on the device (flattened there) after the `uploaded`
+    // host lines, not yet copied into pts / ids (see ensure_mirror)
+    size_t dev_extra = 0;
     std::map<uint16_t, Layer> layers;
     std::priority_queue<uint32_t, std::vector<uint32_t>, std::greater<uint32_t>> free_ids;
     uint32_t next_id = 0;
@@ -141,6 +144,26 @@ struct mg_composition {
     int32_t spec_cache_id = -2;
     bool spec_clips = false;
 
+    size_t n_lines() const { return ids.size() + dev_extra; }
+
+    // Copies device-appended lines into the host arrays, for host work on
+    // them (inserting host lines after them, compaction, estimates).
+    mg_status ensure_mirror() {
+        if (!dev_extra) return MG_OK;
+        const size_t n0 = ids.size();
+        pts.resize(4 * (n0 + dev_extra));
+        ids.resize(n0 + dev_extra);
+        if (hipMemcpy(&pts[4 * n0], dlines.points() + n0, dev_extra * 16, hipMemcpyDeviceToHost) != hipSuccess ||
+            hipMemcpy(&ids[n0], dlines.layer_ids() + n0, dev_extra * 2, hipMemcpyDeviceToHost) != hipSuccess) {
+            pts.resize(4 * n0);
+            ids.resize(n0);
+            return fail(MG_ERR_DEVICE, "download of device lines failed");
+        }
+        uploaded = ids.size();
+        dev_extra = 0;
+        return MG_OK;
+    }
+
     size_t actual_len() const {
         size_t n = 0;
         for (auto &kv : layers)
@@ -150,7 +173,8 @@ struct mg_composition {
 
     // Composition::remove_disabled (composition.rs:123-142)
     void remove_disabled() {
-        if (ids.size() >= actual_len() * 2) {
+        if (n_lines() >= actual_len() * 2 && n_lines() > 0) {
+            if (ensure_mirror()) return;  // compaction stays for a later call
             size_t w = 0;
             for (size_t i = 0; i < ids.size(); ++i) {
                 auto it = layers.find(ids[i]);
@@ -216,6 +240,7 @@ struct mg_composition {
     }
 
     void insert(uint16_t id, const float *lines, size_t n) {
+        ensure_mirror();  // host lines go after the device-appended ones
         ++version;
         pts.insert(pts.end(), lines, lines + 4 * n);
         ids.insert(ids.end(), n, id);
@@ -360,6 +385,7 @@ MG_EXPORT mg_status mg_insert_lines(mg_composition *c, mg_layer_id id, const flo
 
 MG_EXPORT mg_status mg_insert_lines_multi(mg_composition *c, const mg_layer_id *ids, const float *lines, size_t n) {
     if (check_comp(c) || (n && (!lines || !ids))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    if (mg_status ms = c->ensure_mirror()) return ms;
     ++c->version;
     c->pts.insert(c->pts.end(), lines, lines + 4 * n);
     c->ids.insert(c->ids.end(), ids, ids + n);
@@ -387,6 +413,78 @@ MG_EXPORT mg_status mg_insert_in_layer_transformed(mg_composition *c, mg_layer_i
     std::vector<float> segs;
     p->path.flatten(t9, segs);
     c->insert(id, segs.data(), segs.size() / 4);
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_insert_paths_device(mg_composition *c, const mg_path *const *paths, const mg_layer_id *ids,
+                                           const float *t9s, const uint8_t *transformed, size_t n) {
+    if (check_comp(c) || (n && (!paths || !ids))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    for (size_t i = 0; i < n; ++i)
+        if (!paths[i]) return fail(MG_ERR_INVALID_ARG, "path is null");
+    if (!n) return MG_OK;
+    mg_status s = c->ensure_renderer();
+    if (s) return s;
+    std::vector<mg::PathCommand> cmds;
+    std::vector<mg::flat::PathDesc> descs(n);
+    for (size_t i = 0; i < n; ++i) {
+        const auto &pc = paths[i]->path.commands();
+        mg::flat::PathDesc &d = descs[i];
+        d = mg::flat::PathDesc{};
+        d.first_cmd = (uint32_t)cmds.size();
+        d.n_cmds = (uint32_t)pc.size();
+        d.layer = ids[i];
+        d.has_xf = t9s && (!transformed || transformed[i]) ? 1u : 0u;
+        if (d.has_xf) std::memcpy(d.t9, t9s + 9 * i, sizeof d.t9);
+        cmds.insert(cmds.end(), pc.begin(), pc.end());
+    }
+    if (cmds.size() > 0xFFFFFFFFull) return fail(MG_ERR_CAPACITY, "too many path commands");
+    std::lock_guard<std::mutex> g(c->rdr->mu);
+    // device lines follow every host line: those go up first
+    if ((s = c->sync_lines())) return s;
+    const size_t base = c->n_lines();
+    std::vector<uint32_t> counts;
+    std::string err;
+    if (!c->rdr->flatten_paths(cmds.data(), cmds.size(), descs.data(), n, c->dlines, base, counts, err))
+        return fail(err.find("memory") != std::string::npos ? MG_ERR_OOM : MG_ERR_DEVICE, err);
+    size_t added = 0;
+    for (size_t i = 0; i < n; ++i) {
+        // insert_segments (composition.rs:72-78), as for a host insert
+        Layer &l = c->layers[ids[i]];
+        l.has_order = true;
+        l.order = ids[i];
+        l.enabled = true;
+        l.len += counts[i];
+        added += counts[i];
+    }
+    c->dev_extra += added;
+    ++c->version;
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_insert_paths(mg_composition *c, const mg_path *const *paths, const mg_layer_id *ids,
+                                    const float *t9s, const uint8_t *transformed, size_t n) {
+    if (check_comp(c) || (n && (!paths || !ids))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    for (size_t i = 0; i < n; ++i)
+        if (!paths[i]) return fail(MG_ERR_INVALID_ARG, "path is null");
+    std::vector<float> segs;
+    for (size_t i = 0; i < n; ++i) {
+        segs.clear();
+        const bool xf = t9s && (!transformed || transformed[i]);
+        paths[i]->path.flatten(xf ? t9s + 9 * i : nullptr, segs);
+        c->insert(ids[i], segs.data(), segs.size() / 4);
+    }
+    return MG_OK;
+}
+
+MG_EXPORT mg_status mg_lines(mg_composition *c, float *pts, uint16_t *ids, size_t cap, size_t *n) {
+    if (check_comp(c) || !n || (cap && (!pts || !ids))) return fail(MG_ERR_INVALID_ARG, "null argument");
+    if (mg_status s = c->ensure_mirror()) return s;
+    *n = c->ids.size();
+    const size_t m = std::min(cap, c->ids.size());
+    if (m) {
+        std::memcpy(pts, c->pts.data(), m * 16);
+        std::memcpy(ids, c->ids.data(), m * 2);
+    }
     return MG_OK;
 }
 
@@ -531,7 +629,7 @@ MG_EXPORT mg_status mg_remove_disabled(mg_composition *c) {
 
 MG_EXPORT mg_status mg_lines_len(const mg_composition *c, size_t *builder_len, size_t *actual_len) {
     if (check_comp(c)) return MG_ERR_INVALID_ARG;
-    if (builder_len) *builder_len = c->ids.size();
+    if (builder_len) *builder_len = c->n_lines();
     if (actual_len) *actual_len = c->actual_len();
     return MG_OK;
 }
@@ -647,7 +745,7 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.tables_version = c->version * 64 + (uint64_t)(cache_id + 1);
     mg::SceneDesc sd = scene_desc(width, stride, rows_px, bg, crop);
     sd.line_base = 0;
-    sd.n_lines = (uint32_t)c->ids.size();
+    sd.n_lines = (uint32_t)c->n_lines();
     sd.layer_base = 0;
     sd.n_layer_entries = (uint32_t)f.layers.size();
     sd.props_base = 0;
@@ -668,9 +766,9 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     mg_status s;
     if ((s = c->sync_lines())) return s;
     const bool ok = graph_replayed
-                        ? c->rdr->run_graph(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), img, r,
+                        ? c->rdr->run_graph(f, c->dlines.points(), c->dlines.layer_ids(), c->n_lines(), img, r,
                                             *graph_replayed, err)
-                        : c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), img, prev, r, err);
+                        : c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->n_lines(), img, prev, r, err);
     if (!ok) return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
     if (cache_id >= 0) {
         bool changed = false;
@@ -682,7 +780,7 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
         }
         if (changed) ++c->version;
     }
-    fill_stats(c->stats, r, c->ids.size());
+    fill_stats(c->stats, r, c->n_lines());
     return MG_OK;
 }
 
@@ -827,7 +925,7 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
     bool has_clips = false;
     c->tables(-1, f, has_clips);
     mg::SceneDesc sd{};
-    sd.n_lines = (uint32_t)c->ids.size();
+    sd.n_lines = (uint32_t)c->n_lines();
     sd.n_layer_entries = (uint32_t)f.layers.size();
     sd.n_props = (uint32_t)f.props.size();
     f.scenes.push_back(sd);
@@ -837,7 +935,8 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
     if ((s = c->sync_lines())) return s;
     std::string err;
     mg::FrameResult r;
-    if (!c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->ids.size(), nullptr, nullptr, r, err)) return fail(MG_ERR_DEVICE, err);
+    if (!c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->n_lines(), nullptr, nullptr, r, err))
+        return fail(MG_ERR_DEVICE, err);
     c->pixsegs.assign(2 * r.line_pixels, mg::kNone);
     if (!c->rdr->download_sorted(c->pixsegs.data(), r.kept, err)) return fail(MG_ERR_DEVICE, err);
     *out = c->pixsegs.data();
@@ -938,7 +1037,7 @@ mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_f
     for (size_t i = 0; same_frames && i < n; ++i) same_frames = same_frame(b->packed_frames[i], frames[i]);
     if (same_frames) return MG_OK;
     size_t total = 0;
-    for (size_t i = 0; i < n; ++i) total += comps[i]->ids.size();
+    for (size_t i = 0; i < n; ++i) total += comps[i]->n_lines();
     if (total > 0xFFFFFFFFull) return fail(MG_ERR_CAPACITY, "a batch holds at most 2^32 lines");
     b->n_lines = total;
     b->owners.resize(n);
@@ -960,7 +1059,7 @@ mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_f
         const mg_batch_frame &fr = frames[i];
         mg::SceneDesc sd = scene_desc(fr.width, fr.width, fr.height, fr.background, fr.has_crop ? &fr.crop : nullptr);
         sd.line_base = (uint32_t)line_base;
-        sd.n_lines = (uint32_t)c->ids.size();
+        sd.n_lines = (uint32_t)c->n_lines();
         sd.layer_base = (uint32_t)f.layers.size();
         sd.n_layer_entries = (uint32_t)cs.layers.size();
         sd.props_base = (uint32_t)f.props.size();
@@ -983,7 +1082,7 @@ mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_f
         f.scene_line_base[i] = (uint32_t)line_base;
         b->offsets[i] = out;
         b->bytes[i] = (uint64_t)fr.width * fr.height * 4;
-        line_base += c->ids.size();
+        line_base += c->n_lines();
         row_base += sd.tile_rows;
         tile_base += (uint64_t)sd.tile_rows * sd.row_len;
         out += (b->bytes[i] + 255) & ~255ull;
@@ -1066,6 +1165,7 @@ uint64_t footprint(const mg_composition *c, const mg_batch_frame &fr, uint64_t *
     // copy), two 8-byte sort buffers of 2 slots per line-pixel, the frame and
     // per-tile paint state; line-pixels from each line's major extent
     double lp = 0.0;
+    if (const_cast<mg_composition *>(c)->ensure_mirror()) return ~0ull;
     const size_t L = c->ids.size();
     for (size_t i = 0; i < L; ++i) {
         const float *q = &c->pts[4 * i];

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "mg_flatten.h"
 #include "mg_shared.h"
 
 namespace mg {
@@ -69,6 +70,10 @@ __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const Sc
 __global__ void k_order_scatter(const LineSetup *setup, const uint32_t *lengths, uint64_t n_lines,
                                 const SceneDesc *scenes, uint32_t *base, LineSetup *setup_perm,
                                 uint32_t *lengths_perm);
+__global__ void k_path_counts(const PathCommand *cmds, const flat::PathDesc *paths, uint32_t n_paths,
+                              uint32_t *counts, uint32_t *err);
+__global__ void k_path_emit(const PathCommand *cmds, const flat::PathDesc *paths, uint32_t n_paths,
+                            const uint32_t *offsets, const uint32_t *counts, float4 *pts, uint16_t *ids);
 __global__ void k_part_lines(const uint32_t *offsets, const uint32_t *lengths, uint64_t n_lines,
                              uint32_t part_size, uint32_t *part_line, uint64_t n_parts);
 __global__ void k_clamp_line_pixels(FrameCounters *counters, uint64_t cap, uint32_t *flag);

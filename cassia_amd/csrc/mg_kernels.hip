@@ -281,6 +281,46 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
     if (bucket_hist) bucket_count(valid, bucket, bucket_hist);
 }
 
+// ------------------------------------------------------- path flattening
+
+// PathSegments on the device (mold/src/path/segments.rs:51-263): one thread
+// per path counts its lines, and after a scan of the counts writes them in
+// order, with the serial t += 1/n recurrence of each curve.
+__global__ __launch_bounds__(256) void k_path_counts(const PathCommand *__restrict__ cmds,
+                                                     const flat::PathDesc *__restrict__ paths, uint32_t n_paths,
+                                                     uint32_t *__restrict__ counts, uint32_t *__restrict__ err) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_paths) return;
+    const flat::PathDesc d = paths[i];
+    flat::Xform xf;
+    if (d.has_xf) xf.init(d.t9);
+    const uint64_t n = flat::count_commands(cmds + d.first_cmd, d.n_cmds, d.has_xf ? &xf : nullptr);
+    if (n > 0xFFFFFFFFull) atomicOr(err, 1u);
+    counts[i] = n > 0xFFFFFFFFull ? 0u : (uint32_t)n;
+}
+
+__global__ __launch_bounds__(256) void k_path_emit(const PathCommand *__restrict__ cmds,
+                                                   const flat::PathDesc *__restrict__ paths, uint32_t n_paths,
+                                                   const uint32_t *__restrict__ offsets,
+                                                   const uint32_t *__restrict__ counts, float4 *__restrict__ pts,
+                                                   uint16_t *__restrict__ ids) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_paths) return;
+    const flat::PathDesc d = paths[i];
+    flat::Xform xf;
+    if (d.has_xf) xf.init(d.t9);
+    const uint64_t base = offsets[i], cnt = counts[i];
+    uint64_t k = 0;
+    flat::flatten_commands(cmds + d.first_cmd, d.n_cmds, d.has_xf ? &xf : nullptr,
+                           [&](float x0, float y0, float x1, float y1) {
+                               if (k < cnt) {  // the count pass bounds the writes
+                                   pts[base + k] = make_float4(x0, y0, x1, y1);
+                                   ids[base + k] = (uint16_t)d.layer;
+                               }
+                               ++k;
+                           });
+}
+
 // ------------------------------------------------------------------- scan
 
 // Single-pass exclusive scan of u32 values (decoupled look-back), 2048

@@ -9,13 +9,9 @@
 
 #include <vector>
 
-namespace mg {
+#include "mg_flatten.h"
 
-struct PathCommand {
-    enum Kind : uint8_t { Line, Quad, RatQuad, Cubic, RatCubic, Close } kind;
-    // Points as (x, y); rational commands as premultiplied (x*w, y*w, w).
-    float v[12];
-};
+namespace mg {
 
 class Path {
 public:

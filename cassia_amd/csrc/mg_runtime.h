@@ -12,6 +12,7 @@
 #include <string>
 #include <vector>
 
+#include "mg_flatten.h"
 #include "mg_shared.h"
 
 namespace mg {
@@ -32,6 +33,8 @@ struct LineStore {
     DevBuf pts, ids;
     size_t cap = 0;
     bool upload(const float *pts4, const uint16_t *ids, size_t first, size_t n, hipStream_t s, std::string &err);
+    // Capacity for `need` lines, keeping the first `keep`.
+    bool grow(size_t need, size_t keep, hipStream_t s, std::string &err);
     void release();
     const float4 *points() const { return pts.as<float4>(); }
     const uint16_t *layer_ids() const { return ids.as<uint16_t>(); }
@@ -112,6 +115,11 @@ public:
     // and captures anew (`replayed` false).
     bool run_graph(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
                    FrameResult &res, bool &replayed, std::string &err);
+    // Path flattening on the device: the paths' lines appended to `ls` after
+    // its first `base` lines, path after path; `counts` receives each
+    // path's number of lines.
+    bool flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat::PathDesc *paths, size_t n_paths,
+                       LineStore &ls, size_t base, std::vector<uint32_t> &counts, std::string &err);
     bool download_sorted(uint64_t *dst, size_t n, std::string &err);
     bool download_tile_ops(int8_t *dst, size_t n, std::string &err);
     bool download_coverage(std::vector<int32_t> &a, std::vector<uint32_t> &key, uint32_t &count,
@@ -159,6 +167,7 @@ private:
     DevBuf overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
     DevBuf cnt0_, cnt_acc_, cnt_status_;
+    DevBuf flat_cmds_, flat_paths_, flat_counts_, flat_offsets_, flat_status_;
     DevBuf work_, runs_work_, walk_scratch_, tile_start_, tile_nr_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list

@@ -64,7 +64,8 @@ Renderer::~Renderer() {
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
                      &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_nr_,
-                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_work_, &walk_scratch_};
+                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_work_, &walk_scratch_,
+                     &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_};
     for (DevBuf *b : all) b->release();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -151,6 +152,29 @@ bool LineStore::upload(const float *pts4, const uint16_t *ids_h, size_t first, s
         MG_HIP(hipMemcpyAsync(ids.as<char>() + first * 2, ids_h, n * 2, hipMemcpyHostToDevice, s), "upload ids");
         MG_HIP(hipStreamSynchronize(s), "sync");
     }
+    return true;
+}
+
+bool LineStore::grow(size_t need, size_t keep, hipStream_t s, std::string &err) {
+    if (need <= cap) return true;
+    size_t ncap = std::max<size_t>(need, cap * 2);
+    DevBuf np, ni;
+    if (!np.reserve(ncap * 16) || !ni.reserve(ncap * 2)) {
+        np.release();
+        ni.release();
+        err = "out of device memory (lines)";
+        return false;
+    }
+    if (keep && pts.ptr) {
+        MG_HIP(hipMemcpyAsync(np.ptr, pts.ptr, keep * 16, hipMemcpyDeviceToDevice, s), "copy lines");
+        MG_HIP(hipMemcpyAsync(ni.ptr, ids.ptr, keep * 2, hipMemcpyDeviceToDevice, s), "copy ids");
+    }
+    MG_HIP(hipStreamSynchronize(s), "sync");
+    pts.release();
+    ids.release();
+    pts = np;
+    ids = ni;
+    cap = ncap;
     return true;
 }
 
@@ -892,6 +916,59 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
     graph_key_ = key;
     graph_bufs_ = frame_buffers();
     graph_res_ = res;
+    return true;
+}
+
+bool Renderer::flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat::PathDesc *paths, size_t n_paths,
+                             LineStore &ls, size_t base, std::vector<uint32_t> &counts, std::string &err) {
+    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+    counts.assign(n_paths, 0);
+    if (!n_paths) return true;
+    if (n_paths > 0xFFFFFFFFull) {
+        err = "too many paths";
+        return false;
+    }
+    const uint64_t nparts = (n_paths + kScanTileHost - 1) / kScanTileHost;
+    if (!flat_cmds_.reserve(std::max<size_t>(n_cmds, 1) * sizeof(PathCommand)) ||
+        !flat_paths_.reserve(n_paths * sizeof(flat::PathDesc)) || !flat_counts_.reserve(n_paths * 4) ||
+        !flat_offsets_.reserve(n_paths * 4) || !flat_status_.reserve(nparts * 8)) {
+        err = "out of device memory (paths)";
+        return false;
+    }
+    if (n_cmds)
+        MG_HIP(hipMemcpyAsync(flat_cmds_.ptr, cmds, n_cmds * sizeof(PathCommand), hipMemcpyHostToDevice, stream_),
+               "upload commands");
+    MG_HIP(hipMemcpyAsync(flat_paths_.ptr, paths, n_paths * sizeof(flat::PathDesc), hipMemcpyHostToDevice, stream_),
+           "upload paths");
+    FrameCounters *ctr = counters_.as<FrameCounters>();
+    fill64((uint64_t *)ctr, 0, sizeof(FrameCounters) / 8, stream_);
+    fill32(err_.as<uint32_t>(), 0, 3, stream_);
+    fill64(flat_status_.as<uint64_t>(), 0, nparts, stream_);
+    const uint32_t grid = (uint32_t)((n_paths + 255) / 256);
+    k_path_counts<<<grid, 256, 0, stream_>>>(flat_cmds_.as<PathCommand>(), flat_paths_.as<flat::PathDesc>(),
+                                             (uint32_t)n_paths, flat_counts_.as<uint32_t>(), err_.as<uint32_t>());
+    k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(flat_counts_.as<uint32_t>(), flat_offsets_.as<uint32_t>(),
+                                                    n_paths, flat_status_.as<uint64_t>(), err_.as<uint32_t>() + 1,
+                                                    &ctr->line_pixels);
+    MG_HIP(hipGetLastError(), "flatten count launch");
+    uint64_t total = 0;
+    uint32_t herr[2] = {0, 0};
+    MG_HIP(hipMemcpyAsync(&total, &ctr->line_pixels, 8, hipMemcpyDeviceToHost, stream_), "read line count");
+    MG_HIP(hipMemcpyAsync(herr, err_.ptr, 8, hipMemcpyDeviceToHost, stream_), "read flags");
+    MG_HIP(hipStreamSynchronize(stream_), "flatten count");
+    if (herr[0] || herr[1] || base + total > 0xFFFFFFFFull) {
+        err = herr[1] ? "a look-back wait timed out" : "too many lines";
+        return false;
+    }
+    if (!ls.grow(base + total, base, stream_, err)) return false;
+    k_path_emit<<<grid, 256, 0, stream_>>>(flat_cmds_.as<PathCommand>(), flat_paths_.as<flat::PathDesc>(),
+                                           (uint32_t)n_paths, flat_offsets_.as<uint32_t>(),
+                                           flat_counts_.as<uint32_t>(), ls.pts.as<float4>() + base,
+                                           ls.ids.as<uint16_t>() + base);
+    MG_HIP(hipGetLastError(), "flatten launch");
+    MG_HIP(hipMemcpyAsync(counts.data(), flat_counts_.ptr, n_paths * 4, hipMemcpyDeviceToHost, stream_),
+           "read counts");
+    MG_HIP(hipStreamSynchronize(stream_), "flatten");
     return true;
 }
 

@@ -140,6 +140,25 @@ MG_EXPORT mg_status mg_insert_lines(mg_composition *comp, mg_layer_id id, const 
 MG_EXPORT mg_status mg_insert_lines_multi(mg_composition *comp, const mg_layer_id *ids,
                                           const float *lines, size_t n);
 
+/* Bulk Composition::insert_in_layer / insert_in_layer_transformed
+   (composition.rs:58-95) with the flattening (PathSegments,
+   mold/src/path/segments.rs:51-263, transform.rs:57-93) on the GPU: path i
+   goes to layer ids[i], transformed by t9s[9 i .. 9 i + 9] when t9s is not
+   NULL and transformed (NULL = all) has transformed[i] != 0. The lines are
+   written straight into the composition's resident line store, path after
+   path, in the order the host path form gives them; the host keeps no copy
+   until one is needed (mg_lines, host inserts, compaction). */
+MG_EXPORT mg_status mg_insert_paths_device(mg_composition *comp, const mg_path *const *paths,
+                                           const mg_layer_id *ids, const float *t9s, const uint8_t *transformed,
+                                           size_t n);
+/* The same with the flattening on the host (one insert_in_layer per path). */
+MG_EXPORT mg_status mg_insert_paths(mg_composition *comp, const mg_path *const *paths, const mg_layer_id *ids,
+                                    const float *t9s, const uint8_t *transformed, size_t n);
+/* The composition's lines (LinesBuilder contents, segment.rs:31-53):
+   n x {x0, y0, x1, y1} and their layer ids, at most `cap`; *n receives the
+   total. */
+MG_EXPORT mg_status mg_lines(mg_composition *comp, float *pts, uint16_t *ids, size_t cap, size_t *n);
+
 /* Layer accessors (mold/src/layer.rs:152-231). MG_ERR_NO_LAYER when the id
    has no layer entry (Composition::get_mut -> None). */
 MG_EXPORT mg_status mg_layer_exists(const mg_composition *comp, mg_layer_id id, int *out);
