@@ -982,8 +982,9 @@ struct mg_batch {
     mg::FrameSpec spec;
     size_t n_lines = 0;
     std::vector<uint64_t> offsets, bytes;
-    uint32_t radix_bits = 0;    // 0 = autotune
-    uint32_t walk_threads = 0;  // 0 = autotune
+    uint32_t radix_bits = 0;    // 0 = autotune (or the runtime's rule without the autotuner)
+    uint32_t walk_threads = 0;  // 0 = autotune (or the runtime's rule without the autotuner)
+    bool autotune = true;       // false for serving batches, whose shapes change every batch
     // autotuner: the choice holds for batches of the same shape
     uint32_t tuned_bits = 0, tuned_walk = 0, tuned_for = ~0u;
     mg_batch_tuning tuning{};
@@ -1155,6 +1156,7 @@ const char *last_error() { return g_error.c_str(); }
 mg_batch *batch_new(std::shared_ptr<mg::Renderer> engine) {
     mg_batch *b = new mg_batch();
     b->rdr = std::move(engine);
+    b->autotune = false;
     return b;
 }
 
@@ -1192,7 +1194,7 @@ mg_status batch_render(mg_batch *b, mg_composition *const *comps, const mg_batch
         return MG_OK;
     };
     uint32_t bits = b->radix_bits, walk = b->walk_threads;
-    if (!bits || !walk) {
+    if (b->autotune && (!bits || !walk)) {
         // the autotuner: each free knob in turn (radix with the fixed or
         // automatic walk size, then walk sizes with the chosen radix), each
         // candidate timed twice on this batch (the first run also sizes

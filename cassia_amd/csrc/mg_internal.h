@@ -13,7 +13,9 @@ namespace mgi {
 mg_status set_error(mg_status s, const std::string &msg);
 const char *last_error();
 
-// A batch bound to `engine` (its own stream and scratch), for one serving worker.
+// A batch bound to `engine` (its own stream and scratch), for one serving
+// worker: no autotuner (serving batches change shape every batch), the
+// runtime's automatic radix and walk size.
 mg_batch *batch_new(std::shared_ptr<mg::Renderer> engine);
 
 // Frame tables and resident lines of a composition for batched frames (done
