@@ -32,7 +32,8 @@ struct PaintParams {
     int8_t *tile_ops;
     // tile columns: first word per (row, colp) (stride tstride), run counts
     uint32_t *tile_start;
-    uint32_t *tile_nr;
+    uint32_t *tile_run0;         // index of each (row, colp)'s first run in the run arrays
+    uint32_t *tile_nr;           // runs of each (row, colp)
     uint32_t tstride;
     // runs, indexed by their column's first word + run index:
     // layer | even_odd << 16, first word, 8 row cover sums (i8 each)
@@ -88,8 +89,12 @@ template <int R>
 __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
                             uint32_t shift, const uint32_t *digit_base, uint32_t *status,
                             uint32_t *stall, const uint32_t *first_off);
-__global__ void k_tile_starts(PaintParams pp, uint32_t global_rows);
+__global__ void k_tile_starts(PaintParams pp, uint32_t global_rows, uint32_t stream_runs);
 __global__ void k_tile_runs(PaintParams pp, const uint2 *work, uint32_t n_work);
+__global__ void k_tile_nr(PaintParams pp, uint32_t global_rows);
+__global__ void k_runs_scan(PaintParams pp, const FrameCounters *counters, uint64_t *status, uint32_t *stall,
+                            uint32_t *open);
+__global__ void k_run_cov(PaintParams pp, const FrameCounters *counters, const uint32_t *open);
 // T = 64, 256 or 1024 threads per tile row; G: queue in global scratch
 template <uint32_t T, bool G>
 __global__ void k_row_walk(PaintParams pp);
@@ -100,5 +105,6 @@ size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 constexpr uint32_t kEmitTileHost = 1024;
 constexpr uint32_t kSortTileHost = 2048;
 constexpr uint32_t kScanTileHost = 2048;
+constexpr uint32_t kRunsTileHost = 4096;  // words per k_runs_scan partition
 
 }  // namespace mg

@@ -998,7 +998,7 @@ constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flag
 constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
 
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride) {
-    return (size_t)cap * 24 + (size_t)rcap * 16 + (size_t)tstride * 8 + 64;
+    return (size_t)cap * 24 + (size_t)rcap * 16 + (size_t)tstride * 12 + 64;
 }
 size_t paint_lds_bytes_host(uint32_t cap, bool clips) {
     return (size_t)kStageWin * 8 + 2 * 64 * 4 + (clips ? (size_t)cap * 4 : 0);
@@ -1105,7 +1105,7 @@ __device__ __forceinline__ uint32_t scene_h1(const SceneDesc &S) {
     return S.crop_h1 < S.row_len ? S.crop_h1 : S.row_len;
 }
 
-__global__ __launch_bounds__(256) void k_tile_starts(PaintParams pp, uint32_t global_rows) {
+__global__ __launch_bounds__(256) void k_tile_starts(PaintParams pp, uint32_t global_rows, uint32_t stream_runs) {
     const uint32_t i = blockIdx.x * 256 + threadIdx.x;
     if (i >= global_rows * pp.tstride) return;
     const uint32_t grow = i / pp.tstride, k = i - grow * pp.tstride;
@@ -1116,8 +1116,195 @@ __global__ __launch_bounds__(256) void k_tile_starts(PaintParams pp, uint32_t gl
         if (seg_colp(pp.segs[mid], lb, cmask) < k) lo = mid + 1; else hi = mid;
     }
     pp.tile_start[i] = lo;
+    if (!stream_runs) {
+        pp.tile_run0[i] = lo;  // k_tile_runs stores a tile's runs from its first word on
+        return;
+    }
+    // the tile's first run: the first run starting at or after its first word
+    uint32_t rlo = 0, rhi = *pp.n_runs;
+    while (rlo < rhi) {
+        const uint32_t mid = (rlo + rhi) >> 1;
+        if (pp.run_beg[mid] < lo) rlo = mid + 1; else rhi = mid;
+    }
+    pp.tile_run0[i] = rlo;
 }
 
+// Runs per tile of the stream form: the next column's first run minus this one's.
+__global__ __launch_bounds__(256) void k_tile_nr(PaintParams pp, uint32_t global_rows) {
+    const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= global_rows * pp.tstride) return;
+    const uint32_t k = i % pp.tstride;
+    pp.tile_nr[i] = k + 1 < pp.tstride ? pp.tile_run0[i + 1] - pp.tile_run0[i] : 0u;
+}
+
+// Runs of the sorted render words: maximal ranges of equal (row, column,
+// layer) key, i.e. one layer's segments in one tile. Word i starts a run when
+// its key differs from word i - 1's. Run heads are ranked over the whole
+// stream (block scan of ballots, decoupled look-back over partitions of
+// 4,096 words), so the work follows the words, not the tiles: a tile with a
+// million words spreads over many work groups. Run g: run_key[g] = layer |
+// even-odd flag, run_beg[g] = its first word; run_beg[R] = the word count;
+// run_cov[g] = its 8 row cover sums (wrapping i8, packed in 64 bits), from
+// the block's covers staged in LDS; a run that continues past the block's end
+// is left to k_run_cov (open[part] = g + 1, else 0).
+constexpr uint32_t kRunsIPT = 16;
+static_assert(256 * kRunsIPT == kRunsTileHost, "k_runs_scan partition size");
+__global__ __launch_bounds__(256) void k_runs_scan(PaintParams pp, const FrameCounters *__restrict__ counters,
+                                                   uint64_t *__restrict__ status, uint32_t *__restrict__ stall,
+                                                   uint32_t *__restrict__ open) {
+    __shared__ uint32_t s_cnt[kRunsIPT * 4];
+    __shared__ uint64_t s_excl;
+    __shared__ uint64_t s_heads[kRunsIPT * 4];     // head bits in word order
+    __shared__ uint16_t s_cov[256 * kRunsIPT];     // cover byte | tile_y << 8 per word
+    const uint64_t n = counters->kept;
+    const uint32_t part = blockIdx.x;
+    const uint64_t base = (uint64_t)part * 256 * kRunsIPT;
+    if (base >= n) return;  // partitions past the stream (grids sized for a capacity)
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t lt = (1ull << lane) - 1ull;
+    uint64_t key[kRunsIPT], hm[kRunsIPT];
+#pragma unroll
+    for (uint32_t k = 0; k < kRunsIPT; ++k) {
+        const uint64_t i = base + k * 256 + threadIdx.x;
+        const uint64_t wd = i < n ? pp.segs[i] : ~0ull;
+        key[k] = wd >> kPayloadBits;
+        s_cov[k * 256 + threadIdx.x] = (uint16_t)(((uint32_t)wd & 0x3Fu) | (((uint32_t)(wd >> 19) & 7u) << 8));
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kRunsIPT; ++k) {
+        const uint64_t i = base + k * 256 + threadIdx.x;
+        uint64_t prev = __shfl_up(key[k], 1);
+        if (lane == 0) prev = i > 0 && i - 1 < n ? pp.segs[i - 1] >> kPayloadBits : ~0ull;
+        const bool head = i < n && (i == 0 || key[k] != prev);
+        hm[k] = __ballot(head);
+        if (lane == 0) {
+            s_cnt[k * 4 + w] = (uint32_t)__popcll(hm[k]);
+            s_heads[k * 4 + w] = hm[k];
+        }
+    }
+    __syncthreads();
+    // heads before each (step, wave) group, in word order (step-major)
+    if (threadIdx.x < 64) {
+        const uint32_t c = s_cnt[lane];
+        uint32_t x = c;
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t y = __shfl_up(x, off);
+            if (lane >= (uint32_t)off) x += y;
+        }
+        const uint32_t agg = __shfl(x, 63);
+        s_cnt[lane] = x - c;
+        if (lane == 0) st_release(status + part, (part == 0 ? kStInc : kStAgg) | agg);
+        const uint64_t e = part ? lookback_wave(status, part, stall) : 0;
+        if (lane == 0) {
+            s_excl = e;
+            if (part) st_release(status + part, kStInc | (e + agg));
+            if (base + 256 * kRunsIPT >= n) {
+                *pp.n_runs = (uint32_t)(e + agg);
+                pp.run_beg[e + agg] = (uint32_t)n;
+            }
+        }
+    }
+    if (threadIdx.x == 0) open[part] = 0;
+    __syncthreads();
+    const uint64_t excl = s_excl;
+    const uint32_t lmask = (1u << pp.layer_bits) - 1u, row_shift = pp.col_bits + pp.layer_bits;
+    const uint32_t in_block = n - base < 256 * kRunsIPT ? (uint32_t)(n - base) : 256 * kRunsIPT;
+#pragma unroll
+    for (uint32_t k = 0; k < kRunsIPT; ++k) {
+        if (!((hm[k] >> lane) & 1ull)) continue;
+        const uint64_t g = excl + s_cnt[k * 4 + w] + (uint64_t)__popcll(hm[k] & lt);
+        const uint32_t layer = (uint32_t)key[k] & lmask;
+        const SceneDesc &S = pp.scenes[pp.row_scene[(uint32_t)(key[k] >> row_shift)]];
+        const bool eo = pp.props[S.props_base + layer].flags & PF_EVENODD;
+        pp.run_key[g] = layer | (eo ? kRunEvenOdd : 0u);
+        pp.run_beg[g] = (uint32_t)(base + k * 256 + threadIdx.x);
+    }
+    // Cover sums: thread t takes the block's words [16 t, 16 t + 16) from
+    // LDS. A segmented scan (reset at run heads, byte-wise wrapping adds)
+    // over the threads gives every word the sum of its run so far; the last
+    // word of each run in the block writes that run's total.
+    const uint32_t c0 = threadIdx.x * kRunsIPT;
+    const uint32_t hb = (uint32_t)((s_heads[c0 >> 6] >> (c0 & 63)) & 0xFFFFull);
+    uint64_t v[kRunsIPT];
+    uint64_t tail = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < kRunsIPT; ++j) {
+        const uint32_t c = s_cov[c0 + j];
+        const uint64_t byte = (uint64_t)((uint32_t)as_i8((int32_t)((int64_t)((uint64_t)c << 58) >> 58)) & 0xFFu);
+        v[j] = c0 + j < in_block ? byte << (8 * (c >> 8)) : 0ull;
+        tail = ((hb >> j) & 1u) ? v[j] : add_bytes(tail, v[j]);
+    }
+    // exclusive segmented scan of (has head, tail) over the threads
+    uint32_t f = hb != 0u;
+    uint64_t x = tail;
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+        const uint64_t xp = __shfl_up(x, off);
+        const uint32_t fp = __shfl_up(f, off);
+        if (lane >= off) {
+            if (!f) x = add_bytes(xp, x);
+            f |= fp;
+        }
+    }
+    __shared__ uint64_t s_wx[4];
+    __shared__ uint32_t s_wf[4];
+    if (lane == 63) {
+        s_wx[w] = x;
+        s_wf[w] = f;
+    }
+    uint64_t ex = __shfl_up(x, 1);
+    uint32_t ef = __shfl_up(f, 1);
+    if (lane == 0) {
+        ex = 0;
+        ef = 0;
+    }
+    __syncthreads();
+    uint64_t carry = 0;  // from the waves before this one
+    for (uint32_t q = 0; q < w; ++q) carry = s_wf[q] ? s_wx[q] : add_bytes(carry, s_wx[q]);
+    uint64_t run = ef ? ex : add_bytes(carry, ex);
+    const uint64_t *heads = s_heads;
+#pragma unroll
+    for (uint32_t j = 0; j < kRunsIPT; ++j) {
+        const uint32_t li = c0 + j;
+        run = ((hb >> j) & 1u) ? v[j] : add_bytes(run, v[j]);
+        if (li >= in_block) break;
+        const uint32_t nx = li + 1;
+        const bool last = nx == in_block || ((heads[nx >> 6] >> (nx & 63)) & 1ull);
+        if (!last) continue;
+        // the run of word li: heads at or before li in this block
+        const uint32_t q = li >> 6;
+        const uint32_t le = s_cnt[q] + (uint32_t)__popcll(heads[q] & (~0ull >> (63 - (li & 63))));
+        if (le == 0) continue;  // a run from an earlier block: its block's open run
+        const uint64_t g = excl + le - 1;
+        pp.run_cov[g] = run;
+        if (nx == in_block && base + 256 * kRunsIPT < n) open[part] = (uint32_t)g + 1u;
+    }
+}
+
+// Cover sums of the runs that continue past their block's end (one per
+// block at most): one thread per block, over the run's words.
+__global__ __launch_bounds__(256) void k_run_cov(PaintParams pp, const FrameCounters *__restrict__ counters,
+                                                 const uint32_t *__restrict__ open) {
+    const uint64_t n_parts = (counters->kept + kRunsTileHost - 1) / kRunsTileHost;
+    for (uint64_t part = blockIdx.x * 256 + threadIdx.x; part < n_parts; part += (uint64_t)gridDim.x * 256) {
+        const uint32_t o = open[part];
+        if (!o) continue;
+        const uint32_t g = o - 1u;
+        const uint32_t b = pp.run_beg[g], e = pp.run_beg[g + 1];
+        uint64_t s = 0;
+        for (uint32_t i = b; i < e; ++i) {
+            const uint64_t w = pp.segs[i];
+            const int32_t cover = (int32_t)((int64_t)(w << 58) >> 58);
+            const uint32_t ty = (uint32_t)(w >> 19) & 7u;
+            s = add_bytes(s, (uint64_t)((uint32_t)cover & 0xFFu) << (8 * ty));
+        }
+        pp.run_cov[g] = s;
+    }
+}
+
+// Per-tile form of the runs, for frames with many tiles of moderate depth:
+// one wave per tile column, the tile's runs stored from its first word's
+// index on (tile_run0 = tile_start).
 // Runs of tile column colp (col + 1; 0 gathers every col < 0) of row grow,
 // by one wave. Returns the number of runs.
 __device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_t grow, uint32_t colp) {
@@ -1235,7 +1422,8 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     uint32_t *rk = (uint32_t *)p; p += (size_t)rcap * 4;
     uint32_t *rb = (uint32_t *)p; p += (size_t)rcap * 4;
     uint32_t *ts_l = (uint32_t *)p; p += (size_t)tstride * 4;
-    uint32_t *nr_l = (uint32_t *)p;
+    uint32_t *nr_l = (uint32_t *)p; p += (size_t)tstride * 4;
+    uint32_t *r0_l = (uint32_t *)p;
 
     const uint32_t grow = blockIdx.x;
     const SceneDesc S = pp.scenes[pp.row_scene[grow]];
@@ -1244,7 +1432,9 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     const uint32_t h0 = S.crop_h0, h1 = scene_h1(S);
     if (h0 >= h1) return;
     for (uint32_t k = tid; k < tstride; k += T) {
+        const uint32_t *r0 = pp.tile_run0 + (uint64_t)grow * tstride;
         ts_l[k] = pp.tile_start[(uint64_t)grow * tstride + k];
+        r0_l[k] = r0[k];
         nr_l[k] = k + 1 < tstride ? pp.tile_nr[(uint64_t)grow * tstride + k] : 0u;
     }
     __syncthreads();
@@ -1268,13 +1458,13 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     auto issue = [&](Prefetch &f, uint32_t col) {
         f.n = 0;
         if (col > h1) return;
-        const uint32_t nr2 = nr_l[col], ts2 = ts_l[col];
+        const uint32_t nr2 = nr_l[col], r02 = r0_l[col];
         f.n = nr2 < T ? nr2 : T;
         if (f.n > rcap) f.n = rcap;
         if (tid < f.n) {
-            f.key = pp.run_key[ts2 + tid];
-            f.beg = pp.run_beg[ts2 + tid];
-            f.cov = pp.run_cov[ts2 + tid];
+            f.key = pp.run_key[r02 + tid];
+            f.beg = pp.run_beg[r02 + tid];
+            f.cov = pp.run_cov[r02 + tid];
         }
     };
     // One column: the cover queue entering tile colp - 1 meets its runs.
@@ -1292,10 +1482,11 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
             rb[tid] = f.beg;
             rc[tid] = f.cov;
         }
+        const uint32_t r0 = r0_l[colp];
         for (uint32_t k = f.n + tid; k < nr; k += T) {
-            rk[k] = pp.run_key[ts + k];
-            rb[k] = pp.run_beg[ts + k];
-            rc[k] = pp.run_cov[ts + k];
+            rk[k] = pp.run_key[r0 + k];
+            rb[k] = pp.run_beg[r0 + k];
+            rc[k] = pp.run_cov[r0 + k];
         }
         issue(f, col_ahead);
         __syncthreads();

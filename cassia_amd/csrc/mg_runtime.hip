@@ -63,8 +63,8 @@ Renderer::~Renderer() {
                      &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
-                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_nr_,
-                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_work_, &walk_scratch_,
+                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
+                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_, &walk_scratch_,
                      &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_};
     for (DevBuf *b : all) b->release();
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -607,8 +607,9 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     if (n_work == 0) return true;
     const size_t n_cols = (size_t)global_rows * tstride;
     if (!rec_hdr_.reserve(std::max<size_t>(tiles, 1) * 8) || !tile_start_.reserve(n_cols * 4) ||
-        !tile_nr_.reserve(n_cols * 4) || !run_key_.reserve(words_cap * 4) || !run_beg_.reserve(words_cap * 4) ||
-        !run_cov_.reserve(words_cap * 8)) {
+        !tile_run0_.reserve(n_cols * 4) || !tile_nr_.reserve(n_cols * 4) || !run_key_.reserve(words_cap * 4) ||
+        !run_beg_.reserve((words_cap + 1) * 4) || !run_cov_.reserve(words_cap * 8) ||
+        !runs_status_.reserve(((words_cap + kRunsTileHost - 1) / kRunsTileHost + 1) * 12)) {
         err = "out of device memory (tile runs)";
         return false;
     }
@@ -627,6 +628,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.overflow = overflow_.as<uint32_t>();
     pp.tile_ops = spec.want_tile_ops ? tile_ops_.as<int8_t>() : nullptr;
     pp.tile_start = tile_start_.as<uint32_t>();
+    pp.tile_run0 = tile_run0_.as<uint32_t>();
     pp.tile_nr = tile_nr_.as<uint32_t>();
     pp.tstride = tstride;
     pp.run_key = run_key_.as<uint32_t>();
@@ -644,10 +646,30 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.n_runs = overflow_.as<uint32_t>() + 3;
     pp.n_records = overflow_.as<uint32_t>() + 2;
     fill32(overflow_.as<uint32_t>() + 3, 0, 1, stream_);
-    k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows);
-    {
+    // Tile runs: one wave per tile column while tiles are many and of
+    // moderate depth; over the whole sorted stream (work follows the words)
+    // when a frame's tiles are deep: more than 8,192 word slots per column,
+    // as in small canvases under a million lines, where a few tiles hold
+    // most of the words. Both give the same runs.
+    const bool stream_runs = words_cap > (uint64_t)8192 * std::max<uint32_t>(n_runs_work_, 1);
+    if (!stream_runs) {
+        k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows, 0u);
         const uint32_t blocks = (uint32_t)std::min<uint64_t>((n_runs_work_ + 3) / 4, (uint64_t)cus_ * 16);
-        k_tile_runs<<<blocks, 256, 0, stream_>>>(pp, runs_work_.as<uint2>(), n_runs_work_);
+        if (blocks) k_tile_runs<<<blocks, 256, 0, stream_>>>(pp, runs_work_.as<uint2>(), n_runs_work_);
+    } else {
+        // runs over the whole sorted stream, then each tile's first run
+        const uint64_t nparts = (words_cap + kRunsTileHost - 1) / kRunsTileHost;
+        fill64(runs_status_.as<uint64_t>(), 0, std::max<uint64_t>(nparts, 1), stream_);
+        // status words (8 B) then the open-run index (4 B) of each partition
+        uint32_t *open = (uint32_t *)(runs_status_.as<uint64_t>() + std::max<uint64_t>(nparts, 1));
+        if (nparts)
+            k_runs_scan<<<(uint32_t)nparts, 256, 0, stream_>>>(pp, counters_.as<FrameCounters>(),
+                                                               runs_status_.as<uint64_t>(), err_.as<uint32_t>() + 1,
+                                                               open);
+        k_run_cov<<<(uint32_t)std::max<uint64_t>(std::min<uint64_t>((nparts + 255) / 256, (uint64_t)cus_ * 4), 1), 256,
+                    0, stream_>>>(pp, counters_.as<FrameCounters>(), open);
+        k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows, 1u);
+        k_tile_nr<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows);
     }
     MG_HIP(hipGetLastError(), "tile runs launch");
     MG_HIP(mark(evs_[5]), "event");
@@ -771,8 +793,8 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
 bool Renderer::upload_work(const FrameSpec &spec, std::string &err) {
     if (work_scenes_.size() != spec.scenes.size() ||
         std::memcmp(work_scenes_.data(), spec.scenes.data(), spec.scenes.size() * sizeof(SceneDesc)) != 0) {
-        // paint: the tiles inside each crop; tile runs: columns 0..h1 (colp)
-        // of the same rows
+        // paint: the tiles inside each crop; tile runs (per-tile form):
+        // columns 0..h1 (colp) of the same rows
         std::vector<uint32_t> work, rwork;
         for (uint32_t sc = 0; sc < spec.scenes.size(); ++sc) {
             const SceneDesc &s = spec.scenes[sc];
@@ -805,7 +827,7 @@ std::vector<const void *> Renderer::frame_buffers() const {
                            &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_, &segs_b_, &hist_,
                            &bases_, &sort_status_, &part_counters_, &row_start_, &overflow_, &part_line_,
                            &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_,
-                           &cnt_status_, &work_, &runs_work_, &tile_start_, &tile_nr_, &run_key_, &run_beg_,
+                           &cnt_status_, &work_, &runs_work_, &tile_nr_, &runs_status_, &tile_start_, &tile_run0_, &run_key_, &run_beg_,
                            &run_cov_, &rec_, &rec_len_, &rec_hdr_};
     std::vector<const void *> v;
     for (const DevBuf *b : all) v.push_back(b->ptr);
