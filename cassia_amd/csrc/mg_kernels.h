@@ -102,8 +102,8 @@ __global__ void k_paint(PaintParams pp);
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride);
 size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 
-constexpr uint32_t kEmitTileHost = 1024;
-constexpr uint32_t kSortTileHost = 2048;
+constexpr uint32_t kEmitTileHost = 2048;
+constexpr uint32_t kSortTileHost = 4096;
 constexpr uint32_t kScanTileHost = 2048;
 constexpr uint32_t kRunsTileHost = 4096;  // words per k_runs_scan partition
 

@@ -420,12 +420,13 @@ __device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, u
 }
 
 constexpr uint32_t kSortThreads = 256;
-constexpr uint32_t kSortKPT = 8;
+constexpr uint32_t kSortKPT = 16;
 constexpr uint32_t kSortTile = kSortThreads * kSortKPT;  // keys per sort partition
 constexpr uint32_t kEmitThreads = 256;
 constexpr uint32_t kEmitIPT = kSortKPT / 2;
 constexpr uint32_t kEmitTile = kEmitThreads * kEmitIPT;
 static_assert(2 * kEmitTile == kSortTile, "an emission partition fills exactly one sort partition");
+static_assert(kEmitTile == kEmitTileHost && kSortTile == kSortTileHost, "host partition sizes");
 constexpr uint32_t kMaxPasses = 8;
 // Slot sentinels (bit 63 set; no word has it: render words use 57 bits and
 // emitted PixelSegments are never NONE).
@@ -856,10 +857,17 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
         const uint32_t valid_n = (uint32_t)(n - base < kSortTile ? n - base : kSortTile);
         uint64_t keys[kSortKPT];
         uint32_t rank[kSortKPT];
+        // all loads in flight before the first ranking step (bit 63 marks
+        // a dropped slot, and the tail past the input)
+#pragma unroll
         for (uint32_t i = 0; i < kSortKPT; ++i) {
-            uint32_t li = w * (64 * kSortKPT) + i * 64 + lane;
-            bool valid = li < valid_n;
-            uint64_t key = valid ? in[base + li] : 0ull;
+            const uint32_t li = w * (64 * kSortKPT) + i * 64 + lane;
+            keys[i] = li < valid_n ? in[base + li] : ~0ull;
+        }
+#pragma unroll
+        for (uint32_t i = 0; i < kSortKPT; ++i) {
+            bool valid = true;
+            uint64_t key = keys[i];
             if (key >> 63) {
                 valid = false;
                 key = 0;
