@@ -1030,23 +1030,17 @@ __device__ __forceinline__ uint64_t rlane64(uint64_t v, uint32_t l) {
     return (uint64_t)rlane((uint32_t)v, l) | ((uint64_t)rlane((uint32_t)(v >> 32), l) << 32);
 }
 
-// Cover::is_empty / is_full over 8 rows (painter/mod.rs:164-189).
+// Cover::is_empty / is_full over 8 rows (painter/mod.rs:164-189), byte-wise
+// on the packed i8 covers. |v| & 31 == 0 holds exactly when v is a multiple
+// of 32, i.e. when the byte's low 5 bits are zero (-128 included), and
+// |v| & 31 == 16 when they equal 16. |v| == 16 means the byte is 0x10 or
+// 0xF0: low 5 bits 16 and bits 7..5 all equal.
 __device__ __forceinline__ bool cover_empty(uint64_t c, bool eo) {
-    if (!eo) return c == 0;
-    for (uint32_t y = 0; y < 8; ++y) {
-        const int32_t v = cov_byte(c, y);
-        const int32_t a = as_i8(v < 0 ? -v : v);
-        if ((a & 31) != 0) return false;
-    }
-    return true;
+    return (c & (eo ? 0x1F1F1F1F1F1F1F1Full : ~0ull)) == 0;
 }
 __device__ __forceinline__ bool cover_full(uint64_t c, bool eo) {
-    for (uint32_t y = 0; y < 8; ++y) {
-        const int32_t v = cov_byte(c, y);
-        const int32_t a = as_i8(v < 0 ? -v : v);
-        if (eo ? ((a & 31) != 16) : (a != 16)) return false;
-    }
-    return true;
+    if ((c & 0x1F1F1F1F1F1F1F1Full) != 0x1010101010101010ull) return false;
+    return eo || ((((c & 0xE0E0E0E0E0E0E0E0ull) >> 1) ^ c) & 0x6060606060606060ull) == 0;
 }
 
 // Gradient::color_at for pixel row `row` of the column at (gx, gy)
@@ -1523,17 +1517,38 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
                 }
             }
         };
-        auto next_cover = [&](int32_t qi, int32_t ri, bool *keep) {
+        // next cover of a union element, and its fill rule (kRunEvenOdd bit)
+        auto next_cover = [&](int32_t qi, int32_t ri, bool *keep, uint32_t *eo_bit) {
             const uint64_t a = qi >= 0 ? qcc[qi] : 0ull, b = ri >= 0 ? rc[ri] : 0ull;
             const uint64_t v = add_bytes(a, b);
-            const bool eo = qi >= 0 ? (qkc[qi] & kRunEvenOdd) != 0 : (rk[ri] & kRunEvenOdd) != 0;
-            *keep = fold || !cover_empty(v, eo);
+            *eo_bit = (qi >= 0 ? qkc[qi] : rk[ri]) & kRunEvenOdd;
+            *keep = fold || !cover_empty(v, *eo_bit != 0);
             return v;
         };
-        uint32_t cu = 0, cq = 0;
-        walk([&](uint32_t, int32_t qi, int32_t ri) {
+        // The first two elements of the thread's slice keep what the writing
+        // pass needs in registers (one element per thread is the common case
+        // with 1,024 threads); longer slices are walked again.
+        uint32_t e_lay[2] = {0u, 0u};
+        int32_t e_qi[2] = {-1, -1}, e_ri[2] = {-1, -1};
+        uint64_t e_v[2] = {0ull, 0ull};
+        uint32_t cu = 0, cq = 0, e_keep = 0;
+        walk([&](uint32_t lay, int32_t qi, int32_t ri) {
             bool keep;
-            next_cover(qi, ri, &keep);
+            uint32_t eo_bit;
+            const uint64_t v = next_cover(qi, ri, &keep, &eo_bit);
+            if (cu == 0) {
+                e_lay[0] = lay | eo_bit;
+                e_qi[0] = qi;
+                e_ri[0] = ri;
+                e_v[0] = v;
+                e_keep |= keep ? 1u : 0u;
+            } else if (cu == 1) {
+                e_lay[1] = lay | eo_bit;
+                e_qi[1] = qi;
+                e_ri[1] = ri;
+                e_v[1] = v;
+                e_keep |= keep ? 2u : 0u;
+            }
             ++cu;
             cq += keep ? 1u : 0u;
         });
@@ -1565,11 +1580,10 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
         const bool write_rec = !fold && !counting;
         const uint32_t nxt = cur ^ 1u;
         uint32_t ou = rec_off + (uint32_t)ex, oq = (uint32_t)(ex >> 32);
-        walk([&](uint32_t lay, int32_t qi, int32_t ri) {
-            bool keep;
-            const uint64_t v = next_cover(qi, ri, &keep);
+        // lay_eo: layer | kRunEvenOdd bit (the queue entry's key)
+        auto put = [&](uint32_t lay_eo, int32_t qi, int32_t ri, uint64_t v, bool keep) {
             if (write_rec) {
-                uint32_t key = lay, beg = 0, len = 0;
+                uint32_t key = lay_eo & 0xFFFFu, beg = 0, len = 0;
                 uint64_t cv = 0;
                 if (ri >= 0) {
                     key |= kRecRun;
@@ -1585,11 +1599,22 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
                 ++ou;
             }
             if (keep) {
-                qk[nxt][oq] = lay | ((qi >= 0 ? qkc[qi] : rk[ri]) & kRunEvenOdd);
+                qk[nxt][oq] = lay_eo;
                 qc[nxt][oq] = v;
                 ++oq;
             }
-        });
+        };
+        if (cu <= 2) {
+            if (cu > 0) put(e_lay[0], e_qi[0], e_ri[0], e_v[0], (e_keep & 1u) != 0);
+            if (cu > 1) put(e_lay[1], e_qi[1], e_ri[1], e_v[1], (e_keep & 2u) != 0);
+        } else {
+            walk([&](uint32_t lay, int32_t qi, int32_t ri) {
+                bool keep;
+                uint32_t eo_bit;
+                const uint64_t v = next_cover(qi, ri, &keep, &eo_bit);
+                put(lay | eo_bit, qi, ri, v, keep);
+            });
+        }
         if (write_rec && tid == 0) {
             const uint64_t tile = (uint64_t)S.tile_base + (uint64_t)j * S.row_len + (colp - 1);
             pp.rec_hdr[tile] = make_uint2(rec_off, n_union);
