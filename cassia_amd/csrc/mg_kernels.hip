@@ -490,27 +490,53 @@ constexpr uint32_t kCountSlices = 256;  // accumulation slices of the later pass
 // output offsets, so that pass needs no look-back) and the other counts are
 // added into slice part % kCountSlices of `acc` (reduced to totals).
 
+__device__ __forceinline__ uint32_t wave_sum(uint32_t v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
 __device__ __forceinline__ uint32_t count_rows(const FrameDims &d) { return d.passes * (1u << d.radix_bits) + 2; }
 
 __device__ __forceinline__ void count_zero(uint32_t *s_cnt, const FrameDims &d) {
     for (uint32_t k = threadIdx.x; k < count_rows(d); k += blockDim.x) s_cnt[k] = 0;
 }
 
-__device__ __forceinline__ void count_slot(uint32_t *s_cnt, uint64_t v, const FrameDims &d) {
+// Counts the slot of every active lane. A wave's slots are consecutive
+// line-pixels of a few lines, so their tile rows and columns, and hence the
+// digits of every pass, take only a few distinct values: the lanes holding
+// one digit add together, with one LDS atomic per distinct digit (per-lane
+// atomics on one address serialize over the wave). Kept words are counted in
+// the thread's `kept`, culled words once per wave.
+__device__ __forceinline__ void count_slot(uint32_t *s_cnt, uint64_t v, const FrameDims &d, uint32_t &kept) {
     const uint32_t radix = 1u << d.radix_bits, nd = d.passes * radix;
-    if (v >> 63) {
-        if (v == kSlotCull) atomicAdd(&s_cnt[nd + 1], 1u);
-        return;
-    }
-    atomicAdd(&s_cnt[nd], 1u);
+    const uint32_t lane = lane_id();
+    const uint64_t exec = __ballot(true);
+    const uint32_t leader0 = (uint32_t)__ffsll((long long)exec) - 1u;
+    const uint64_t culled = __ballot(v == kSlotCull);
+    if (culled && lane == leader0) atomicAdd(&s_cnt[nd + 1], (uint32_t)__popcll(culled));
+    const bool ok = !(v >> 63);
+    kept += ok ? 1u : 0u;
+    const uint64_t oks = __ballot(ok);
     uint32_t shift = d.sort_shift;
-    for (uint32_t ps = 0; ps < d.passes; ++ps, shift += d.radix_bits)
-        atomicAdd(&s_cnt[ps * radix + ((uint32_t)(v >> shift) & (radix - 1))], 1u);
+    for (uint32_t ps = 0; ps < d.passes; ++ps, shift += d.radix_bits) {
+        const uint32_t dg = ok ? (uint32_t)(v >> shift) & (radix - 1) : 0xFFFFFFFFu;
+        uint64_t todo = oks;
+        while (todo) {
+            const uint32_t leader = (uint32_t)__ffsll((long long)todo) - 1u;
+            const uint32_t dl = (uint32_t)__builtin_amdgcn_readlane((int)dg, (int)leader);
+            const uint64_t same = __ballot(dg == dl);
+            if (lane == leader) atomicAdd(&s_cnt[ps * radix + dl], (uint32_t)__popcll(same));
+            todo &= ~same;
+        }
+    }
 }
 
 // Called by the whole work group after its last count_slot.
-__device__ __forceinline__ void count_flush(const uint32_t *s_cnt, const FrameDims &d, const SlotCounts &sc,
-                                            uint32_t part, uint32_t nparts) {
+__device__ __forceinline__ void count_flush(uint32_t *s_cnt, const FrameDims &d, const SlotCounts &sc,
+                                            uint32_t part, uint32_t nparts, uint32_t kept) {
+    const uint32_t nd = d.passes * (1u << d.radix_bits);
+    const uint32_t kw = wave_sum(kept);
+    if (lane_id() == 0 && kw) atomicAdd(&s_cnt[nd], kw);
     __syncthreads();
     const uint32_t radix = 1u << d.radix_bits, rest = count_rows(d) - radix;
     for (uint32_t k = threadIdx.x; k < radix; k += blockDim.x) sc.cnt0[(uint64_t)k * nparts + part] = s_cnt[k];
@@ -526,7 +552,7 @@ template <bool Staged>
 __device__ __forceinline__ void emit_span(const LineSetup *lines, const uint32_t *offs, uint32_t l0, uint32_t l1,
                                           uint64_t b0, uint64_t b1, uint64_t P,
                                           const SceneDesc *__restrict__ scenes, const FrameDims &dims,
-                                          uint4 *__restrict__ slots, uint32_t *s_cnt) {
+                                          uint4 *__restrict__ slots, uint32_t *s_cnt, uint32_t &kept) {
     const uint32_t base = Staged ? l0 : 0u;
     // offset of line l in [l0, l1 + 1]: first line-pixel of l
     auto off = [&](uint32_t l) -> uint64_t { return l >= dims.n_lines ? P : (uint64_t)offs[l - base]; };
@@ -557,8 +583,8 @@ __device__ __forceinline__ void emit_span(const LineSetup *lines, const uint32_t
         }
         if (n < 2) v[1] = kSlotNone;
         slots[p] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
-        count_slot(s_cnt, v[0], dims);
-        count_slot(s_cnt, v[1], dims);
+        count_slot(s_cnt, v[0], dims, kept);
+        count_slot(s_cnt, v[1], dims, kept);
     }
 }
 
@@ -584,17 +610,18 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     const uint32_t l1 = part + 1 < nparts ? part_line[part + 1] : dims.n_lines - 1;
     const uint32_t nl = l1 - l0 + 1;
     uint4 *slots = reinterpret_cast<uint4 *>(out);
+    uint32_t kept = 0;
     if (nl <= kEmitLines) {
         for (uint32_t k = threadIdx.x; k <= nl; k += blockDim.x)
             s_off[k] = l0 + k < dims.n_lines ? offsets[l0 + k] : (uint32_t)P;
         for (uint32_t k = threadIdx.x; k < nl; k += blockDim.x) s_lines[k] = lines[l0 + k];
         __syncthreads();
-        emit_span<true>(s_lines, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt);
+        emit_span<true>(s_lines, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
     } else {  // a partition of very short or zero-length lines
         __syncthreads();
-        emit_span<false>(lines, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt);
+        emit_span<false>(lines, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
     }
-    count_flush(s_cnt, dims, sc, part, nparts);
+    count_flush(s_cnt, dims, sc, part, nparts, kept);
 }
 
 // Totals of the later passes' digits (into hist[radix ..]) and of the kept
@@ -781,6 +808,7 @@ __global__ __launch_bounds__(256) void k_ingest(const uint64_t *__restrict__ in,
     count_zero(s_cnt, dims);
     __syncthreads();
     const SceneDesc &S = scenes[0];
+    uint32_t kept = 0;
     for (uint32_t k = 0; k < kSortTile / 256; ++k) {
         const uint64_t i = (uint64_t)part * kSortTile + k * 256 + threadIdx.x;
         if (i >= n_slots) break;
@@ -793,9 +821,9 @@ __global__ __launch_bounds__(256) void k_ingest(const uint64_t *__restrict__ in,
             if (((uint32_t)(w >> 22) & 0xFFFFu) < S.n_props && render_word(w, S, dims, &rw, &grow)) v = rw;
         }
         out[i] = v;
-        count_slot(s_cnt, v, dims);
+        count_slot(s_cnt, v, dims, kept);
     }
-    count_flush(s_cnt, dims, sc, part, nparts);
+    count_flush(s_cnt, dims, sc, part, nparts, kept);
 }
 
 // Exclusive scan of each pass's digit histogram (one block per pass).
