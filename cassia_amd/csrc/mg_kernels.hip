@@ -1686,8 +1686,9 @@ __device__ __forceinline__ void stage_load(Stage &st, const uint64_t *segs, uint
     __syncthreads();
     for (uint32_t k = lane; k < to - from; k += 64) st.w[k] = segs[from + k];
     __syncthreads();
-    st.base = from;
-    st.end = to;
+    // wave-uniform: kept in scalar registers
+    st.base = (uint32_t)__builtin_amdgcn_readfirstlane((int)from);
+    st.end = (uint32_t)__builtin_amdgcn_readfirstlane((int)to);
 }
 
 // A = 16 * P + area for this lane's pixel (painter/mod.rs:229-243, 347-362):
@@ -1697,26 +1698,25 @@ __device__ __forceinline__ void stage_load(Stage &st, const uint64_t *segs, uint
 // The lanes take the run's words 64 at a time and add area and cover into
 // the wave's 64 cells (LDS, no return values); each lane then reads its own
 // cell and the column prefix is a stride-8 scan across lanes (lane = x*8+y).
-__device__ __forceinline__ int32_t run_area(Stage &st, int32_t *cells, const uint64_t *segs, uint32_t beg,
-                                            uint32_t len, uint32_t lim, int32_t cy) {
+// The words come from the staged window when the run lies inside it, else
+// from global memory.
+__device__ __forceinline__ int32_t run_area(const Stage &st, int32_t *cells, const uint64_t *segs, uint32_t beg,
+                                            uint32_t len, int32_t cy) {
     const uint32_t lane = lane_id();
     int32_t *c_area = cells, *c_cov = cells + 64;
     c_area[lane] = 0;
     c_cov[lane] = 0;
     const uint32_t e = beg + len;
-    const uint32_t hi = lim > e ? lim : e;
-    for (uint32_t s0 = beg; s0 < e; s0 += 64) {
-        if (s0 < st.base || s0 + 64 > st.end) {
-            if (!(s0 >= st.base && e <= st.end))
-                stage_load(st, segs, s0, hi - s0 < kStageWin ? hi : s0 + kStageWin);
-        }
-        const uint32_t s = s0 + lane;
-        if (s < e) {
-            const uint64_t w = st.w[s - st.base];
-            const uint32_t cell = ((uint32_t)(w >> 16) & 7u) * 8u + ((uint32_t)(w >> 19) & 7u);
-            atomicAdd(&c_area[cell], (int32_t)((int64_t)(w << 48) >> 54));
-            atomicAdd(&c_cov[cell], (int32_t)((int64_t)(w << 58) >> 58));
-        }
+    auto add = [&](uint64_t w) {
+        const uint32_t cell = ((uint32_t)(w >> 16) & 7u) * 8u + ((uint32_t)(w >> 19) & 7u);
+        atomicAdd(&c_area[cell], (int32_t)((int64_t)(w << 48) >> 54));
+        atomicAdd(&c_cov[cell], (int32_t)((int64_t)(w << 58) >> 58));
+    };
+    // separate loops keep the LDS and global reads in their own address spaces
+    if (beg >= st.base && e <= st.end) {
+        for (uint32_t s = beg + lane; s < e; s += 64) add(st.w[s - st.base]);
+    } else {
+        for (uint32_t s = beg + lane; s < e; s += 64) add(segs[s]);
     }
     const int32_t ar = c_area[lane], cv = c_cov[lane];
     int32_t inc = cv, t;
@@ -1729,12 +1729,16 @@ __device__ __forceinline__ int32_t run_area(Stage &st, int32_t *cells, const uin
     return 16 * as_i8(cy + inc - cv) + as_i16(ar);
 }
 
-// One batch of up to 64 records, lane k holding record k0 + k.
+// One batch of up to 64 records, lane k holding record k0 + k. Only the
+// first `cnt` are used: as many as keep the words of their runs within one
+// staging window (at least one record).
 struct Batch {
     uint32_t key, beg, len, flags;
     uint64_t cov;
     float4 color;
-    uint32_t lim;  // end of the batch's last run
+    uint32_t cnt;    // records of the batch (wave-uniform)
+    uint32_t first;  // first word of the batch's runs
+    uint32_t lim;    // end of the batch's last run (lim == first: no run)
 };
 
 __device__ __forceinline__ void load_batch(const PaintParams &pp, const PropsEntry *props, const uint32_t *fl,
@@ -1756,28 +1760,42 @@ __device__ __forceinline__ void load_batch(const PaintParams &pp, const PropsEnt
         b.flags = 0;
         b.color = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
-    const uint64_t rm = __ballot(in && (rv.x & kRecRun));
-    b.lim = 0;
+    const uint32_t avail = n - k0 < 64 ? n - k0 : 64;
+    const bool run = in && (rv.x & kRecRun);
+    const uint64_t rm = __ballot(run);
+    b.cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)avail);
+    b.first = b.lim = 0;
     if (rm) {
-        const uint32_t last = 63u - (uint32_t)__clzll((long long)rm);
-        b.lim = rlane(b.beg + b.len, last);
+        b.first = rlane(b.beg, (uint32_t)__ffsll((long long)rm) - 1u);
+        // the first run ending beyond the window closes the batch (a run
+        // longer than the window makes a batch of its own)
+        const uint64_t over = __ballot(run && b.beg + b.len - b.first > kStageWin);
+        uint64_t used = rm;
+        if (over) {
+            const uint32_t fo = (uint32_t)__ffsll((long long)over) - 1u;
+            b.cnt = fo > 0 ? fo : 1u;
+            used &= b.cnt >= 64 ? ~0ull : (1ull << b.cnt) - 1ull;
+        }
+        const uint32_t last = 63u - (uint32_t)__clzll((long long)used);
+        b.lim = used ? rlane(b.beg + b.len, last) : b.first;
     }
 }
 
-// Stages the batch's runs when they fit the window.
+// Stages the batch's runs (when they fit the window and are not staged yet).
 __device__ __forceinline__ void stage_batch(Stage &st, const uint64_t *segs, const Batch &b) {
-    const uint64_t rm = __ballot(b.key & kRecRun);
-    if (!rm) return;
-    const uint32_t first = rlane(b.beg, (uint32_t)__ffsll((long long)rm) - 1u);
-    if (b.lim - first <= kStageWin && (first < st.base || b.lim > st.end)) stage_load(st, segs, first, b.lim);
+    if (b.lim > b.first && b.lim - b.first <= kStageWin && (b.first < st.base || b.lim > st.end))
+        stage_load(st, segs, b.first, b.lim);
 }
 
-__device__ __forceinline__ int32_t record_area(Stage &st, int32_t *cells, const uint64_t *segs, const Batch &b,
+__device__ __forceinline__ int32_t record_area(const Stage &st, int32_t *cells, const uint64_t *segs, const Batch &b,
                                                uint32_t q, uint32_t py) {
     const uint32_t key = rlane(b.key, q);
-    const int32_t cy = (key & kRecCarry) ? cov_byte(rlane64(b.cov, q), py) : 0;
+    // carried cover of row py (records without a carry hold zero covers)
+    const uint32_t lo = rlane((uint32_t)b.cov, q), hi = rlane((uint32_t)(b.cov >> 32), q);
+    const uint32_t half = py < 4 ? lo : hi;
+    const int32_t cy = as_i8((int32_t)(half >> (8 * (py & 3))));
     if (!(key & kRecRun)) return 16 * cy;
-    return run_area(st, cells, segs, rlane(b.beg, q), rlane(b.len, q), b.lim, cy);
+    return run_area(st, cells, segs, rlane(b.beg, q), rlane(b.len, q), cy);
 }
 
 // blend_at for one pixel (painter/mod.rs:364-400): one dispatch on the
@@ -1829,11 +1847,11 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
     bool has_clip = false;
     uint32_t clip_last = 0;
     float clip_mask = 0.0f;
-    for (uint32_t k0 = skipped; k0 < n; k0 += 64) {
+    for (uint32_t k0 = skipped, cnt = 0; k0 < n; k0 += cnt) {
         Batch b;
         load_batch(pp, props, fl, rec, rlen, n, k0, b);
         stage_batch(st, segs, b);
-        const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
+        cnt = b.cnt;
         for (uint32_t q = 0; q < cnt; ++q) {
             const uint32_t f = rlane(b.flags, q);
             if (Clips && !(f & kMaskBit)) continue;
@@ -1854,17 +1872,21 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
             const uint64_t nz = __ballot(cov != 0.0f);
             const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
             if (col_zero || (apply_clip && !has_clip)) continue;
-            float f0 = rlane_f(b.color.x, q), f1 = rlane_f(b.color.y, q), f2 = rlane_f(b.color.z, q),
-                  f3 = rlane_f(b.color.w, q);
+            // blend_at (painter/mod.rs:364-400); a solid layer's colour stays
+            // in scalar registers (its own copy of the blend code)
+            const uint32_t mode = (f >> PF_BLEND_SHIFT) & 15u;
             if (f & PF_GRADIENT) {
                 float g[4];
                 gradient_color(props[lay], pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, g);
-                f0 = g[0]; f1 = g[1]; f2 = g[2]; f3 = g[3];
+                float a = g[3] * cov;
+                if (apply_clip) a *= clip_mask;
+                blend_pixel(mode, c0, c1, c2, ca, g[0], g[1], g[2], a);
+            } else {
+                float a = rlane_f(b.color.w, q) * cov;
+                if (apply_clip) a *= clip_mask;
+                blend_pixel(mode, c0, c1, c2, ca, rlane_f(b.color.x, q), rlane_f(b.color.y, q), rlane_f(b.color.z, q),
+                            a);
             }
-            // blend_at (painter/mod.rs:364-400)
-            float a = f3 * cov;
-            if (apply_clip) a *= clip_mask;
-            blend_pixel((f >> PF_BLEND_SHIFT) & 15u, c0, c1, c2, ca, f0, f1, f2, a);
         }
     }
 }
@@ -1903,11 +1925,11 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
 
     // ---- coverage dump (debug): A of every id, independent of the passes
     if (pp.cov_a) {
-        for (uint32_t k0 = 0; k0 < n; k0 += 64) {
+        for (uint32_t k0 = 0, cnt = 0; k0 < n; k0 += cnt) {
             Batch b;
             load_batch(pp, props, fl, rec, rlen, n, k0, b);
             stage_batch(st, segs, b);
-            const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
+            cnt = b.cnt;
             for (uint32_t q = 0; q < cnt; ++q) {
                 const int32_t A = record_area(st, cells, segs, b, q, py);
                 uint32_t r = 0;
@@ -2041,7 +2063,7 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
                 for (uint32_t k0 = from; k0 < n; k0 += 64) {
                     Batch b;
                     load_batch(pp, props, fl, rec, rlen, n, k0, b);
-                    const uint32_t cnt = n - k0 < 64 ? n - k0 : 64;
+                    const uint32_t cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)(n - k0 < 64 ? n - k0 : 64));
                     for (uint32_t q = 0; q < cnt; ++q) {
                         const uint32_t f = rlane(b.flags, q);
                         if (!(f & kMaskBit)) continue;
