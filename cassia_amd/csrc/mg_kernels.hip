@@ -502,8 +502,8 @@ __device__ __forceinline__ void count_zero(uint32_t *s_cnt, const FrameDims &d) 
 }
 
 // Counts the slot of every active lane. A wave's slots are consecutive
-// line-pixels of a few lines, so their tile rows and columns, and hence the
-// digits of every pass, take only a few distinct values: the lanes holding
+// line-pixels of a few lines, so the digits of the later passes (high tile
+// column and tile row bits) take only one or two values: the lanes holding
 // one digit add together, with one LDS atomic per distinct digit (per-lane
 // atomics on one address serialize over the wave). Kept words are counted in
 // the thread's `kept`, culled words once per wave.
@@ -517,8 +517,12 @@ __device__ __forceinline__ void count_slot(uint32_t *s_cnt, uint64_t v, const Fr
     const bool ok = !(v >> 63);
     kept += ok ? 1u : 0u;
     const uint64_t oks = __ballot(ok);
+    // pass 0 (the low tile-column bits) spreads over several digits: per-lane
+    // atomics cost less than the grouping loop there
     uint32_t shift = d.sort_shift;
-    for (uint32_t ps = 0; ps < d.passes; ++ps, shift += d.radix_bits) {
+    if (ok) atomicAdd(&s_cnt[(uint32_t)(v >> shift) & (radix - 1)], 1u);
+    shift += d.radix_bits;
+    for (uint32_t ps = 1; ps < d.passes; ++ps, shift += d.radix_bits) {
         const uint32_t dg = ok ? (uint32_t)(v >> shift) & (radix - 1) : 0xFFFFFFFFu;
         uint64_t todo = oks;
         while (todo) {
@@ -546,10 +550,12 @@ __device__ __forceinline__ void count_flush(uint32_t *s_cnt, const FrameDims &d,
 }
 
 // Line-pixels b0 + threadIdx.x + 256 k of [.., b1) (see k_emit). With
-// Staged, `offs` and `lines` are LDS tables of lines l0 .. (indexed l - l0);
-// otherwise they are the global arrays.
+// Staged, `offs`, `lines` and `culls` are LDS tables of lines l0 ..
+// (indexed l - l0); otherwise `offs` and `lines` are the global arrays and
+// the cull bounds come from the scene table.
 template <bool Staged>
-__device__ __forceinline__ void emit_span(const LineSetup *lines, const uint32_t *offs, uint32_t l0, uint32_t l1,
+__device__ __forceinline__ void emit_span(const LineSetup *lines, const SceneCull *culls, const uint32_t *offs,
+                                          uint32_t l0, uint32_t l1,
                                           uint64_t b0, uint64_t b1, uint64_t P,
                                           const SceneDesc *__restrict__ scenes, const FrameDims &dims,
                                           uint4 *__restrict__ slots, uint32_t *s_cnt, uint32_t &kept) {
@@ -573,7 +579,7 @@ __device__ __forceinline__ void emit_span(const LineSetup *lines, const uint32_t
             next = off(li + 1);
         }
         const LineSetup L = lines[li - base];
-        const SceneCull sc = scene_cull(scenes[L.packed >> 19]);
+        const SceneCull sc = Staged ? culls[li - base] : scene_cull(scenes[L.packed >> 19]);
         uint64_t w[2];
         const int n = rasterize_pixel(L, (int32_t)(p - first), w);
         uint64_t v[2] = {w[0], w[1]};
@@ -596,6 +602,7 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
                                               SlotCounts sc) {
     __shared__ uint32_t s_off[kEmitLines + 1];
     __shared__ LineSetup s_lines[kEmitLines];
+    __shared__ SceneCull s_culls[kEmitLines];
     extern __shared__ uint32_t s_cnt[];
     const uint64_t P = counters->line_pixels;
     const uint32_t nparts = (uint32_t)((P + kEmitTile - 1) / kEmitTile);
@@ -614,12 +621,16 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     if (nl <= kEmitLines) {
         for (uint32_t k = threadIdx.x; k <= nl; k += blockDim.x)
             s_off[k] = l0 + k < dims.n_lines ? offsets[l0 + k] : (uint32_t)P;
-        for (uint32_t k = threadIdx.x; k < nl; k += blockDim.x) s_lines[k] = lines[l0 + k];
+        for (uint32_t k = threadIdx.x; k < nl; k += blockDim.x) {
+            const LineSetup L = lines[l0 + k];
+            s_lines[k] = L;
+            s_culls[k] = scene_cull(scenes[L.packed >> 19]);
+        }
         __syncthreads();
-        emit_span<true>(s_lines, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
+        emit_span<true>(s_lines, s_culls, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
     } else {  // a partition of very short or zero-length lines
         __syncthreads();
-        emit_span<false>(lines, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
+        emit_span<false>(lines, nullptr, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
     }
     count_flush(s_cnt, dims, sc, part, nparts, kept);
 }
