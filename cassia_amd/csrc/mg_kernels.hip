@@ -99,25 +99,37 @@ __device__ uint64_t lookback_wave(const uint64_t *status, uint32_t part, uint32_
     return excl;
 }
 
-// Block-wide exclusive scan of one u32 per thread (blockDim = 256).
-__device__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *wsum /* 4 */, uint32_t *total) {
+// Inclusive sum over the 64 lanes of a wave (all lanes active): shifts by
+// 1, 2, 4, 8 inside each 16-lane row, then the row totals carried forward
+// by the row broadcasts of lanes 15 and 31 (one DPP add each).
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false);  // row_shr:1
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false);  // row_shr:2
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false);  // row_shr:4
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false);  // row_shr:8
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false);  // row_bcast:15
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false);  // row_bcast:31
+    return x;
+}
+
+// Block-wide exclusive scan of one u32 per thread over W waves (W <= 64):
+// wave scans, then one wave-scan of the wave totals read from LDS.
+template <uint32_t W>
+__device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *wsum /* W */, uint32_t *total) {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
-    uint32_t x = v;
-    for (int off = 1; off < 64; off <<= 1) {
-        uint32_t y = __shfl_up(x, off);
-        if (lane >= (uint32_t)off) x += y;
-    }
+    const uint32_t x = wave_incl_scan(v);
     if (lane == 63) wsum[w] = x;
     __syncthreads();
-    uint32_t base = 0, all = 0;
-    for (uint32_t k = 0; k < 4; ++k) {
-        uint32_t s = wsum[k];
-        if (k < w) base += s;
-        all += s;
-    }
+    const uint32_t t = wave_incl_scan(lane < W ? wsum[lane] : 0u);
+    const uint32_t base = w ? (uint32_t)__builtin_amdgcn_readlane((int)t, (int)w - 1) : 0u;
+    *total = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)W - 1);
     __syncthreads();
-    *total = all;
     return base + x - v;
+}
+
+// Block-wide exclusive scan of one u32 per thread (blockDim = 256).
+__device__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *wsum /* 4 */, uint32_t *total) {
+    return block_excl_scan_u32<4>(v, wsum, total);
 }
 
 // Block-wide exclusive scan of one value (u32 or u64) per thread over W waves.
@@ -1591,9 +1603,17 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
             ++cu;
             cq += keep ? 1u : 0u;
         });
-        uint64_t all;
-        // (union entries, kept queue entries) in the low and high halves
-        const uint64_t ex = block_excl_scan<T / 64>((uint64_t)cu | ((uint64_t)cq << 32), s_wsum, &all);
+        // (union entries, kept queue entries) in the low and high halves:
+        // 16 bits each when the LDS arrays bound them (cap < 2^16)
+        uint64_t ex, all;
+        if (G) {
+            ex = block_excl_scan<T / 64>((uint64_t)cu | ((uint64_t)cq << 32), s_wsum, &all);
+        } else {
+            uint32_t all32;
+            const uint32_t ex32 = block_excl_scan_u32<T / 64>(cu | (cq << 16), (uint32_t *)s_wsum, &all32);
+            ex = (uint64_t)(ex32 & 0xFFFFu) | ((uint64_t)(ex32 >> 16) << 32);
+            all = (uint64_t)(all32 & 0xFFFFu) | ((uint64_t)(all32 >> 16) << 32);
+        }
         const uint32_t n_union = (uint32_t)all, nq_new = (uint32_t)(all >> 32);
         uint32_t rec_off = 0;
         if (!fold && counting) {
