@@ -1822,19 +1822,16 @@ __device__ __forceinline__ int32_t record_area(const Stage &st, int32_t *cells, 
                                                uint32_t q, uint32_t py) {
     const uint32_t key = rlane(b.key, q);
     // carried cover of row py (records without a carry hold zero covers)
-    const uint32_t lo = rlane((uint32_t)b.cov, q), hi = rlane((uint32_t)(b.cov >> 32), q);
-    const uint32_t half = py < 4 ? lo : hi;
-    const int32_t cy = as_i8((int32_t)(half >> (8 * (py & 3))));
+    const int32_t cy = as_i8((int32_t)(rlane64(b.cov, q) >> (8 * py)));
     if (!(key & kRecRun)) return 16 * cy;
     return run_area(st, cells, segs, rlane(b.beg, q), rlane(b.len, q), cy);
 }
 
-// blend_at for one pixel (painter/mod.rs:364-400): one dispatch on the
-// layer's (uniform) blend mode for all three channels.
-__device__ __forceinline__ void blend_pixel(uint32_t mode, float &c0, float &c1, float &c2, float &ca, float f0,
-                                            float f1, float f2, float a) {
-    const float inv = 1.0f - a;
-    float b0, b1, b2;
+// The source terms of blend_at for one pixel (painter/mod.rs:364-400):
+// blend(dst, src) * alpha per channel, one dispatch on the layer's (uniform)
+// blend mode for all three channels.
+__device__ __forceinline__ void blend_terms(uint32_t mode, float c0, float c1, float c2, float f0, float f1,
+                                            float f2, float a, float &b0, float &b1, float &b2) {
     switch (mode) {
 #define MG_BLEND_CASE(m)                  \
     case m:                               \
@@ -1860,10 +1857,21 @@ __device__ __forceinline__ void blend_pixel(uint32_t mode, float &c0, float &c1,
         b2 = f2 * a;
         break;
     }
-    c0 = __builtin_fmaf(c0, inv, b0);
-    c1 = __builtin_fmaf(c1, inv, b1);
-    c2 = __builtin_fmaf(c2, inv, b2);
-    ca = __builtin_fmaf(ca, inv, a);
+}
+
+// dst * (1 - alpha) + term (painter/mod.rs:364-400), as fused multiply-adds
+// that update the accumulators in place (the compiler's accumulate form
+// writes into the term's register and copies it back on every layer).
+__device__ __forceinline__ void fma_in_place(float &c, float m, float t) {
+    asm("v_fma_f32 %0, %0, %1, %2" : "+v"(c) : "v"(m), "v"(t));
+}
+__device__ __forceinline__ void blend_apply(float &c0, float &c1, float &c2, float &ca, float b0, float b1, float b2,
+                                            float a) {
+    const float inv = 1.0f - a;
+    fma_in_place(c0, inv, b0);
+    fma_in_place(c1, inv, b1);
+    fma_in_place(c2, inv, b2);
+    fma_in_place(ca, inv, a);
 }
 
 // The painter over records [skipped, n) of a tile (painter/mod.rs:260-400):
@@ -1904,20 +1912,23 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
             const bool col_zero = ((nz >> (8 * px)) & 0xFFull) == 0;
             if (col_zero || (apply_clip && !has_clip)) continue;
             // blend_at (painter/mod.rs:364-400); a solid layer's colour stays
-            // in scalar registers (its own copy of the blend code)
+            // in scalar registers (its own copy of the blend terms), and one
+            // accumulation serves both
             const uint32_t mode = (f >> PF_BLEND_SHIFT) & 15u;
+            float t0, t1, t2, a;
             if (f & PF_GRADIENT) {
                 float g[4];
                 gradient_color(props[lay], pp.stops, (float)(px + 1 + c * 8), (float)(j * 8), py, g);
-                float a = g[3] * cov;
+                a = g[3] * cov;
                 if (apply_clip) a *= clip_mask;
-                blend_pixel(mode, c0, c1, c2, ca, g[0], g[1], g[2], a);
+                blend_terms(mode, c0, c1, c2, g[0], g[1], g[2], a, t0, t1, t2);
             } else {
-                float a = rlane_f(b.color.w, q) * cov;
+                a = rlane_f(b.color.w, q) * cov;
                 if (apply_clip) a *= clip_mask;
-                blend_pixel(mode, c0, c1, c2, ca, rlane_f(b.color.x, q), rlane_f(b.color.y, q), rlane_f(b.color.z, q),
-                            a);
+                blend_terms(mode, c0, c1, c2, rlane_f(b.color.x, q), rlane_f(b.color.y, q), rlane_f(b.color.z, q), a,
+                            t0, t1, t2);
             }
+            blend_apply(c0, c1, c2, ca, t0, t1, t2, a);
         }
     }
 }
