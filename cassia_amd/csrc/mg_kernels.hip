@@ -1355,6 +1355,27 @@ __global__ __launch_bounds__(256) void k_run_cov(PaintParams pp, const FrameCoun
     }
 }
 
+// Segmented inclusive scan over the 64 lanes of a wave (all active) of
+// byte-wise wrapping sums: (f, s) then (f', s') gives (f | f', f' ? s' :
+// s + s'). The DPP steps of wave_incl_scan; a lane whose source is outside
+// the row or the enabled rows combines with (0, 0), the identity.
+template <int Ctrl, int Rows>
+__device__ __forceinline__ void seg_scan_step(uint32_t &f, uint64_t &s) {
+    const uint32_t fp = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)f, Ctrl, Rows, 0xF, false);
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)s, Ctrl, Rows, 0xF, false);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(s >> 32), Ctrl, Rows, 0xF, false);
+    if (!f) s = add_bytes(((uint64_t)hi << 32) | lo, s);
+    f |= fp;
+}
+__device__ __forceinline__ void wave_seg_scan_bytes(uint32_t &f, uint64_t &s) {
+    seg_scan_step<0x111, 0xF>(f, s);  // row_shr:1
+    seg_scan_step<0x112, 0xF>(f, s);  // row_shr:2
+    seg_scan_step<0x114, 0xF>(f, s);  // row_shr:4
+    seg_scan_step<0x118, 0xF>(f, s);  // row_shr:8
+    seg_scan_step<0x142, 0xA>(f, s);  // row_bcast:15
+    seg_scan_step<0x143, 0xC>(f, s);  // row_bcast:31
+}
+
 // Per-tile form of the runs, for frames with many tiles of moderate depth:
 // one wave per tile column, the tile's runs stored from its first word's
 // index on (tile_run0 = tile_start).
@@ -1383,7 +1404,7 @@ __device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_
             const uint32_t nvalid = te - base < 64 ? te - base : 64;
             const bool valid = lane < nvalid;
             const uint32_t layer = valid ? (uint32_t)(w[b] >> kPayloadBits) & lmask : 0xFFFFFFFFu;
-            uint32_t prevl = __shfl_up(layer, 1);
+            uint32_t prevl = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)layer, 0x138, 0xF, 0xF, false);  // wave_shr:1
             if (lane == 0) prevl = prev_layer;
             const bool head = valid && layer != prevl;
             const uint64_t heads = __ballot(head);
@@ -1397,15 +1418,7 @@ __device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_
             }
             // segmented inclusive scan: (f, s) (+) (f', s') = (f | f', f' ? s' : s + s')
             uint32_t f = head ? 1u : 0u;
-#pragma unroll
-            for (uint32_t d = 1; d < 64; d <<= 1) {
-                const uint64_t sp = __shfl_up(s, d);
-                const uint32_t fp = __shfl_up(f, d);
-                if (lane >= d) {
-                    if (!f) s = add_bytes(sp, s);
-                    f |= fp;
-                }
-            }
+            wave_seg_scan_bytes(f, s);
             if (!f) s = add_bytes(open, s);
             const uint32_t r = nr - 1u + (uint32_t)__popcll(heads & ((2ull << lane) - 1ull));
             if (head) {
