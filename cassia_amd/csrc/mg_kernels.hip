@@ -127,6 +127,18 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *ws
     return base + x - v;
 }
 
+// Inclusive maximum over the 64 lanes of a wave (all active), the DPP steps
+// of wave_incl_scan.
+__device__ __forceinline__ uint32_t wave_incl_max(uint32_t x) {
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x111, 0xF, 0xF, false));  // row_shr:1
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x112, 0xF, 0xF, false));  // row_shr:2
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x114, 0xF, 0xF, false));  // row_shr:4
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x118, 0xF, 0xF, false));  // row_shr:8
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x142, 0xA, 0xF, false));  // row_bcast:15
+    x = max(x, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x143, 0xC, 0xF, false));  // row_bcast:31
+    return x;
+}
+
 // Block-wide exclusive scan of one u32 per thread (blockDim = 256).
 __device__ uint32_t block_excl_scan_256(uint32_t v, uint32_t *wsum /* 4 */, uint32_t *total) {
     return block_excl_scan_u32<4>(v, wsum, total);
@@ -561,35 +573,22 @@ __device__ __forceinline__ void count_flush(uint32_t *s_cnt, const FrameDims &d,
         if (s_cnt[radix + k]) atomicAdd(&a[k], s_cnt[radix + k]);
 }
 
-// Line-pixels b0 + threadIdx.x + 256 k of [.., b1) (see k_emit). With
-// Staged, `offs`, `lines` and `culls` are LDS tables of lines l0 ..
-// (indexed l - l0); otherwise `offs` and `lines` are the global arrays and
-// the cull bounds come from the scene table.
+// Line-pixels b0 + threadIdx.x + 256 k of [.., b1) (see k_emit); owner[i]
+// is the line (relative to l0) holding line-pixel b0 + i. With Staged,
+// `offs`, `lines` and `culls` are LDS tables of lines l0 .. (indexed
+// l - l0); otherwise `offs` and `lines` are the global arrays and the cull
+// bounds come from the scene table.
 template <bool Staged>
 __device__ __forceinline__ void emit_span(const LineSetup *lines, const SceneCull *culls, const uint32_t *offs,
                                           uint32_t l0, uint32_t l1,
                                           uint64_t b0, uint64_t b1, uint64_t P,
                                           const SceneDesc *__restrict__ scenes, const FrameDims &dims,
-                                          uint4 *__restrict__ slots, uint32_t *s_cnt, uint32_t &kept) {
+                                          uint4 *__restrict__ slots, uint32_t *s_cnt, const uint32_t *owner,
+                                          uint32_t &kept) {
     const uint32_t base = Staged ? l0 : 0u;
-    // offset of line l in [l0, l1 + 1]: first line-pixel of l
-    auto off = [&](uint32_t l) -> uint64_t { return l >= dims.n_lines ? P : (uint64_t)offs[l - base]; };
-    uint64_t p = b0 + threadIdx.x;
-    if (p >= b1) return;
-    // the line holding p: last line in [l0, l1] with offset <= p
-    uint32_t lo = l0, hi = l1 + 1;
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (off(mid) <= p) lo = mid; else hi = mid;
-    }
-    uint32_t li = lo;
-    uint64_t first = off(li), next = off(li + 1);
-    for (; p < b1; p += 256) {
-        while (p >= next) {  // zero-length lines share offsets
-            ++li;
-            first = next;
-            next = off(li + 1);
-        }
+    for (uint64_t p = b0 + threadIdx.x; p < b1; p += 256) {
+        const uint32_t li = l0 + owner[p - b0];
+        const uint64_t first = offs[li - base];
         const LineSetup L = lines[li - base];
         const SceneCull sc = Staged ? culls[li - base] : scene_cull(scenes[L.packed >> 19]);
         uint64_t w[2];
@@ -615,6 +614,8 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     __shared__ uint32_t s_off[kEmitLines + 1];
     __shared__ LineSetup s_lines[kEmitLines];
     __shared__ SceneCull s_culls[kEmitLines];
+    __shared__ uint32_t s_owner[kEmitTile];
+    __shared__ uint32_t s_wmax[4];
     extern __shared__ uint32_t s_cnt[];
     const uint64_t P = counters->line_pixels;
     const uint32_t nparts = (uint32_t)((P + kEmitTile - 1) / kEmitTile);
@@ -630,6 +631,34 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     const uint32_t nl = l1 - l0 + 1;
     uint4 *slots = reinterpret_cast<uint4 *>(out);
     uint32_t kept = 0;
+    // owner table: the first line-pixel of every line of nonzero length that
+    // starts inside the partition gets the line's index (relative to l0,
+    // which holds the partition's first line-pixel); a running maximum fills
+    // the positions in between. Lines after l0 start after b0.
+    for (uint32_t i = threadIdx.x; i < kEmitTile; i += 256) s_owner[i] = 0;
+    __syncthreads();
+    for (uint32_t k = threadIdx.x + 1; k < nl; k += 256) {
+        const uint64_t o = offsets[l0 + k];
+        const uint64_t e = l0 + k + 1 < dims.n_lines ? (uint64_t)offsets[l0 + k + 1] : P;
+        if (e > o && o < b1) s_owner[o - b0] = k;
+    }
+    __syncthreads();
+    {
+        constexpr uint32_t E = kEmitTile / 256;
+        uint32_t v[E], m = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < E; ++j) {
+            m = max(m, s_owner[threadIdx.x * E + j]);
+            v[j] = m;
+        }
+        const uint32_t wm = wave_incl_max(m);
+        uint32_t before = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)wm, 0x138, 0xF, 0xF, false);  // wave_shr:1
+        if (lane_id() == 63) s_wmax[threadIdx.x >> 6] = wm;
+        __syncthreads();
+        for (uint32_t k = 0; k < (threadIdx.x >> 6); ++k) before = max(before, s_wmax[k]);
+#pragma unroll
+        for (uint32_t j = 0; j < E; ++j) s_owner[threadIdx.x * E + j] = max(before, v[j]);
+    }
     if (nl <= kEmitLines) {
         for (uint32_t k = threadIdx.x; k <= nl; k += blockDim.x)
             s_off[k] = l0 + k < dims.n_lines ? offsets[l0 + k] : (uint32_t)P;
@@ -639,10 +668,10 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
             s_culls[k] = scene_cull(scenes[L.packed >> 19]);
         }
         __syncthreads();
-        emit_span<true>(s_lines, s_culls, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
+        emit_span<true>(s_lines, s_culls, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, s_owner, kept);
     } else {  // a partition of very short or zero-length lines
         __syncthreads();
-        emit_span<false>(lines, nullptr, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, kept);
+        emit_span<false>(lines, nullptr, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, s_owner, kept);
     }
     count_flush(s_cnt, dims, sc, part, nparts, kept);
 }
