@@ -953,12 +953,18 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
                 key = 0;
             }
             uint32_t d = (uint32_t)(key >> shift) & (RADIX - 1);
-            uint64_t peers = __ballot(valid);
+            // lanes with the same digit: for each bit, the lanes whose bit
+            // equals this lane's (m: the bit spread over 32 bits)
+            const uint64_t vb = __ballot(valid);
+            uint32_t pl = (uint32_t)vb, ph = (uint32_t)(vb >> 32);
 #pragma unroll
             for (int b = 0; b < R; ++b) {
-                uint64_t bits = __ballot((d >> b) & 1u);
-                peers &= ((d >> b) & 1u) ? bits : ~bits;
+                const uint32_t m = (uint32_t)((int32_t)(d << (31 - b)) >> 31);
+                const uint64_t bits = __ballot(m != 0u);
+                pl &= ~((uint32_t)bits ^ m);
+                ph &= ~((uint32_t)(bits >> 32) ^ m);
             }
+            const uint64_t peers = (uint64_t)ph << 32 | pl;
             uint32_t below = (uint32_t)__popcll(peers & lanemask_lt());
             uint32_t c = valid ? s_wcnt[w][d] : 0u;
             __builtin_amdgcn_wave_barrier();
