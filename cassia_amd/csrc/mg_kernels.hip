@@ -1907,6 +1907,53 @@ __device__ __forceinline__ void blend_terms(uint32_t mode, float c0, float c1, f
     }
 }
 
+// blend_terms for a source colour that is the same over the wave (a solid
+// layer): the parts of a formula that depend on the source alone are not
+// repeated per lane. Overlay's choice depends on the source, so only the
+// chosen side is computed; Soft Light's source terms (polynomial or square
+// root, s·(1−s), D(s)−s) are computed once per channel in three active lanes
+// and read back: the lanes blend in whole columns of 8, so the first active
+// lane and the two after it are active. The operations and their operands
+// are those of blend_lane, so the results are the same bits.
+__device__ __forceinline__ float overlay_uniform(float d, float s) {
+    if (s <= 0.5f) return (d * s) * 2.0f;
+    return 2.0f * ((s + d) - __builtin_fmaf(s, d, 0.5f));
+}
+__device__ __forceinline__ void blend_terms_uniform(uint32_t mode, float c0, float c1, float c2, float f0,
+                                                    float f1, float f2, float a, float &b0, float &b1,
+                                                    float &b2) {
+    if (mode == BM_OVERLAY) {
+        b0 = overlay_uniform(c0, f0) * a;
+        b1 = overlay_uniform(c1, f1) * a;
+        b2 = overlay_uniform(c2, f2) * a;
+        return;
+    }
+    if (mode == BM_SOFT_LIGHT) {
+        const uint32_t l0 = (uint32_t)__builtin_ctzll(__ballot(true));
+        const uint32_t own = lane_id() - l0;
+        const float sl = own == 0 ? f0 : (own == 1 ? f1 : f2);
+        const float poly = __builtin_fmaf(__builtin_fmaf(16.0f, sl, -12.0f), sl, 4.0f) * sl;
+        const float dd = sl <= 0.25f ? poly : __builtin_sqrtf(sl);
+        const float u = sl * (1.0f - sl), v = dd - sl;
+        const float fs[3] = {f0, f1, f2};
+        const float cs[3] = {c0, c1, c2};
+        float out[3];
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float uc = rlane_f(u, l0 + (uint32_t)ch), vc = rlane_f(v, l0 + (uint32_t)ch);
+            const float k = __builtin_fmaf(2.0f, cs[ch], -1.0f);
+            const float lo = __builtin_fmaf(uc, k, fs[ch]);
+            const float hi = __builtin_fmaf(vc, k, fs[ch]);
+            out[ch] = (cs[ch] <= 0.5f ? lo : hi) * a;
+        }
+        b0 = out[0];
+        b1 = out[1];
+        b2 = out[2];
+        return;
+    }
+    blend_terms(mode, c0, c1, c2, f0, f1, f2, a, b0, b1, b2);
+}
+
 // dst * (1 - alpha) + term (painter/mod.rs:364-400), as fused multiply-adds
 // that update the accumulators in place (the compiler's accumulate form
 // writes into the term's register and copies it back on every layer).
@@ -1973,8 +2020,8 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
             } else {
                 a = rlane_f(b.color.w, q) * cov;
                 if (apply_clip) a *= clip_mask;
-                blend_terms(mode, c0, c1, c2, rlane_f(b.color.x, q), rlane_f(b.color.y, q), rlane_f(b.color.z, q), a,
-                            t0, t1, t2);
+                blend_terms_uniform(mode, c0, c1, c2, rlane_f(b.color.x, q), rlane_f(b.color.y, q),
+                                    rlane_f(b.color.z, q), a, t0, t1, t2);
             }
             blend_apply(c0, c1, c2, ca, t0, t1, t2, a);
         }
