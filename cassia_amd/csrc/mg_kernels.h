@@ -23,6 +23,7 @@ struct PaintParams {
     uint32_t rec_chunk;          // id records a row reserves at a time
     unsigned char *walk_scratch; // row-walk arrays in global memory (k_row_walk<T, true>)
     uint64_t walk_stride;        // bytes per row of walk_scratch
+    uint32_t walk_run_bufs;      // sets of row-walk run arrays (1, or 2 to overlap columns)
     uint32_t *overflow;          // bit 0: LDS capacity, bit 1: record capacity
     // debug: coverage records (A values) and per-tile write ops
     int32_t *cov_a;
@@ -99,7 +100,7 @@ __global__ void k_run_cov(PaintParams pp, const FrameCounters *counters, const u
 template <uint32_t T, bool G>
 __global__ void k_row_walk(PaintParams pp);
 __global__ void k_paint(PaintParams pp);
-size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride);
+size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride, uint32_t run_bufs);
 size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 
 constexpr uint32_t kEmitTileHost = 2048;

@@ -113,8 +113,9 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x) {
 }
 
 // Block-wide exclusive scan of one u32 per thread over W waves (W <= 64):
-// wave scans, then one wave-scan of the wave totals read from LDS.
-template <uint32_t W>
+// wave scans, then one wave-scan of the wave totals read from LDS. Without
+// TrailingSync the caller must pass a barrier before wsum is written again.
+template <uint32_t W, bool TrailingSync = true>
 __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *wsum /* W */, uint32_t *total) {
     const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
     const uint32_t x = wave_incl_scan(v);
@@ -123,7 +124,7 @@ __device__ __forceinline__ uint32_t block_excl_scan_u32(uint32_t v, uint32_t *ws
     const uint32_t t = wave_incl_scan(lane < W ? wsum[lane] : 0u);
     const uint32_t base = w ? (uint32_t)__builtin_amdgcn_readlane((int)t, (int)w - 1) : 0u;
     *total = (uint32_t)__builtin_amdgcn_readlane((int)t, (int)W - 1);
-    __syncthreads();
+    if (TrailingSync) __syncthreads();
     return base + x - v;
 }
 
@@ -1091,8 +1092,8 @@ constexpr uint32_t kRunEvenOdd = 1u << 16;                    // run key: layer 
 constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flags
 constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
 
-size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride) {
-    return (size_t)cap * 24 + (size_t)rcap * 16 + (size_t)tstride * 12 + 64;
+size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride, uint32_t run_bufs) {
+    return (size_t)cap * 24 + (size_t)rcap * 16 * run_bufs + (size_t)tstride * 12 + 64;
 }
 size_t paint_lds_bytes_host(uint32_t cap, bool clips) {
     return (size_t)kStageWin * 8 + 2 * 64 * 4 + (clips ? (size_t)cap * 4 : 0);
@@ -1513,15 +1514,22 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     const uint32_t tid = threadIdx.x;
     const uint32_t cap = pp.cap_ids, rcap = pp.cap_runs, tstride = pp.tstride;
     unsigned char *p = G ? pp.walk_scratch + (uint64_t)blockIdx.x * pp.walk_stride : lds_raw;
-    uint64_t *qc[2];
+    // two queues (the next column's is written while the current one is
+    // read) and one or two sets of run arrays (pp.walk_run_bufs; with two,
+    // the next column's runs are stored while the current ones are read)
+    const uint32_t nb = pp.walk_run_bufs;
+    uint64_t *qc[2], *rcs[2];
     qc[0] = (uint64_t *)p; p += (size_t)cap * 8;
     qc[1] = (uint64_t *)p; p += (size_t)cap * 8;
-    uint64_t *rc = (uint64_t *)p; p += (size_t)rcap * 8;
-    uint32_t *qk[2];
+    rcs[0] = (uint64_t *)p; p += (size_t)rcap * 8;
+    rcs[1] = nb == 2 ? (uint64_t *)p : rcs[0]; p += (size_t)rcap * 8 * (nb - 1);
+    uint32_t *qk[2], *rks[2], *rbs[2];
     qk[0] = (uint32_t *)p; p += (size_t)cap * 4;
     qk[1] = (uint32_t *)p; p += (size_t)cap * 4;
-    uint32_t *rk = (uint32_t *)p; p += (size_t)rcap * 4;
-    uint32_t *rb = (uint32_t *)p; p += (size_t)rcap * 4;
+    rks[0] = (uint32_t *)p; p += (size_t)rcap * 4;
+    rks[1] = nb == 2 ? (uint32_t *)p : rks[0]; p += (size_t)rcap * 4 * (nb - 1);
+    rbs[0] = (uint32_t *)p; p += (size_t)rcap * 4;
+    rbs[1] = nb == 2 ? (uint32_t *)p : rbs[0]; p += (size_t)rcap * 4 * (nb - 1);
     uint32_t *ts_l = (uint32_t *)p; p += (size_t)tstride * 4;
     uint32_t *nr_l = (uint32_t *)p; p += (size_t)tstride * 4;
     uint32_t *r0_l = (uint32_t *)p;
@@ -1551,7 +1559,10 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     };
     // Runs are prefetched into registers two columns ahead (one run per
     // thread, two register sets taking turns), so their loads overlap a
-    // whole column's merge; runs beyond T are loaded when used.
+    // whole column's merge; runs beyond T are loaded when stored. A column's
+    // runs are stored at the end of the column before it: into the other set
+    // of run arrays without waiting (one barrier per column), or, with one
+    // set, after a barrier.
     struct Prefetch {
         uint32_t key, beg, n;
         uint64_t cov;
@@ -1568,29 +1579,41 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
             f.cov = pp.run_cov[r02 + tid];
         }
     };
-    // One column: the cover queue entering tile colp - 1 meets its runs.
-    // Returns false when the work group must stop (capacity exceeded).
-    auto step = [&](uint32_t colp, Prefetch &f, uint32_t col_ahead) -> bool {
+    // The runs of column col into run arrays `buf`; false when they exceed
+    // the arrays (the work group must stop).
+    auto store = [&](const Prefetch &f, uint32_t col, uint32_t buf) -> bool {
+        if (col > h1) return true;
+        const uint32_t nr = nr_l[col];
+        if (nr > rcap) {
+            if (tid == 0) atomicOr(pp.overflow, 1u);
+            return false;
+        }
+        if (tid < f.n) {
+            rks[buf][tid] = f.key;
+            rbs[buf][tid] = f.beg;
+            rcs[buf][tid] = f.cov;
+        }
+        const uint32_t r0 = r0_l[col];
+        for (uint32_t k = f.n + tid; k < nr; k += T) {
+            rks[buf][k] = pp.run_key[r0 + k];
+            rbs[buf][k] = pp.run_beg[r0 + k];
+            rcs[buf][k] = pp.run_cov[r0 + k];
+        }
+        return true;
+    };
+    // One column: the cover queue entering tile colp - 1 meets its runs (in
+    // run arrays `buf`). Returns false when the work group must stop
+    // (capacity exceeded). Ends without a barrier.
+    auto step = [&](uint32_t colp, uint32_t buf) -> bool {
         const bool fold = colp <= h0;
         const uint32_t nr = nr_l[colp];
-        if (nr > rcap || nq + nr > cap) {
+        if (nq + nr > cap) {
             if (tid == 0) atomicOr(pp.overflow, 1u);
             return false;
         }
         const uint32_t ts = ts_l[colp], te = ts_l[colp + 1];
-        if (tid < f.n) {
-            rk[tid] = f.key;
-            rb[tid] = f.beg;
-            rc[tid] = f.cov;
-        }
-        const uint32_t r0 = r0_l[colp];
-        for (uint32_t k = f.n + tid; k < nr; k += T) {
-            rk[k] = pp.run_key[r0 + k];
-            rb[k] = pp.run_beg[r0 + k];
-            rc[k] = pp.run_cov[r0 + k];
-        }
-        issue(f, col_ahead);
-        __syncthreads();
+        const uint32_t *rk = rks[buf], *rb = rbs[buf];
+        const uint64_t *rc = rcs[buf];
         const uint32_t *qkc = qk[cur];
         const uint64_t *qcc = qc[cur];
         const uint32_t total = nq + nr;
@@ -1658,7 +1681,8 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
             ex = block_excl_scan<T / 64>((uint64_t)cu | ((uint64_t)cq << 32), s_wsum, &all);
         } else {
             uint32_t all32;
-            const uint32_t ex32 = block_excl_scan_u32<T / 64>(cu | (cq << 16), (uint32_t *)s_wsum, &all32);
+            // (no trailing barrier: the column ends with one)
+            const uint32_t ex32 = block_excl_scan_u32<T / 64, false>(cu | (cq << 16), (uint32_t *)s_wsum, &all32);
             ex = (uint64_t)(ex32 & 0xFFFFu) | ((uint64_t)(ex32 >> 16) << 32);
             all = (uint64_t)(all32 & 0xFFFFu) | ((uint64_t)(all32 >> 16) << 32);
         }
@@ -1727,7 +1751,6 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
             pp.rec_hdr[tile] = make_uint2(rec_off, n_union);
             row_records += n_union;
         }
-        __syncthreads();
         nq = nq_new;
         cur = nxt;
         return true;
@@ -1736,14 +1759,31 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     Prefetch A, B;
     issue(A, c0);
     issue(B, c1);
+    if (!store(A, c0, 0)) return;
+    issue(A, c2);
+    __syncthreads();
+    uint32_t buf = 0;
     while (c0 <= h1) {
-        if (!step(c0, A, c2)) return;
+        // column c0 in arrays buf; B holds c1, A holds c2
+        if (!step(c0, buf)) return;
         const uint32_t c3 = next_colp(c2);
+        if (nb == 1) __syncthreads();
+        if (!store(B, c1, buf ^ (nb - 1))) return;
+        issue(B, c3);
+        __syncthreads();
+        buf ^= nb - 1;
         if (c1 > h1) break;
-        if (!step(c1, B, c3)) return;
+        // column c1 in arrays buf; A holds c2, B holds c3
+        if (!step(c1, buf)) return;
+        const uint32_t c4 = next_colp(c3);
+        if (nb == 1) __syncthreads();
+        if (!store(A, c2, buf ^ (nb - 1))) return;
+        issue(A, c4);
+        __syncthreads();
+        buf ^= nb - 1;
         c0 = c2;
         c1 = c3;
-        c2 = next_colp(c3);
+        c2 = c4;
     }
     if (tid == 0 && row_records) atomicAdd(pp.n_records, row_records);
 }
