@@ -701,6 +701,7 @@ static void fill_stats(mg_frame_stats &st, const mg::FrameResult &r, size_t line
     st.runs = r.runs;
     st.records = r.records;
     st.walk_threads = r.walk_threads;
+    st.deep_tiles = r.deep_tiles;
 }
 
 // Scene descriptor of one frame of `width` x `rows_px` pixels at `stride`

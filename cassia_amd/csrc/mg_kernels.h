@@ -24,6 +24,11 @@ struct PaintParams {
     unsigned char *walk_scratch; // row-walk arrays in global memory (k_row_walk<T, true>)
     uint64_t walk_stride;        // bytes per row of walk_scratch
     uint32_t walk_run_bufs;      // sets of row-walk run arrays (1, or 2 to overlap columns)
+    // tiles of deep_min records or more (0: none) painted by k_paint_deep,
+    // listed by the walk: (global row, tile column), at most deep_cap
+    uint32_t deep_min, deep_cap;
+    uint32_t *deep_count;
+    uint2 *deep_list;
     uint32_t *overflow;          // bit 0: LDS capacity, bit 1: record capacity
     // debug: coverage records (A values) and per-tile write ops
     int32_t *cov_a;
@@ -100,6 +105,8 @@ __global__ void k_run_cov(PaintParams pp, const FrameCounters *counters, const u
 template <uint32_t T, bool G>
 __global__ void k_row_walk(PaintParams pp);
 __global__ void k_paint(PaintParams pp);
+__global__ void k_paint_deep(PaintParams pp);
+size_t paint_deep_lds_bytes_host();
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride, uint32_t run_bufs);
 size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 

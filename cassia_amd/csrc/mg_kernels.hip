@@ -1091,6 +1091,9 @@ constexpr uint32_t kMaskBit = 1u << 30, kSkipClipBit = 1u << 31;
 constexpr uint32_t kRunEvenOdd = 1u << 16;                    // run key: layer | even_odd << 16
 constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flags
 constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
+constexpr uint32_t kPaintWaveLds = kStageWin * 8 + 2 * 64 * 4;  // stage window + cells of one wave
+constexpr uint32_t kDeepFlag = 1u << 31;                      // record header: tile of the deep list
+constexpr uint32_t kDeepChunk = 192;                          // deep paint: records per round
 
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride, uint32_t run_bufs) {
     return (size_t)cap * 24 + (size_t)rcap * 16 * run_bufs + (size_t)tstride * 12 + 64;
@@ -1748,7 +1751,15 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
         }
         if (write_rec && tid == 0) {
             const uint64_t tile = (uint64_t)S.tile_base + (uint64_t)j * S.row_len + (colp - 1);
-            pp.rec_hdr[tile] = make_uint2(rec_off, n_union);
+            uint32_t cnt = n_union;
+            if (pp.deep_min && n_union >= pp.deep_min && !(S.flags & SF_CLIPS)) {
+                const uint32_t di = atomicAdd(pp.deep_count, 1u);
+                if (di < pp.deep_cap) {
+                    pp.deep_list[di] = make_uint2(grow, colp - 1);
+                    cnt |= kDeepFlag;
+                }
+            }
+            pp.rec_hdr[tile] = make_uint2(rec_off, cnt);
             row_records += n_union;
         }
         nq = nq_new;
@@ -1800,11 +1811,21 @@ struct Stage {
     uint32_t base, end;
 };
 
+// LDS hand-off between the lanes of one wave: the wave's LDS operations
+// complete in issue order, so ordering the compiler (and waiting for the
+// wave's own outstanding LDS operations) is all it takes. Waves of a deep
+// tile stage independently, so no work-group barrier may be used here.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __device__ __forceinline__ void stage_load(Stage &st, const uint64_t *segs, uint32_t from, uint32_t to) {
     const uint32_t lane = lane_id();
-    __syncthreads();
+    wave_lds_sync();
     for (uint32_t k = lane; k < to - from; k += 64) st.w[k] = segs[from + k];
-    __syncthreads();
+    wave_lds_sync();
     // wave-uniform: kept in scalar registers
     st.base = (uint32_t)__builtin_amdgcn_readfirstlane((int)from);
     st.end = (uint32_t)__builtin_amdgcn_readfirstlane((int)to);
@@ -1860,6 +1881,9 @@ struct Batch {
     uint32_t lim;    // end of the batch's last run (lim == first: no run)
 };
 
+// Without Window the batch is simply the next 64 records (for a reader that
+// does not stage their runs).
+template <bool Window = true>
 __device__ __forceinline__ void load_batch(const PaintParams &pp, const PropsEntry *props, const uint32_t *fl,
                                            const uint4 *rec, const uint32_t *rlen, uint32_t n, uint32_t k0,
                                            Batch &b) {
@@ -1884,7 +1908,7 @@ __device__ __forceinline__ void load_batch(const PaintParams &pp, const PropsEnt
     const uint64_t rm = __ballot(run);
     b.cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)avail);
     b.first = b.lim = 0;
-    if (rm) {
+    if (Window && rm) {
         b.first = rlane(b.beg, (uint32_t)__ffsll((long long)rm) - 1u);
         // the first run ending beyond the window closes the batch (a run
         // longer than the window makes a batch of its own)
@@ -2012,26 +2036,27 @@ __device__ __forceinline__ void blend_apply(float &c0, float &c1, float &c2, flo
 // The painter over records [skipped, n) of a tile (painter/mod.rs:260-400):
 // coverage, clip masks (only in scenes with clips), column skipping and
 // blending into this lane's pixel colour.
-template <bool Clips>
+// `area(b, k0, q)` gives the accumulated area A of record k0 + q for this
+// lane's pixel; with Staged the batch's runs are staged for it first.
+template <bool Clips, bool Staged, class AreaFn>
 __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsEntry *props, const uint32_t *fl,
                                              const uint4 *rec, const uint32_t *rlen, uint32_t n, uint32_t skipped,
-                                             Stage &st, int32_t *cells, const uint64_t *segs, uint32_t px,
-                                             uint32_t py, uint32_t c, uint32_t j, float &c0, float &c1, float &c2,
-                                             float &ca) {
+                                             Stage &st, const uint64_t *segs, uint32_t px, uint32_t py, uint32_t c,
+                                             uint32_t j, float &c0, float &c1, float &c2, float &ca, AreaFn &&area) {
     bool has_clip = false;
     uint32_t clip_last = 0;
     float clip_mask = 0.0f;
     for (uint32_t k0 = skipped, cnt = 0; k0 < n; k0 += cnt) {
         Batch b;
-        load_batch(pp, props, fl, rec, rlen, n, k0, b);
-        stage_batch(st, segs, b);
+        load_batch<Staged>(pp, props, fl, rec, rlen, n, k0, b);
+        if (Staged) stage_batch(st, segs, b);
         cnt = b.cnt;
         for (uint32_t q = 0; q < cnt; ++q) {
             const uint32_t f = rlane(b.flags, q);
             if (Clips && !(f & kMaskBit)) continue;
             const uint32_t lay = rlane(b.key, q) & 0xFFFFu;
             if (Clips && has_clip && clip_last < lay) has_clip = false;
-            const int32_t A = record_area(st, cells, segs, b, q, py);
+            const int32_t A = area(b, k0, q);
             const float cov = coverage_from_area(A, f & PF_EVENODD);
             if (Clips && (f & PF_CLIP)) {
                 // clip_at (painter/mod.rs:402-416)
@@ -2068,15 +2093,65 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
     }
 }
 
-// One wave paints one tile (LayerWorkbench::drive_tile_painting,
-// layer_workbench.rs:430-486, and Painter, painter/mod.rs:260-493).
-__global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
-    extern __shared__ __align__(16) unsigned char lds_raw[];
-    const uint32_t lane = lane_id();
-    // the row walk stopped early: its records are incomplete
-    if (__hip_atomic_load(pp.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
-    const uint2 wk = pp.work[blockIdx.x];
-    const uint32_t grow = wk.x, c = wk.y;
+// Deep tiles (pp.deep_min records or more, no clips, no coverage dump) are
+// painted by k_paint_deep, four waves per tile: waves 1..3 compute the
+// coverage of the next 192 records while wave 0 blends the current 192. The
+// row walk lists them (kDeepFlag in the header's count) up to pp.deep_cap.
+
+// Waves 1..3 store the area A of their 64 records of a round for every pixel,
+// reduced so that coverage_from_area gives the same coverage (nonzero: A
+// clamped to [-256, 256]; even-odd: A mod 512); wave 0 blends from them.
+__device__ __forceinline__ void paint_deep(const PaintParams &pp, const PropsEntry *props, const uint4 *rec,
+                                           const uint32_t *rlen, uint32_t n, uint32_t skipped, Stage &st,
+                                           int32_t *cells, const uint64_t *segs, uint32_t px, uint32_t py,
+                                           uint32_t c, uint32_t j, float &c0, float &c1, float &c2, float &ca,
+                                           int16_t *abuf) {
+    const uint32_t wv = threadIdx.x >> 6, lane = lane_id();
+    const uint32_t nchunks = (n - skipped + kDeepChunk - 1) / kDeepChunk;
+    auto produce = [&](uint32_t chunk) {
+        int16_t *A = abuf + (size_t)(chunk & 1u) * kDeepChunk * 64;
+        const uint32_t base = skipped + chunk * kDeepChunk;
+        const uint32_t r0 = base + (wv - 1) * 64;
+        const uint32_t r1 = r0 + 64 < n ? r0 + 64 : n;
+        for (uint32_t k0 = r0, cnt = 0; k0 < r1; k0 += cnt) {
+            Batch b;
+            load_batch(pp, props, nullptr, rec, rlen, r1, k0, b);
+            stage_batch(st, segs, b);
+            cnt = b.cnt;
+            for (uint32_t q = 0; q < cnt; ++q) {
+                const int32_t a = record_area(st, cells, segs, b, q, py);
+                const bool eo = rlane(b.flags, q) & PF_EVENODD;
+                const int32_t red = eo ? (a & 511) : (a < -256 ? -256 : (a > 256 ? 256 : a));
+                A[(k0 + q - base) * 64 + lane] = (int16_t)red;
+            }
+        }
+    };
+    if (wv > 0 && nchunks > 0) produce(0);
+    __syncthreads();
+    for (uint32_t ch = 0; ch < nchunks; ++ch) {
+        if (wv > 0) {
+            if (ch + 1 < nchunks) produce(ch + 1);
+        } else {
+            const int16_t *A = abuf + (size_t)(ch & 1u) * kDeepChunk * 64;
+            const uint32_t base = skipped + ch * kDeepChunk;
+            const uint32_t end = base + kDeepChunk < n ? base + kDeepChunk : n;
+            paint_layers<false, false>(pp, props, nullptr, rec, rlen, end, base, st, segs, px, py, c, j, c0, c1, c2,
+                                       ca, [&](const Batch &, uint32_t k0, uint32_t q) {
+                                           return (int32_t)A[(k0 + q - base) * 64 + lane];
+                                       });
+        }
+        __syncthreads();
+    }
+}
+
+// One tile (LayerWorkbench::drive_tile_painting, layer_workbench.rs:430-486,
+// and Painter, painter/mod.rs:260-493): one wave, or (Deep) four waves of
+// which wave 0 holds the pixels' colours. In Deep every wave runs the
+// workbench passes; only thread 0 / wave 0 write.
+template <bool Deep>
+__device__ __forceinline__ void paint_tile(const PaintParams &pp, uint32_t grow, uint32_t c,
+                                           unsigned char *lds_raw) {
+    const uint32_t lane = lane_id(), wv = Deep ? threadIdx.x >> 6 : 0u;
     const uint32_t sc = pp.row_scene[grow];
     const SceneDesc S = pp.scenes[sc];
     const uint32_t j = grow - S.row_base;
@@ -2085,13 +2160,15 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     const bool clips = S.flags & SF_CLIPS;
     const uint64_t tile = (uint64_t)S.tile_base + (uint64_t)j * S.row_len + c;
     const uint2 hdr = pp.rec_hdr[tile];
-    const uint32_t n = hdr.y;
+    // a tile listed for the deep kernel is painted there
+    if (!Deep && (hdr.y & kDeepFlag)) return;
+    const uint32_t n = hdr.y & ~kDeepFlag;
     const uint4 *rec = pp.rec + hdr.x;
     const uint32_t *rlen = pp.rec_len + hdr.x;
     Stage st;
-    st.w = (uint64_t *)lds_raw;
+    st.w = (uint64_t *)(lds_raw + (size_t)wv * kPaintWaveLds);
     st.base = st.end = 0;
-    int32_t *cells = (int32_t *)(lds_raw + kStageWin * 8);
+    int32_t *cells = (int32_t *)(lds_raw + (size_t)wv * kPaintWaveLds + kStageWin * 8);
     uint32_t *fl = clips ? (uint32_t *)(lds_raw + kStageWin * 8 + 2 * 64 * 4) : nullptr;
     const uint64_t *segs = pp.segs;
     if (fl) {
@@ -2151,7 +2228,7 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
         const int32_t old = *prev;
         const int32_t now = (int32_t)(uint16_t)n;
         __syncthreads();
-        if (lane == 0) *prev = now;
+        if (threadIdx.x == 0) *prev = now;
         if (old >= 0 && old == now && !changed) op = 0;
     }
     if (op != 0 && clips) {
@@ -2259,24 +2336,50 @@ __global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
     uint32_t pix = 0;
     if (op == 2) {
         float c0 = S.clear[0], c1 = S.clear[1], c2 = S.clear[2], ca = S.clear[3];
-        if (clips)
-            paint_layers<true>(pp, props, fl, rec, rlen, n, skipped, st, cells, segs, px, py, c, j, c0, c1, c2, ca);
+        auto area = [&](const Batch &b, uint32_t, uint32_t q) { return record_area(st, cells, segs, b, q, py); };
+        if (Deep)
+            paint_deep(pp, props, rec, rlen, n, skipped, st, cells, segs, px, py, c, j, c0, c1, c2, ca,
+                       (int16_t *)(lds_raw + 4 * kPaintWaveLds));
+        else if (clips)
+            paint_layers<true, true>(pp, props, fl, rec, rlen, n, skipped, st, segs, px, py, c, j, c0, c1, c2, ca,
+                                     area);
         else
-            paint_layers<false>(pp, props, nullptr, rec, rlen, n, skipped, st, cells, segs, px, py, c, j, c0, c1,
-                                c2, ca);
+            paint_layers<false, true>(pp, props, nullptr, rec, rlen, n, skipped, st, segs, px, py, c, j, c0, c1, c2,
+                                      ca, area);
         pix = pixel_bytes(c0, c1, c2, ca);
     } else if (op == 1) {
         const float sc4[4] = {solid0, solid1, solid2, solid3};
         pix = solid_bytes(sc4);
     }
-    if (op != 0) {
+    if (op != 0 && wv == 0) {
         const uint32_t gx = c * 8 + px, gy = j * 8 + py;
         if (gx < S.width && gy < S.rows_px) {
             uint32_t *o = (uint32_t *)(pp.out + S.out_offset) + (uint64_t)gy * S.stride + gx;
             *o = pix;
         }
     }
-    if (pp.tile_ops && lane == 0) pp.tile_ops[tile] = (int8_t)op;
+    if (pp.tile_ops && threadIdx.x == 0) pp.tile_ops[tile] = (int8_t)op;
 }
+
+// One wave per tile of the work list.
+__global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    // the row walk stopped early: its records are incomplete
+    if (__hip_atomic_load(pp.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const uint2 wk = pp.work[blockIdx.x];
+    paint_tile<false>(pp, wk.x, wk.y, lds_raw);
+}
+
+// Four waves per deep tile: work group t paints entry t of the deep list.
+__global__ __launch_bounds__(256) void k_paint_deep(PaintParams pp) {
+    extern __shared__ __align__(16) unsigned char lds_raw[];
+    if (__hip_atomic_load(pp.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+    const uint32_t count = *pp.deep_count;
+    if (blockIdx.x >= count || blockIdx.x >= pp.deep_cap) return;
+    const uint2 wk = pp.deep_list[blockIdx.x];
+    paint_tile<true>(pp, wk.x, wk.y, lds_raw);
+}
+
+size_t paint_deep_lds_bytes_host() { return (size_t)4 * kPaintWaveLds + (size_t)2 * kDeepChunk * 64 * 2; }
 
 }  // namespace mg

@@ -76,6 +76,7 @@ struct FrameSpec {
 struct FrameResult {
     uint64_t line_pixels = 0, segments = 0, kept = 0;
     uint32_t passes = 0, radix_bits = 0, sort_bits = 0, walk_threads = 0;
+    uint32_t deep_tiles = 0;  // tiles painted by the four-wave deep painter
     uint64_t tiles = 0;
     double device_ms = 0.0;
     // setup+scan, emit, histogram+bases, sort passes, tile runs, row walk, paint
@@ -169,10 +170,12 @@ private:
     DevBuf cnt0_, cnt_acc_, cnt_status_;
     DevBuf flat_cmds_, flat_paths_, flat_counts_, flat_offsets_, flat_status_;
     DevBuf work_, runs_work_, tile_nr_, runs_status_, walk_scratch_, tile_start_, tile_run0_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
+    DevBuf deep_list_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
     uint32_t n_work_ = 0, n_runs_work_ = 0;
     uint64_t n_records_ = 0, n_runs_ = 0;
+    uint32_t n_deep_ = 0;
     const uint64_t *last_sorted_ = nullptr;
     uint64_t capture_P_ = 0;  // nonzero while a frame is enqueued for capture: its line-pixel capacity
     hipGraph_t graph_ = nullptr;

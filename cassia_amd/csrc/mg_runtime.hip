@@ -65,7 +65,7 @@ Renderer::~Renderer() {
                      &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
                      &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
                      &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_, &walk_scratch_,
-                     &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_};
+                     &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_, &deep_list_};
     for (DevBuf *b : all) b->release();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
@@ -95,7 +95,8 @@ bool Renderer::init(std::string &err) {
     // once to the device maximum (the attribute is per kernel, shared by
     // every engine of the process)
     for (const void *fn : {(const void *)k_row_walk<64, false>, (const void *)k_row_walk<256, false>,
-                           (const void *)k_row_walk<1024, false>, (const void *)k_paint}) {
+                           (const void *)k_row_walk<1024, false>, (const void *)k_paint,
+                           (const void *)k_paint_deep}) {
         hipFuncAttributes a;
         MG_HIP(hipFuncGetAttributes(&a, fn), "kernel attributes");
         MG_HIP(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)(lds_max_ - a.sharedSizeBytes)),
@@ -587,6 +588,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     res.device_ms = ms;
     res.runs = n_runs_;
     res.records = n_records_;
+    res.deep_tiles = n_deep_;
     return true;
 }
 
@@ -646,6 +648,18 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.n_runs = overflow_.as<uint32_t>() + 3;
     pp.n_records = overflow_.as<uint32_t>() + 2;
     fill32(overflow_.as<uint32_t>() + 3, 0, 1, stream_);
+    // Tiles of 8,192 records or more go to the four-wave deep painter (at most
+    // two per CU; more stay with the one-wave painter). A tile holds one wave
+    // of pixels, so a frame of few deep tiles would otherwise leave the GPU
+    // waiting on the serial record loops of a few waves.
+    pp.deep_min = spec.want_coverage ? 0u : 8192u;
+    pp.deep_cap = 2 * cus_;
+    pp.deep_count = overflow_.as<uint32_t>() + 4;
+    if (!deep_list_.reserve((size_t)pp.deep_cap * 8)) {
+        err = "out of device memory (deep tile list)";
+        return false;
+    }
+    pp.deep_list = deep_list_.as<uint2>();
     // Tile runs: one wave per tile column while tiles are many and of
     // moderate depth; over the whole sorted stream (work follows the words)
     // when a frame's tiles are deep: more than 8,192 word slots per column,
@@ -737,6 +751,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.rec_cap = rec_cap_;
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
+        fill32(overflow_.as<uint32_t>() + 4, 0, 1, stream_);
         MG_HIP(mark(evs_[5]), "event");
         const size_t lds = global_walk ? 0 : lds_w;
         if (walk_threads == 64)
@@ -750,15 +765,17 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
                         : k_row_walk<1024, false><<<global_rows, 1024, lds, stream_>>>(pp);
         MG_HIP(mark(evs_[6]), "event");
         k_paint<<<n_work, 64, lds_p, stream_>>>(pp);
+        if (pp.deep_min) k_paint_deep<<<pp.deep_cap, 256, paint_deep_lds_bytes_host(), stream_>>>(pp);
         MG_HIP(mark(evs_[7]), "event");
         MG_HIP(hipGetLastError(), "paint launch");
         if (capture_P_) return true;  // one attempt; the replay reads the flags
-        uint32_t ov[4] = {0, 0, 0, 0};
-        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 16, hipMemcpyDeviceToHost, stream_), "read overflow");
+        uint32_t ov[5] = {0, 0, 0, 0, 0};
+        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 20, hipMemcpyDeviceToHost, stream_), "read overflow");
         MG_HIP(hipStreamSynchronize(stream_), "paint");
         if (!ov[0]) {
             n_records_ = ov[2];
             n_runs_ = ov[3];
+            n_deep_ = std::min(ov[4], pp.deep_cap);
             return true;
         }
         if (ov[0] & 1u) {
@@ -834,7 +851,7 @@ std::vector<const void *> Renderer::frame_buffers() const {
                            &bases_, &sort_status_, &part_counters_, &row_start_, &overflow_, &part_line_,
                            &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_,
                            &cnt_status_, &work_, &runs_work_, &tile_nr_, &runs_status_, &tile_start_, &tile_run0_, &run_key_, &run_beg_,
-                           &run_cov_, &rec_, &rec_len_, &rec_hdr_};
+                           &run_cov_, &rec_, &rec_len_, &rec_hdr_, &deep_list_};
     std::vector<const void *> v;
     for (const DevBuf *b : all) v.push_back(b->ptr);
     return v;

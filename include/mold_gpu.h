@@ -292,7 +292,7 @@ typedef struct mg_frame_stats {
     double paint_ms;           /* tile paint: workbench + painter        */
     uint64_t runs, records;    /* tile layer runs, tile id records       */
     uint32_t walk_threads;     /* row-walk work-group size used          */
-    uint32_t _pad;
+    uint32_t deep_tiles;       /* tiles painted four waves per tile      */
 } mg_frame_stats;
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
 

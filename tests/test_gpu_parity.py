@@ -110,9 +110,13 @@ def test_tiles_deeper_than_lds(gpu, cpu, n_layers):
     outs = []
     for m in (cpu, gpu):
         buf = np.full((w * h, 4), 7, np.uint8)
-        deep_scene(m, n_layers, w, h, 77).render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
+        comp = deep_scene(m, n_layers, w, h, 77)
+        comp.render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
         outs.append(buf)
     assert_same_image(outs[0], outs[1], w)
+    # 9000 records per tile: the four-wave deep painter (8,192 or more)
+    deep = comp.stats().deep_tiles
+    assert deep > 0 if n_layers >= 8192 else deep == 0
 
 
 def test_pixel_segments_canonical_stream(gpu, cpu):
