@@ -365,7 +365,9 @@ def measured_traffic(workload):
 
 def copy_calibration(nbytes):
     """One fill and one device-to-device copy of `nbytes`: the reference
-    kernels whose FETCH_SIZE / WRITE_SIZE counters scale the renderer's."""
+    kernels whose FETCH_SIZE / WRITE_SIZE counters scale the renderer's (the
+    first copy dispatch). Then 10 timed copies: the device-copy bandwidth
+    (read + write bytes per second) the HBM-bound kernels compare with."""
     import torch
     a = torch.empty(nbytes, dtype=torch.uint8, device="cuda")
     b = torch.empty_like(a)
@@ -373,7 +375,15 @@ def copy_calibration(nbytes):
     torch.cuda.synchronize()
     b.copy_(a)
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes": nbytes, "fill_bytes": nbytes}))
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(10):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / 10
+    print(json.dumps({"copy_bytes": nbytes, "fill_bytes": nbytes, "copy_ms": round(ms, 4),
+                      "copy_gbs": round(2 * nbytes / ms / 1e6, 1)}))
 
 
 def main():
