@@ -560,6 +560,41 @@ __device__ __forceinline__ void count_slot(uint32_t *s_cnt, uint64_t v, const Fr
     }
 }
 
+// count_slot for the two slots of every active lane's line-pixel at once:
+// they mostly lie in one tile, so each grouping step takes the lanes of
+// either slot holding the step's digit.
+__device__ __forceinline__ void count_slot_pair(uint32_t *s_cnt, uint64_t v0, uint64_t v1, const FrameDims &d,
+                                                uint32_t &kept) {
+    const uint32_t radix = 1u << d.radix_bits, nd = d.passes * radix;
+    const uint32_t lane = lane_id();
+    const uint64_t exec = __ballot(true);
+    const uint32_t leader0 = (uint32_t)__ffsll((long long)exec) - 1u;
+    const uint32_t nc = (uint32_t)(__popcll(__ballot(v0 == kSlotCull)) + __popcll(__ballot(v1 == kSlotCull)));
+    if (nc && lane == leader0) atomicAdd(&s_cnt[nd + 1], nc);
+    const bool ok0 = !(v0 >> 63), ok1 = !(v1 >> 63);
+    kept += (ok0 ? 1u : 0u) + (ok1 ? 1u : 0u);
+    uint32_t shift = d.sort_shift;
+    if (ok0) atomicAdd(&s_cnt[(uint32_t)(v0 >> shift) & (radix - 1)], 1u);
+    if (ok1) atomicAdd(&s_cnt[(uint32_t)(v1 >> shift) & (radix - 1)], 1u);
+    shift += d.radix_bits;
+    const uint64_t oks0 = __ballot(ok0), oks1 = __ballot(ok1);
+    for (uint32_t ps = 1; ps < d.passes; ++ps, shift += d.radix_bits) {
+        const uint32_t dg0 = ok0 ? (uint32_t)(v0 >> shift) & (radix - 1) : 0xFFFFFFFFu;
+        const uint32_t dg1 = ok1 ? (uint32_t)(v1 >> shift) & (radix - 1) : 0xFFFFFFFFu;
+        uint64_t todo0 = oks0, todo1 = oks1;
+        while (todo0 | todo1) {
+            const uint32_t dl = todo0 ? (uint32_t)__builtin_amdgcn_readlane(
+                                            (int)dg0, (int)((uint32_t)__ffsll((long long)todo0) - 1u))
+                                      : (uint32_t)__builtin_amdgcn_readlane(
+                                            (int)dg1, (int)((uint32_t)__ffsll((long long)todo1) - 1u));
+            const uint64_t s0 = __ballot(dg0 == dl), s1 = __ballot(dg1 == dl);
+            if (lane == leader0) atomicAdd(&s_cnt[ps * radix + dl], (uint32_t)(__popcll(s0) + __popcll(s1)));
+            todo0 &= ~s0;
+            todo1 &= ~s1;
+        }
+    }
+}
+
 // Called by the whole work group after its last count_slot.
 __device__ __forceinline__ void count_flush(uint32_t *s_cnt, const FrameDims &d, const SlotCounts &sc,
                                             uint32_t part, uint32_t nparts, uint32_t kept) {
@@ -601,8 +636,7 @@ __device__ __forceinline__ void emit_span(const LineSetup *lines, const SceneCul
         }
         if (n < 2) v[1] = kSlotNone;
         slots[p] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
-        count_slot(s_cnt, v[0], dims, kept);
-        count_slot(s_cnt, v[1], dims, kept);
+        count_slot_pair(s_cnt, v[0], v[1], dims, kept);
     }
 }
 
