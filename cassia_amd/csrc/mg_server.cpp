@@ -3,7 +3,10 @@
 // one per engine (its own HIP stream and scratch), so that a batch admitted
 // while another is on the GPU runs beside it. Continuous batching: a worker
 // whose engine is free takes every queued request that fits the budget left,
-// up to max_batch, as its next batch.
+// up to max_batch, as its next batch. A heavy request (estimated above
+// kHeavyLinePixels) is a batch of its own and ends the batch before it, so
+// light requests do not wait inside a heavy batch; the next free engine
+// takes the light requests queued behind it.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -29,6 +32,10 @@ using Clock = std::chrono::steady_clock;
 // Estimated line-pixels one batch may take: half of a frame's limit (2^29),
 // so that estimates below the actual count still fit.
 constexpr uint64_t kBatchLinePixels = 1ull << 28;
+// A request above this many estimated line-pixels renders alone (a 1,024 x
+// 1,024 scene of 170,000 polygon lines is about 7 M; a 64 x 64 scene of a
+// million lines of large polygons, 43 M and 9 ms).
+constexpr uint64_t kHeavyLinePixels = 1ull << 24;
 
 double ms_between(Clock::time_point a, Clock::time_point b) {
     return std::chrono::duration<double, std::milli>(b - a).count();
@@ -93,10 +100,13 @@ struct mg_server {
                     const auto &r = pending.front();
                     if (inflight_bytes + bytes + r->footprint > cfg.hbm_budget) break;
                     if (!take.empty() && lp + r->line_pixels > kBatchLinePixels) break;
+                    const bool heavy = r->line_pixels > kHeavyLinePixels;
+                    if (!take.empty() && heavy) break;
                     bytes += r->footprint;
                     lp += r->line_pixels;
                     take.push_back(r);
                     pending.pop_front();
+                    if (heavy) break;
                 }
                 inflight_bytes += bytes;
                 ++busy_engines;
