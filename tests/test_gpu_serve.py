@@ -106,3 +106,42 @@ def test_request_over_budget_is_refused(gpu, pool):
     with pytest.raises(gpu.MoldError):
         srv.submit(comp, w, h, bg)
     srv.close()
+
+
+def test_heavy_request_is_a_batch_of_its_own(gpu, pool):
+    # 100 lines of 200,000 pixels each (mostly off a 64 x 64 canvas): about 2e7
+    # estimated line-pixels, above the server's heavy mark (2^24). Submitted
+    # between light requests, it renders alone and the light ones are batched
+    # without it; every frame still equals its render alone.
+    heavy = gpu.Composition()
+    heavy.create_layers(4)
+    rng = np.random.default_rng(5)
+    for k in range(100):
+        y0, y1 = rng.uniform(-10, 74, 2)
+        p = gpu.Path()
+        p.line(gpu.Point(-100000.0, float(y0)), gpu.Point(100000.0, float(y1)))
+        p.line(gpu.Point(100000.0, float(y1)), gpu.Point(100000.0, float(y1) + 3.0))
+        p.line(gpu.Point(100000.0, float(y1) + 3.0), gpu.Point(-100000.0, float(y0) + 3.0))
+        p.close()
+        heavy.insert_in_layer(k % 4, p)
+    for lid in range(4):
+        heavy.get_mut(lid).set_props(gpu.Props(fill_rule=gpu.FillRule(lid % 2), func=gpu.Func.Draw(
+            gpu.Style(fill=gpu.Fill.Solid([0.2 * lid, 0.5, 0.3, 0.6]), blend_mode=gpu.BlendMode(lid)))))
+    ref = np.zeros((64 * 64, 4), np.uint8)
+    heavy.render(gpu.Buffer(ref, 64), [1.0, 1.0, 1.0, 1.0], None)
+    srv = gpu.Server(engines=2)
+    light = [i for i in range(len(pool)) if pool[i][1] * pool[i][2] <= 256 * 256][:6]
+    tickets = []
+    for n, i in enumerate(light):
+        tickets.append((i, srv.submit(pool[i][0], pool[i][1], pool[i][2], pool[i][3])))
+        if n == 1:
+            tickets.append((-1, srv.submit(heavy, 64, 64, (1.0, 1.0, 1.0, 1.0))))
+    for i, t in tickets:
+        frame, res = srv.wait(t)
+        assert res.status == 0
+        if i < 0:
+            assert res.batch_size == 1, "a heavy request renders alone"
+            assert np.array_equal(frame, ref)
+        else:
+            assert np.array_equal(frame, pool[i][4])
+    srv.close()
