@@ -69,6 +69,7 @@ __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64
 struct SlotCounts {
     uint32_t *cnt0;
     uint32_t *acc;
+    uint32_t *part_kept;  // per sort partition and wave: leading slots the first pass reads
 };
 constexpr uint32_t kCountSlicesHost = 256;
 __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
@@ -94,7 +95,7 @@ __global__ void k_digit_bases(const uint32_t *hist, uint32_t *bases, uint32_t ra
 template <int R>
 __global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
                             uint32_t shift, const uint32_t *digit_base, uint32_t *status,
-                            uint32_t *stall, const uint32_t *first_off);
+                            uint32_t *stall, const uint32_t *first_off, const uint32_t *part_kept);
 __global__ void k_tile_starts(PaintParams pp, uint32_t global_rows, uint32_t stream_runs);
 __global__ void k_tile_runs(PaintParams pp, const uint2 *work, uint32_t n_work);
 __global__ void k_tile_nr(PaintParams pp, uint32_t global_rows);

@@ -450,7 +450,9 @@ constexpr uint32_t kSortTile = kSortThreads * kSortKPT;  // keys per sort partit
 constexpr uint32_t kEmitThreads = 256;
 constexpr uint32_t kEmitIPT = kSortKPT / 2;
 constexpr uint32_t kEmitTile = kEmitThreads * kEmitIPT;
+constexpr uint32_t kEmitWaveLp = kEmitTile / 4;  // line-pixels of one emission wave
 static_assert(2 * kEmitTile == kSortTile, "an emission partition fills exactly one sort partition");
+static_assert(2 * kEmitWaveLp == 64 * kSortKPT, "an emission wave fills exactly one sort wave's slots");
 static_assert(kEmitTile == kEmitTileHost && kSortTile == kSortTileHost, "host partition sizes");
 constexpr uint32_t kMaxPasses = 8;
 // Slot sentinels (bit 63 set; no word has it: render words use 57 bits and
@@ -615,29 +617,49 @@ __device__ __forceinline__ void count_flush(uint32_t *s_cnt, const FrameDims &d,
 // l - l0); otherwise `offs` and `lines` are the global arrays and the cull
 // bounds come from the scene table.
 template <bool Staged>
-__device__ __forceinline__ void emit_span(const LineSetup *lines, const SceneCull *culls, const uint32_t *offs,
+__device__ __forceinline__ uint32_t emit_span(const LineSetup *lines, const SceneCull *culls, const uint32_t *offs,
                                           uint32_t l0, uint32_t l1,
                                           uint64_t b0, uint64_t b1, uint64_t P,
                                           const SceneDesc *__restrict__ scenes, const FrameDims &dims,
-                                          uint4 *__restrict__ slots, uint32_t *s_cnt, const uint32_t *owner,
+                                          uint64_t *__restrict__ dst, uint32_t *s_cnt, const uint32_t *owner,
                                           uint32_t &kept) {
     const uint32_t base = Staged ? l0 : 0u;
-    for (uint64_t p = b0 + threadIdx.x; p < b1; p += 256) {
-        const uint32_t li = l0 + owner[p - b0];
-        const uint64_t first = offs[li - base];
-        const LineSetup L = lines[li - base];
-        const SceneCull sc = Staged ? culls[li - base] : scene_cull(scenes[L.packed >> 19]);
-        uint64_t w[2];
-        const int n = rasterize_pixel(L, (int32_t)(p - first), w);
-        uint64_t v[2] = {w[0], w[1]};
-        if (dims.mode == 0) {
+    const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+    const uint64_t lt = lanemask_lt();
+    // wave w: line-pixels b0 + kEmitWaveLp w .. in steps of 64; its kept
+    // words, in slot order, go to the front of its 2 kEmitWaveLp slots
+    const uint64_t wb0 = b0 + (uint64_t)kEmitWaveLp * w;
+    const uint32_t nsteps = wb0 < b1 ? (uint32_t)((b1 - wb0 < kEmitWaveLp ? b1 - wb0 : kEmitWaveLp) + 63) / 64 : 0u;
+    uint64_t *wdst = dst + 2 * (uint64_t)kEmitWaveLp * w;
+    uint32_t written = 0;  // kept words of the wave so far
+    for (uint32_t k = 0; k < nsteps; ++k) {
+        const uint64_t p = wb0 + 64 * k + lane;
+        uint64_t v[2] = {kSlotNone, kSlotNone};
+        if (p < b1) {
+            const uint32_t li = l0 + owner[p - b0];
+            const uint64_t first = offs[li - base];
+            const LineSetup L = lines[li - base];
+            const SceneCull sc = Staged ? culls[li - base] : scene_cull(scenes[L.packed >> 19]);
+            uint64_t wd[2];
+            const int n = rasterize_pixel(L, (int32_t)(p - first), wd);
+            v[0] = wd[0];
+            v[1] = wd[1];
+            if (dims.mode == 0) {
 #pragma unroll
-            for (int q = 0; q < 2; ++q) v[q] = render_word_cull(w[q], sc, dims);
+                for (int q = 0; q < 2; ++q) v[q] = render_word_cull(wd[q], sc, dims);
+            }
+            if (n < 2) v[1] = kSlotNone;
+            count_slot_pair(s_cnt, v[0], v[1], dims, kept);
         }
-        if (n < 2) v[1] = kSlotNone;
-        slots[p] = make_uint4((uint32_t)v[0], (uint32_t)(v[0] >> 32), (uint32_t)v[1], (uint32_t)(v[1] >> 32));
-        count_slot_pair(s_cnt, v[0], v[1], dims, kept);
+        // the step's kept words in slot order (line-pixel, then slot)
+        const bool k0 = !(v[0] >> 63), k1 = !(v[1] >> 63);
+        const uint64_t m0 = __ballot(k0), m1 = __ballot(k1);
+        const uint32_t pos = written + (uint32_t)(__popcll(m0 & lt) + __popcll(m1 & lt));
+        if (k0) wdst[pos] = v[0];
+        if (k1) wdst[pos + (k0 ? 1u : 0u)] = v[1];
+        written += (uint32_t)(__popcll(m0) + __popcll(m1));
     }
+    return written;
 }
 
 __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ lines,
@@ -664,8 +686,9 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
     const uint32_t l0 = part_line[part];
     const uint32_t l1 = part + 1 < nparts ? part_line[part + 1] : dims.n_lines - 1;
     const uint32_t nl = l1 - l0 + 1;
-    uint4 *slots = reinterpret_cast<uint4 *>(out);
-    uint32_t kept = 0;
+    // the partition's kept words go to the front of its 2 x kEmitTile slots
+    uint64_t *dst = out + 2 * b0;
+    uint32_t kept = 0, written = 0;
     // owner table: the first line-pixel of every line of nonzero length that
     // starts inside the partition gets the line's index (relative to l0,
     // which holds the partition's first line-pixel); a running maximum fills
@@ -703,11 +726,14 @@ __global__ __launch_bounds__(256) void k_emit(const LineSetup *__restrict__ line
             s_culls[k] = scene_cull(scenes[L.packed >> 19]);
         }
         __syncthreads();
-        emit_span<true>(s_lines, s_culls, s_off, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, s_owner, kept);
+        written = emit_span<true>(s_lines, s_culls, s_off, l0, l1, b0, b1, P, scenes, dims, dst, s_cnt, s_owner,
+                                  kept);
     } else {  // a partition of very short or zero-length lines
         __syncthreads();
-        emit_span<false>(lines, nullptr, offsets, l0, l1, b0, b1, P, scenes, dims, slots, s_cnt, s_owner, kept);
+        written = emit_span<false>(lines, nullptr, offsets, l0, l1, b0, b1, P, scenes, dims, dst, s_cnt, s_owner,
+                                   kept);
     }
+    if (lane_id() == 0) sc.part_kept[4 * part + (threadIdx.x >> 6)] = written;
     count_flush(s_cnt, dims, sc, part, nparts, kept);
 }
 
@@ -910,6 +936,12 @@ __global__ __launch_bounds__(256) void k_ingest(const uint64_t *__restrict__ in,
         out[i] = v;
         count_slot(s_cnt, v, dims, kept);
     }
+    // the first pass reads every slot of the partition (sentinels included)
+    if (lane_id() == 0) {
+        const uint64_t b = (uint64_t)part * kSortTile + 2 * (uint64_t)kEmitWaveLp * (threadIdx.x >> 6);
+        sc.part_kept[4 * part + (threadIdx.x >> 6)] =
+            b < n_slots ? (uint32_t)(n_slots - b < 2 * kEmitWaveLp ? n_slots - b : 2 * kEmitWaveLp) : 0u;
+    }
     count_flush(s_cnt, dims, sc, part, nparts, kept);
 }
 
@@ -949,7 +981,8 @@ template <int R>
 __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ in, uint64_t *__restrict__ out,
                                                    const FrameCounters *__restrict__ counters, uint32_t shift,
                                                    const uint32_t *__restrict__ digit_base, uint32_t *status,
-                                                   uint32_t *stall_flag, const uint32_t *__restrict__ first_off) {
+                                                   uint32_t *stall_flag, const uint32_t *__restrict__ first_off,
+                                                   const uint32_t *__restrict__ part_kept) {
     constexpr uint32_t RADIX = 1u << R;
     constexpr uint32_t DPT = (RADIX + 255) / 256;  // digits per thread
     __shared__ uint64_t s_keys[kSortTile];
@@ -970,6 +1003,10 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
         if (part >= nparts) return;
         const uint64_t base = (uint64_t)part * kSortTile;
         const uint32_t valid_n = (uint32_t)(n - base < kSortTile ? n - base : kSortTile);
+        // the first pass reads, of each wave's 64 x kSortKPT slots, the
+        // leading ones that hold words (the emission wave of the same slots
+        // wrote them there)
+        const uint32_t wave_lim = from_slots ? w * (64 * kSortKPT) + part_kept[4 * part + w] : valid_n;
         uint64_t keys[kSortKPT];
         uint32_t rank[kSortKPT];
         // all loads in flight before the first ranking step (bit 63 marks
@@ -977,7 +1014,7 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
 #pragma unroll
         for (uint32_t i = 0; i < kSortKPT; ++i) {
             const uint32_t li = w * (64 * kSortKPT) + i * 64 + lane;
-            keys[i] = li < valid_n ? in[base + li] : ~0ull;
+            keys[i] = li < valid_n && li < wave_lim ? in[base + li] : ~0ull;
         }
 #pragma unroll
         for (uint32_t i = 0; i < kSortKPT; ++i) {
@@ -1091,13 +1128,17 @@ __global__ __launch_bounds__(256) void k_sort_pass(const uint64_t *__restrict__ 
 }
 
 template __global__ void k_sort_pass<4>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *,
+                                        const uint32_t *);
 template __global__ void k_sort_pass<6>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *,
+                                        const uint32_t *);
 template __global__ void k_sort_pass<8>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *,
+                                        const uint32_t *);
 template __global__ void k_sort_pass<9>(const uint64_t *, uint64_t *, const FrameCounters *, uint32_t,
-                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *);
+                                        const uint32_t *, uint32_t *, uint32_t *, const uint32_t *,
+                                        const uint32_t *);
 
 // ------------------------------------------------------------------ paint
 //

@@ -170,7 +170,7 @@ private:
     DevBuf cnt0_, cnt_acc_, cnt_status_;
     DevBuf flat_cmds_, flat_paths_, flat_counts_, flat_offsets_, flat_status_;
     DevBuf work_, runs_work_, tile_nr_, runs_status_, walk_scratch_, tile_start_, tile_run0_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
-    DevBuf deep_list_;
+    DevBuf deep_list_, part_kept_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
     uint32_t n_work_ = 0, n_runs_work_ = 0;

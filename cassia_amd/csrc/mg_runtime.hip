@@ -64,7 +64,7 @@ Renderer::~Renderer() {
                      &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
                      &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
                      &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
-                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_, &walk_scratch_,
+                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_, &walk_scratch_, &part_kept_,
                      &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_, &deep_list_};
     for (DevBuf *b : all) b->release();
     if (ev0_) (void)hipEventDestroy(ev0_);
@@ -479,14 +479,15 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     const uint64_t n_cnt0 = (uint64_t)radix * nparts_sort;
     const uint64_t cnt_parts = (n_cnt0 + kScanTileHost - 1) / kScanTileHost;
     if (!cnt0_.reserve(std::max<uint64_t>(n_cnt0, 1) * 4) || !cnt_acc_.reserve((uint64_t)kCountSlicesHost * rest_rows * 4) ||
-        !cnt_status_.reserve(std::max<uint64_t>(cnt_parts, 1) * 8)) {
+        !cnt_status_.reserve(std::max<uint64_t>(cnt_parts, 1) * 8) ||
+        !part_kept_.reserve(std::max<uint64_t>(nparts_sort, 1) * 16)) {
         err = "out of device memory (slot counts)";
         return false;
     }
     fill32(cnt_acc_.as<uint32_t>(), 0, (uint64_t)kCountSlicesHost * rest_rows, stream_);
     // a capacity frame has partitions beyond the frame's: their counts are zero
     if (capture_P_) fill32(cnt0_.as<uint32_t>(), 0, n_cnt0, stream_);
-    SlotCounts counts{cnt0_.as<uint32_t>(), cnt_acc_.as<uint32_t>()};
+    SlotCounts counts{cnt0_.as<uint32_t>(), cnt_acc_.as<uint32_t>(), part_kept_.as<uint32_t>()};
     const size_t count_lds = (size_t)count_rows * 4;
     // every line-pixel owns two word slots; the ingest path uses the same layout
     if (ingest && spec.n_in) {
@@ -526,16 +527,16 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         const uint32_t *from_slots = p == 0 ? cnt0_.as<uint32_t>() : nullptr;
         if (R == 9)
             k_sort_pass<9><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots);
+                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
         else if (R == 8)
             k_sort_pass<8><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots);
+                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
         else if (R == 6)
             k_sort_pass<6><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots);
+                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
         else
             k_sort_pass<4><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots);
+                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
         std::swap(in, out);
     }
     MG_HIP(hipGetLastError(), "sort launch");
@@ -851,7 +852,7 @@ std::vector<const void *> Renderer::frame_buffers() const {
                            &bases_, &sort_status_, &part_counters_, &row_start_, &overflow_, &part_line_,
                            &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_,
                            &cnt_status_, &work_, &runs_work_, &tile_nr_, &runs_status_, &tile_start_, &tile_run0_, &run_key_, &run_beg_,
-                           &run_cov_, &rec_, &rec_len_, &rec_hdr_, &deep_list_};
+                           &run_cov_, &rec_, &rec_len_, &rec_hdr_, &deep_list_, &part_kept_};
     std::vector<const void *> v;
     for (const DevBuf *b : all) v.push_back(b->ptr);
     return v;
