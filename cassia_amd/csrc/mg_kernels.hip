@@ -1470,24 +1470,48 @@ __global__ __launch_bounds__(256) void k_run_cov(PaintParams pp, const FrameCoun
 }
 
 // Segmented inclusive scan over the 64 lanes of a wave (all active) of
-// byte-wise wrapping sums: (f, s) then (f', s') gives (f | f', f' ? s' :
-// s + s'). The DPP steps of wave_incl_scan; a lane whose source is outside
-// the row or the enabled rows combines with (0, 0), the identity.
+// eight row sums: (f, s) then (f', s') gives (f | f', f' ? s' : s + s').
+// The rows are 16-bit fields of four words (row 2i in the low half of word
+// i, row 2i + 1 in the high half) holding byte values; 64 of them sum to at
+// most 64 * 255 < 2^16, so plain word adds never carry between fields and
+// the low byte of a field is the byte-wise wrapping sum. The DPP steps of
+// wave_incl_scan; a lane whose source is outside the row or the enabled
+// rows combines with (0, 0), the identity.
+// The words move in every row (reading zero from outside the row,
+// bound_ctrl) and are masked off in lanes with f set and in the rows the
+// step does not combine, so that the move and the mask are one DPP AND.
 template <int Ctrl, int Rows>
-__device__ __forceinline__ void seg_scan_step(uint32_t &f, uint64_t &s) {
+__device__ __forceinline__ void seg_scan_step(uint32_t &f, uint32_t (&s)[4]) {
     const uint32_t fp = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)f, Ctrl, Rows, 0xF, false);
-    const uint32_t lo = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)s, Ctrl, Rows, 0xF, false);
-    const uint32_t hi = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)(uint32_t)(s >> 32), Ctrl, Rows, 0xF, false);
-    if (!f) s = add_bytes(((uint64_t)hi << 32) | lo, s);
+    const uint32_t row = lane_id() >> 4;
+    const bool on = Rows == 0xF || ((Rows >> row) & 1);
+    uint32_t m = (f == 0u && on) ? ~0u : 0u;
+    asm("" : "+v"(m));  // kept as a mask: an AND that takes the move as its DPP operand
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s[i] += (uint32_t)__builtin_amdgcn_mov_dpp((int)s[i], Ctrl, 0xF, 0xF, true) & m;
     f |= fp;
 }
-__device__ __forceinline__ void wave_seg_scan_bytes(uint32_t &f, uint64_t &s) {
+__device__ __forceinline__ void wave_seg_scan_rows(uint32_t &f, uint32_t (&s)[4]) {
     seg_scan_step<0x111, 0xF>(f, s);  // row_shr:1
     seg_scan_step<0x112, 0xF>(f, s);  // row_shr:2
     seg_scan_step<0x114, 0xF>(f, s);  // row_shr:4
     seg_scan_step<0x118, 0xF>(f, s);  // row_shr:8
     seg_scan_step<0x142, 0xA>(f, s);  // row_bcast:15
     seg_scan_step<0x143, 0xC>(f, s);  // row_bcast:31
+}
+// Eight packed bytes (byte r = row r) to and from the 16-bit fields.
+__device__ __forceinline__ void bytes_to_fields(uint64_t b, uint32_t (&s)[4]) {
+    const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
+    // v_perm_b32 selector byte 0x0C yields zero
+    s[0] = __builtin_amdgcn_perm(0u, lo, 0x0C010C00u);
+    s[1] = __builtin_amdgcn_perm(0u, lo, 0x0C030C02u);
+    s[2] = __builtin_amdgcn_perm(0u, hi, 0x0C010C00u);
+    s[3] = __builtin_amdgcn_perm(0u, hi, 0x0C030C02u);
+}
+__device__ __forceinline__ uint64_t fields_to_bytes(const uint32_t (&s)[4]) {
+    const uint32_t lo = __builtin_amdgcn_perm(s[1], s[0], 0x06040200u);
+    const uint32_t hi = __builtin_amdgcn_perm(s[3], s[2], 0x06040200u);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
 }
 
 // Per-tile form of the runs, for frames with many tiles of moderate depth:
@@ -1524,16 +1548,20 @@ __device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_
             const uint64_t heads = __ballot(head);
             // the open run ended exactly at the previous chunk's end
             if (lane == 0 && head && nr) pp.run_cov[ts + nr - 1] = open;
-            uint64_t s = 0;
+            uint64_t s0 = 0;
             if (valid) {
                 const int32_t cover = (int32_t)((int64_t)(w[b] << 58) >> 58);
                 const uint32_t ty = (uint32_t)(w[b] >> 19) & 7u;
-                s = (uint64_t)((uint32_t)cover & 0xFFu) << (8 * ty);
+                s0 = (uint64_t)((uint32_t)cover & 0xFFu) << (8 * ty);
             }
             // segmented inclusive scan: (f, s) (+) (f', s') = (f | f', f' ? s' : s + s')
-            uint32_t f = head ? 1u : 0u;
-            wave_seg_scan_bytes(f, s);
-            if (!f) s = add_bytes(open, s);
+            uint32_t f = head ? 1u : 0u, sf[4], of[4];
+            bytes_to_fields(s0, sf);
+            wave_seg_scan_rows(f, sf);
+            bytes_to_fields(open, of);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) sf[i] += f ? 0u : of[i];
+            const uint64_t s = fields_to_bytes(sf);
             const uint32_t r = nr - 1u + (uint32_t)__popcll(heads & ((2ull << lane) - 1ull));
             if (head) {
                 const bool eo = props[layer].flags & PF_EVENODD;
