@@ -1612,8 +1612,10 @@ __device__ __forceinline__ uint32_t merge_split(const uint32_t *qk, uint32_t nq,
 // layer_workbench.rs:194,475).
 // G: the queue and run arrays live in the row's slice of a global scratch
 // buffer instead of LDS (tiles with more layers than LDS holds).
+// 1024 threads in LDS: at most 64 VGPRs and few enough SGPRs for eight
+// waves per SIMD, two work groups (two tile rows) per CU.
 template <uint32_t T, bool G>
-__global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
+__global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ uint64_t s_wsum[T / 64];
     __shared__ uint32_t s_rec[3];

@@ -172,6 +172,9 @@ private:
     DevBuf work_, runs_work_, tile_nr_, runs_status_, walk_scratch_, tile_start_, tile_run0_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     DevBuf deep_list_, part_kept_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
+    // the large-frame row walk outgrew half the LDS once: it takes the whole
+    // LDS from then on
+    bool walk_whole_lds_ = false;
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
     uint32_t n_work_ = 0, n_runs_work_ = 0;
     uint64_t n_records_ = 0, n_runs_ = 0;

@@ -706,24 +706,37 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     uint32_t walk_threads = spec.walk_threads;
     if (walk_threads != 64 && walk_threads != 256 && walk_threads != 1024) walk_threads = fits ? 256 : 1024;
     const bool small_walk = fits;
-    // The whole-LDS configuration keeps two sets of run arrays so that the
+    // The large configuration keeps two sets of run arrays so that the
     // next column's runs are stored while the current column is merged (one
     // barrier per column; C2 walk 2.73 -> 2.61 ms). The small configuration
     // keeps one: its LDS share sets how many rows a CU walks at once.
+    // The large configuration first takes half the LDS (1,024 runs per tile,
+    // a queue of about 1,700 entries), so that two rows share a CU: with one
+    // row per CU, the rows beyond the CU count (270 rows on 256 CUs in a
+    // 3840x2160 frame) wait for a whole row to finish (C2 walk 2.69 -> 2.03
+    // ms). A frame that outgrows it is walked again with the whole LDS, and
+    // the walk keeps the whole LDS for later frames.
     const uint32_t run_bufs = small_walk ? 1u : 2u;
     pp.walk_run_bufs = run_bufs;
     uint32_t cap, rcap;
+    bool half_lds = false;
+    auto large_caps = [&](bool half) {
+        rcap = half ? 1024u : 2048u;
+        const size_t budget = half ? lds_max / 2 - 512 : lds_max;
+        const size_t fixed = walk_lds_bytes_host(0, rcap, tstride, run_bufs) + 256;
+        if (fixed + 64 * 24 > budget) return false;
+        cap = (uint32_t)std::min<size_t>(((budget - fixed) / 24) & ~size_t(63), 0xFFFFu - rcap);
+        return true;
+    };
     if (small_walk) {
         rcap = rcap_fit;
         cap = 2 * rcap;
     } else {
-        rcap = 2048;
-        const size_t fixed = walk_lds_bytes_host(0, rcap, tstride, run_bufs) + 256;
-        if (fixed + 64 * 24 > lds_max) {
+        half_lds = walk_threads == 1024 && !walk_whole_lds_ && large_caps(true);
+        if (!half_lds && !large_caps(false)) {
             err = "canvas too wide for the row walk";
             return false;
         }
-        cap = (uint32_t)std::min<size_t>(((lds_max - fixed) / 24) & ~size_t(63), 0xFFFFu - rcap);
     }
     walk_threads_out = walk_threads;
     size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride, run_bufs), lds_p = paint_lds_bytes_host(cap, clips);
@@ -780,6 +793,17 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
             return true;
         }
         if (ov[0] & 1u) {
+            if (half_lds) {
+                // outgrew half the LDS: walk again with the whole LDS
+                half_lds = false;
+                walk_whole_lds_ = true;
+                large_caps(false);
+                pp.cap_ids = cap;
+                pp.cap_runs = rcap;
+                lds_w = walk_lds_bytes_host(cap, rcap, tstride, run_bufs);
+                lds_p = paint_lds_bytes_host(cap, clips);
+                continue;
+            }
             // a tile holds more layers than the LDS arrays: walk again with
             // them in global memory, sized so that no tile can exceed them
             if (global_walk) break;

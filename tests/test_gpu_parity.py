@@ -119,6 +119,26 @@ def test_tiles_deeper_than_lds(gpu, cpu, n_layers):
     assert deep > 0 if n_layers >= 8192 else deep == 0
 
 
+def test_tiles_deeper_than_half_lds(gpu, cpu):
+    # 2500 layer orders (1500 of them cover the canvas, 1000 are empty): the
+    # large-frame row walk, which first takes half the LDS (1024 runs per
+    # tile); its 1500 runs per tile make it walk again with the whole LDS
+    # (2048), and the next frame starts there
+    w, h = 24, 20
+    ref = None
+    for m in (cpu, gpu):
+        comp = deep_scene(m, 1500, w, h, 78)
+        for _ in range(1000):
+            comp.create_layer()
+        for _ in range(2 if m is gpu else 1):
+            buf = np.full((w * h, 4), 7, np.uint8)
+            comp.render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
+            if ref is None:
+                ref = buf
+            else:
+                assert_same_image(ref, buf, w)
+
+
 def test_pixel_segments_canonical_stream(gpu, cpu):
     for seed in range(3):
         s_cpu = scenegen.build_scene(cpu, 500 + seed, 90, 70, 25, transforms=True, curves=True).pixel_segments()
