@@ -955,8 +955,8 @@ MG_EXPORT mg_status mg_set_tuning(mg_composition *c, uint32_t radix_bits) {
 
 MG_EXPORT mg_status mg_set_walk_threads(mg_composition *c, uint32_t threads) {
     if (check_comp(c)) return MG_ERR_INVALID_ARG;
-    if (threads != 0 && threads != 64 && threads != 256 && threads != 1024)
-        return fail(MG_ERR_INVALID_ARG, "walk threads must be 0, 64, 256 or 1024");
+    if (threads != 0 && threads != 64 && threads != 256 && threads != 512 && threads != 1024)
+        return fail(MG_ERR_INVALID_ARG, "walk threads must be 0, 64, 256, 512 or 1024");
     c->walk_threads = threads;
     return MG_OK;
 }
@@ -1129,8 +1129,8 @@ MG_EXPORT mg_status mg_batch_set_tuning(mg_batch *b, uint32_t radix_bits) {
 
 MG_EXPORT mg_status mg_batch_set_walk_threads(mg_batch *b, uint32_t threads) {
     if (!b) return fail(MG_ERR_INVALID_ARG, "batch is null");
-    if (threads != 0 && threads != 64 && threads != 256 && threads != 1024)
-        return fail(MG_ERR_INVALID_ARG, "walk threads must be 0, 64, 256 or 1024");
+    if (threads != 0 && threads != 64 && threads != 256 && threads != 512 && threads != 1024)
+        return fail(MG_ERR_INVALID_ARG, "walk threads must be 0, 64, 256, 512 or 1024");
     b->walk_threads = threads;
     b->tuned_for = ~0u;
     return MG_OK;

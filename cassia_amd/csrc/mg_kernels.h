@@ -102,7 +102,7 @@ __global__ void k_tile_nr(PaintParams pp, uint32_t global_rows);
 __global__ void k_runs_scan(PaintParams pp, const FrameCounters *counters, uint64_t *status, uint32_t *stall,
                             uint32_t *open);
 __global__ void k_run_cov(PaintParams pp, const FrameCounters *counters, const uint32_t *open);
-// T = 64, 256 or 1024 threads per tile row; G: queue in global scratch
+// T = 64, 256, 512 or 1024 threads per tile row; G: queue in global scratch
 template <uint32_t T, bool G>
 __global__ void k_row_walk(PaintParams pp);
 __global__ void k_paint(PaintParams pp);

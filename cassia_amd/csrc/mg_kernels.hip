@@ -1905,9 +1905,11 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
 }
 template __global__ void k_row_walk<64, false>(PaintParams);
 template __global__ void k_row_walk<256, false>(PaintParams);
+template __global__ void k_row_walk<512, false>(PaintParams);
 template __global__ void k_row_walk<1024, false>(PaintParams);
 template __global__ void k_row_walk<64, true>(PaintParams);
 template __global__ void k_row_walk<256, true>(PaintParams);
+template __global__ void k_row_walk<512, true>(PaintParams);
 template __global__ void k_row_walk<1024, true>(PaintParams);
 
 // Staged window of sorted words for the paint wave.

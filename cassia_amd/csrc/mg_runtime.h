@@ -63,7 +63,7 @@ struct FrameSpec {
     const uint64_t *in_segs = nullptr;  // mode 2: device PixelSegment stream
     uint64_t n_in = 0;
     uint32_t radix_bits = 0;            // sort digit width; 0 = automatic
-    uint32_t walk_threads = 0;          // row-walk work-group size (64, 256, 1024); 0 = automatic
+    uint32_t walk_threads = 0;          // row-walk work-group size (64, 256, 512, 1024); 0 = automatic
     bool want_tile_ops = false;
     bool want_coverage = false;
     uint32_t coverage_cap = 0;

@@ -273,7 +273,7 @@ MG_EXPORT mg_status mg_render_coverage(mg_composition *comp, uint32_t width, uin
    only the speed changes. */
 MG_EXPORT mg_status mg_set_tuning(mg_composition *comp, uint32_t radix_bits);
 /* Second result-invariant knob: threads per tile row in the row walk (64,
-   256 or 1024; 0 = automatic). */
+   256, 512 or 1024; 0 = automatic). */
 MG_EXPORT mg_status mg_set_walk_threads(mg_composition *comp, uint32_t threads);
 
 /* Statistics of the last render: lines, line-pixels, non-NONE segments,

@@ -164,7 +164,7 @@ def test_tuning_is_result_invariant(gpu, cpu, radix):
     assert np.array_equal(s_cpu, g2.pixel_segments())
 
 
-@pytest.mark.parametrize("walk", [64, 256, 1024])
+@pytest.mark.parametrize("walk", [64, 256, 512, 1024])
 @pytest.mark.parametrize("n_layers", [60, 700])
 def test_walk_threads_are_result_invariant(gpu, cpu, walk, n_layers):
     # 60 layers: the small-LDS walk; 700: more than 512 orders (the automatic
