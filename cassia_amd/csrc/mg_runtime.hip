@@ -690,15 +690,15 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     MG_HIP(hipGetLastError(), "tile runs launch");
     MG_HIP(mark(evs_[5]), "event");
 
-    // Row-walk LDS: the whole work group's share (one work group per CU is
-    // enough for the tile rows); runs of one tile up to rcap.
+    // Row-walk LDS: a share per work group (one tile row); runs of one tile
+    // up to rcap.
     const size_t lds_max = lds_max_;
     // A tile holds at most one run per layer order and the cover queue one
     // entry per order, so the walk never needs more than runs = orders and
     // queue + runs = 2 orders. Frames whose arrays sized that way fit in a
     // LDS share walk each row with 256 threads (several rows per CU when the
-    // share is small); others use 1024 threads and the whole LDS, which holds
-    // the deepest tiles (measured on MI355X: 256 threads are 12% faster for a
+    // share is small); others use 1024 threads and half the LDS, or the whole
+    // LDS for deeper tiles (measured on MI355X: 256 threads are 12% faster for a
     // 2,000-order frame, 1024 threads 34% faster for a 65,536-order one). The
     // work-group size is a result-invariant tuning knob (FrameSpec::walk_threads).
     const uint64_t orders = (uint64_t)spec.max_order + 1;
@@ -706,16 +706,18 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     const bool fits = orders <= 2048 && walk_lds_bytes_host(2 * rcap_fit, rcap_fit, tstride, 1) + 256 <= lds_max;
     uint32_t walk_threads = spec.walk_threads;
     if (walk_threads != 64 && walk_threads != 256 && walk_threads != 512 && walk_threads != 1024)
-        walk_threads = fits ? 256 : 1024;  // 512 threads: walk 0.08 ms faster on C2, paint as much slower
+        // (512 threads: the walk of a 3840x2160 frame of 200,000 polygons is
+        // 0.08 ms faster, its paint as much slower)
+        walk_threads = fits ? 256 : 1024;
     const bool small_walk = fits;
     // The large configuration keeps two sets of run arrays so that the
     // next column's runs are stored while the current column is merged (one
-    // barrier per column; C2 walk 2.73 -> 2.61 ms). The small configuration
+    // barrier per column; 3840x2160 walk 2.73 -> 2.61 ms). The small configuration
     // keeps one: its LDS share sets how many rows a CU walks at once.
     // The large configuration first takes half the LDS (1,024 runs per tile,
     // a queue of about 1,700 entries), so that two rows share a CU: with one
     // row per CU, the rows beyond the CU count (270 rows on 256 CUs in a
-    // 3840x2160 frame) wait for a whole row to finish (C2 walk 2.69 -> 2.03
+    // 3840x2160 frame) wait for a whole row to finish (its walk 2.69 -> 2.03
     // ms). A frame that outgrows it is walked again with the whole LDS, and
     // the walk keeps the whole LDS for later frames.
     const uint32_t run_bufs = small_walk ? 1u : 2u;
