@@ -507,7 +507,10 @@ __device__ __forceinline__ uint64_t render_word_cull(uint64_t w, const SceneCull
 // cross-work-group prefix is needed here. One work group per kEmitTile
 // consecutive line-pixels; thread t takes line-pixels b0 + t + 256 k, so each
 // store instruction of a wave writes 64 consecutive slot pairs (1 KB).
-constexpr uint32_t kEmitLines = 512;  // lines of a partition staged in LDS
+// Lines of a partition staged in LDS (partitions with more lines read them
+// from global memory). 256 keeps the emission's LDS near 22 KB, seven work
+// groups per CU instead of four at 512 (3840x2160 frame: 2.60 -> 2.10 ms).
+constexpr uint32_t kEmitLines = 256;
 constexpr uint32_t kCountSlices = 256;  // accumulation slices of the later passes' digit totals
 
 // Per-partition counts of the word slots (a partition is kSortTile slots).
