@@ -69,7 +69,10 @@ class FrameStats(C.Structure):
                 ("radix_bits", C.c_uint32), ("device_ms", C.c_double), ("setup_ms", C.c_double),
                 ("emit_ms", C.c_double), ("hist_ms", C.c_double), ("sort_ms", C.c_double),
                 ("runs_ms", C.c_double), ("walk_ms", C.c_double), ("paint_ms", C.c_double),
-                ("runs", C.c_uint64), ("records", C.c_uint64), ("walk_threads", C.c_uint32), ("deep_tiles", C.c_uint32)]
+                ("runs", C.c_uint64), ("records", C.c_uint64), ("walk_threads", C.c_uint32), ("deep_tiles", C.c_uint32),
+                ("walk_lds", C.c_uint32), ("walk_rewalks", C.c_uint32)]
+
+WALK_SHARE, WALK_HALF, WALK_WHOLE, WALK_GLOBAL = 0, 1, 2, 3
 
 
 def _props_dtype():

@@ -137,6 +137,9 @@ struct mg_composition {
     uint64_t version = 1;      // bumped by every change of layers, props or lines
     uint32_t radix_bits = 0;
     uint32_t walk_threads = 0;
+    // the row walk's need on the last render (largest runs of a tile, queue
+    // + runs), which picks its LDS configuration on the next one
+    uint32_t walk_need_runs = 0, walk_need_ids = 0;
     // frame tables (layers by id, props by order) of the last render, valid
     // while `version` and the layer cache id are unchanged
     mg::FrameSpec spec;
@@ -702,6 +705,8 @@ static void fill_stats(mg_frame_stats &st, const mg::FrameResult &r, size_t line
     st.records = r.records;
     st.walk_threads = r.walk_threads;
     st.deep_tiles = r.deep_tiles;
+    st.walk_lds = r.walk_lds;
+    st.walk_rewalks = r.walk_rewalks;
 }
 
 // Scene descriptor of one frame of `width` x `rows_px` pixels at `stride`
@@ -764,6 +769,8 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.coverage_cap = coverage_cap;
     f.radix_bits = c->radix_bits;
     f.walk_threads = c->walk_threads;
+    f.walk_need_runs = c->walk_need_runs;
+    f.walk_need_ids = c->walk_need_ids;
     mg_status s;
     if ((s = c->sync_lines())) return s;
     const bool ok = graph_replayed
@@ -771,6 +778,8 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
                                             *graph_replayed, err)
                         : c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->n_lines(), img, prev, r, err);
     if (!ok) return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
+    c->walk_need_runs = r.walk_max_runs;
+    c->walk_need_ids = r.walk_max_ids;
     if (cache_id >= 0) {
         bool changed = false;
         for (auto &kv : c->layers) {
@@ -988,6 +997,7 @@ struct mg_batch {
     bool autotune = true;       // false for serving batches, whose shapes change every batch
     // autotuner: the choice holds for batches of the same shape
     uint32_t tuned_bits = 0, tuned_walk = 0, tuned_for = ~0u;
+    uint32_t walk_need_runs = 0, walk_need_ids = 0;  // as mg_composition's
     mg_batch_tuning tuning{};
     mg_frame_stats stats{};
 
@@ -1190,8 +1200,12 @@ mg_status batch_render(mg_batch *b, mg_composition *const *comps, const mg_batch
     auto run = [&](uint32_t bits, uint32_t walk) -> mg_status {
         f.radix_bits = bits;
         f.walk_threads = walk;
+        f.walk_need_runs = b->walk_need_runs;
+        f.walk_need_ids = b->walk_need_ids;
         if (!b->rdr->run(f, nullptr, nullptr, b->n_lines, b->frame.as<uint8_t>(), nullptr, r, err))
             return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
+        b->walk_need_runs = r.walk_max_runs;
+        b->walk_need_ids = r.walk_max_ids;
         return MG_OK;
     };
     uint32_t bits = b->radix_bits, walk = b->walk_threads;

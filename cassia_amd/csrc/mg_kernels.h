@@ -30,6 +30,7 @@ struct PaintParams {
     uint32_t *deep_count;
     uint2 *deep_list;
     uint32_t *overflow;          // bit 0: LDS capacity, bit 1: record capacity
+    uint32_t *walk_max;          // the walk's largest runs of a tile [0] and queue + runs [1] (atomicMax)
     // debug: coverage records (A values) and per-tile write ops
     int32_t *cov_a;
     uint32_t *cov_key;

@@ -1692,11 +1692,22 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
     };
     // The runs of column col into run arrays `buf`; false when they exceed
     // the arrays (the work group must stop).
+    // largest runs of a tile and queue + runs met by this row (thread 0),
+    // published at the row's end or when it stops
+    uint32_t max_runs = 0, max_ids = 0;
+    auto publish_max = [&]() {
+        if (tid == 0) {
+            atomicMax(pp.walk_max, max_runs);
+            atomicMax(pp.walk_max + 1, max_ids);
+        }
+    };
     auto store = [&](const Prefetch &f, uint32_t col, uint32_t buf) -> bool {
         if (col > h1) return true;
         const uint32_t nr = nr_l[col];
         if (nr > rcap) {
+            max_runs = nr > max_runs ? nr : max_runs;
             if (tid == 0) atomicOr(pp.overflow, 1u);
+            publish_max();
             return false;
         }
         if (tid < f.n) {
@@ -1718,8 +1729,11 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
     auto step = [&](uint32_t colp, uint32_t buf) -> bool {
         const bool fold = colp <= h0;
         const uint32_t nr = nr_l[colp];
+        max_runs = nr > max_runs ? nr : max_runs;
+        max_ids = nq + nr > max_ids ? nq + nr : max_ids;
         if (nq + nr > cap) {
             if (tid == 0) atomicOr(pp.overflow, 1u);
+            publish_max();
             return false;
         }
         const uint32_t ts = ts_l[colp], te = ts_l[colp + 1];
@@ -1905,6 +1919,7 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
         c2 = c4;
     }
     if (tid == 0 && row_records) atomicAdd(pp.n_records, row_records);
+    publish_max();
 }
 template __global__ void k_row_walk<64, false>(PaintParams);
 template __global__ void k_row_walk<256, false>(PaintParams);

@@ -64,6 +64,10 @@ struct FrameSpec {
     uint64_t n_in = 0;
     uint32_t radix_bits = 0;            // sort digit width; 0 = automatic
     uint32_t walk_threads = 0;          // row-walk work-group size (64, 256, 512, 1024); 0 = automatic
+    // what the row walk of this frame's previous render needed (FrameResult::
+    // walk_max_*), 0 = unknown: a large-frame walk that needed more than half
+    // the LDS starts with the whole LDS
+    uint32_t walk_need_runs = 0, walk_need_ids = 0;
     bool want_tile_ops = false;
     bool want_coverage = false;
     uint32_t coverage_cap = 0;
@@ -77,6 +81,10 @@ struct FrameResult {
     uint64_t line_pixels = 0, segments = 0, kept = 0;
     uint32_t passes = 0, radix_bits = 0, sort_bits = 0, walk_threads = 0;
     uint32_t deep_tiles = 0;  // tiles painted by the four-wave deep painter
+    // row walk: LDS configuration of the final walk (WalkLds), walks repeated
+    // for LDS capacity, and the largest runs per tile and queue + runs it met
+    uint32_t walk_lds = 0, walk_rewalks = 0;
+    uint32_t walk_max_runs = 0, walk_max_ids = 0;
     uint64_t tiles = 0;
     double device_ms = 0.0;
     // setup+scan, emit, histogram+bases, sort passes, tile runs, row walk, paint
@@ -84,6 +92,11 @@ struct FrameResult {
     uint64_t runs = 0, records = 0;     // tile runs and tile id records of the painted frame
     const uint64_t *sorted = nullptr;   // device pointer to the sorted words (mode 1)
 };
+
+// LDS configuration of the row walk: a share of the LDS (several rows per
+// CU), half the LDS (two rows per CU), the whole LDS, or arrays in global
+// memory for tiles deeper than LDS holds.
+enum WalkLds : uint32_t { WALK_SHARE = 0, WALK_HALF = 1, WALK_WHOLE = 2, WALK_GLOBAL = 3 };
 
 // Bit layout of a frame's render key: (row, column + 1, layer order) above
 // the 22-bit payload, and the bits the LSD sort covers.
@@ -151,7 +164,7 @@ private:
     void drop_graph();
     bool paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
                const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
-               uint32_t &walk_threads, std::string &err);
+               FrameResult &res, std::string &err);
 
     int device_ = -1;
     int cus_ = 0;
@@ -172,9 +185,6 @@ private:
     DevBuf work_, runs_work_, tile_nr_, runs_status_, walk_scratch_, tile_start_, tile_run0_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;
     DevBuf deep_list_, part_kept_;
     uint64_t rec_cap_ = 0;  // id records of the last frame's size
-    // the large-frame row walk outgrew half the LDS once: it takes the whole
-    // LDS from then on
-    bool walk_whole_lds_ = false;
     std::vector<SceneDesc> work_scenes_;  // scenes of the resident paint work list
     uint32_t n_work_ = 0, n_runs_work_ = 0;
     uint64_t n_records_ = 0, n_runs_ = 0;

@@ -564,8 +564,8 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
             }
             fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         }
-        if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, cap_segs, image, prev_layers, tiles,
-                   res.walk_threads, err))
+        if (!paint(spec, global_rows, dims.col_bits, dims.layer_bits, in, cap_segs, image, prev_layers, tiles, res,
+                   err))
             return false;
     }
     MG_HIP(mark(ev1_), "event");
@@ -596,7 +596,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
 
 bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_bits, uint32_t layer_bits,
                      const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
-                     uint32_t &walk_threads_out, std::string &err) {
+                     FrameResult &res, std::string &err) {
     if (global_rows == 0) return true;
     // one paint work group per tile inside the crop; the list is rebuilt
     // only when the scene descriptors change
@@ -657,6 +657,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.deep_min = spec.want_coverage ? 0u : 8192u;
     pp.deep_cap = 2 * cus_;
     pp.deep_count = overflow_.as<uint32_t>() + 4;
+    pp.walk_max = overflow_.as<uint32_t>() + 5;
     if (!deep_list_.reserve((size_t)pp.deep_cap * 8)) {
         err = "out of device memory (deep tile list)";
         return false;
@@ -718,8 +719,10 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     // a queue of about 1,700 entries), so that two rows share a CU: with one
     // row per CU, the rows beyond the CU count (270 rows on 256 CUs in a
     // 3840x2160 frame) wait for a whole row to finish (its walk 2.69 -> 2.03
-    // ms). A frame that outgrows it is walked again with the whole LDS, and
-    // the walk keeps the whole LDS for later frames.
+    // ms). A frame that outgrows it is walked again with the whole LDS. The
+    // walk reports the largest tile it met, and a frame whose previous render
+    // (FrameSpec::walk_need_*) needed more than half the LDS starts with the
+    // whole LDS; one that fits again returns to half.
     const uint32_t run_bufs = small_walk ? 1u : 2u;
     pp.walk_run_bufs = run_bufs;
     uint32_t cap, rcap;
@@ -736,13 +739,16 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         rcap = rcap_fit;
         cap = 2 * rcap;
     } else {
-        half_lds = walk_threads >= 512 && !walk_whole_lds_ && large_caps(true);
+        half_lds = walk_threads >= 512 && large_caps(true) && spec.walk_need_runs <= rcap &&
+                   spec.walk_need_ids <= cap;
         if (!half_lds && !large_caps(false)) {
             err = "canvas too wide for the row walk";
             return false;
         }
     }
-    walk_threads_out = walk_threads;
+    res.walk_threads = walk_threads;
+    res.walk_lds = small_walk ? WALK_SHARE : (half_lds ? WALK_HALF : WALK_WHOLE);
+    res.walk_rewalks = 0;
     size_t lds_w = walk_lds_bytes_host(cap, rcap, tstride, run_bufs), lds_p = paint_lds_bytes_host(cap, clips);
     if (lds_w > lds_max) {
         err = "canvas too wide for the row walk";
@@ -769,7 +775,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.rec_cap = rec_cap_;
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
-        fill32(overflow_.as<uint32_t>() + 4, 0, 1, stream_);
+        fill32(overflow_.as<uint32_t>() + 4, 0, 3, stream_);
         MG_HIP(mark(evs_[5]), "event");
         const size_t lds = global_walk ? 0 : lds_w;
         if (walk_threads == 64)
@@ -790,9 +796,11 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         MG_HIP(mark(evs_[7]), "event");
         MG_HIP(hipGetLastError(), "paint launch");
         if (capture_P_) return true;  // one attempt; the replay reads the flags
-        uint32_t ov[5] = {0, 0, 0, 0, 0};
-        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 20, hipMemcpyDeviceToHost, stream_), "read overflow");
+        uint32_t ov[7] = {0, 0, 0, 0, 0, 0, 0};
+        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, sizeof ov, hipMemcpyDeviceToHost, stream_), "read overflow");
         MG_HIP(hipStreamSynchronize(stream_), "paint");
+        res.walk_max_runs = ov[5];
+        res.walk_max_ids = ov[6];
         if (!ov[0]) {
             n_records_ = ov[2];
             n_runs_ = ov[3];
@@ -800,10 +808,11 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
             return true;
         }
         if (ov[0] & 1u) {
+            ++res.walk_rewalks;
             if (half_lds) {
                 // outgrew half the LDS: walk again with the whole LDS
                 half_lds = false;
-                walk_whole_lds_ = true;
+                res.walk_lds = WALK_WHOLE;
                 large_caps(false);
                 pp.cap_ids = cap;
                 pp.cap_runs = rcap;
@@ -832,6 +841,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
             pp.cap_ids = cap;
             pp.cap_runs = rcap;
             global_walk = true;
+            res.walk_lds = WALK_GLOBAL;
             continue;
         }
         if (rec_cap_ >= 0xFFFF0000ull) break;

@@ -94,7 +94,10 @@ class Scene:
             for lid, p in enumerate(self.props):
                 if present[lid]:
                     comp.get_mut(lid).set_props(props_of(m, p))
-        if self.orders is not None:
+        if self.orders is not None and hasattr(comp, "set_orders"):
+            ids = np.flatnonzero(present).astype(np.uint16)
+            comp.set_orders(ids, np.asarray(self.orders, np.uint16)[ids])
+        elif self.orders is not None:
             for lid in range(n):
                 layer = comp.get_mut(lid)
                 if layer is not None:

@@ -293,6 +293,10 @@ typedef struct mg_frame_stats {
     uint64_t runs, records;    /* tile layer runs, tile id records       */
     uint32_t walk_threads;     /* row-walk work-group size used          */
     uint32_t deep_tiles;       /* tiles painted four waves per tile      */
+    /* row-walk arrays of the final walk: 0 a share of the LDS, 1 half the
+       LDS, 2 the whole LDS, 3 global memory; walks repeated because a tile
+       outgrew the arrays */
+    uint32_t walk_lds, walk_rewalks;
 } mg_frame_stats;
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
 

@@ -119,24 +119,52 @@ def test_tiles_deeper_than_lds(gpu, cpu, n_layers):
     assert deep > 0 if n_layers >= 8192 else deep == 0
 
 
+def render_deep(m, n_layers, order_step, seed, frames=1, w=24, h=20):
+    """deep_scene with layer i at order i * order_step (orders beyond 2,048
+    select the large-frame row walk; layers without lines do not exist in
+    mold, so only orders of drawn layers count); the buffers of `frames`
+    renders and the composition."""
+    comp = deep_scene(m, n_layers, w, h, seed)
+    for i in range(n_layers):
+        comp.get_mut(i).set_order(i * order_step)
+    bufs = []
+    for _ in range(frames):
+        buf = np.full((w * h, 4), 7, np.uint8)
+        comp.render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
+        bufs.append(buf)
+    return bufs, comp
+
+
 def test_tiles_deeper_than_half_lds(gpu, cpu):
-    # 2500 layer orders (1500 of them cover the canvas, 1000 are empty): the
-    # large-frame row walk, which first takes half the LDS (1024 runs per
-    # tile); its 1500 runs per tile make it walk again with the whole LDS
-    # (2048), and the next frame starts there
+    # 1,500 layers covering the canvas at orders up to 2,998: the
+    # large-frame row walk, which first takes half the LDS (1,024 runs per
+    # tile); 1,500 runs per tile make the first frame walk again with the
+    # whole LDS, and the composition's next frame starts there
     w, h = 24, 20
-    ref = None
-    for m in (cpu, gpu):
-        comp = deep_scene(m, 1500, w, h, 78)
-        for _ in range(1000):
-            comp.create_layer()
-        for _ in range(2 if m is gpu else 1):
-            buf = np.full((w * h, 4), 7, np.uint8)
-            comp.render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
-            if ref is None:
-                ref = buf
-            else:
-                assert_same_image(ref, buf, w)
+    (ref,), _ = render_deep(cpu, 1500, 2, 78)
+    (f0, f1), g = render_deep(gpu, 1500, 2, 78, frames=2)
+    assert_same_image(ref, f0, w)
+    assert_same_image(ref, f1, w)
+    st = g.stats()
+    assert st.walk_lds == gpu.WALK_WHOLE and st.walk_rewalks == 0, (st.walk_lds, st.walk_rewalks)
+    # a new composition of the same shape starts with half the LDS again
+    (f2,), g2 = render_deep(gpu, 1500, 2, 78)
+    assert_same_image(ref, f2, w)
+    st2 = g2.stats()
+    assert st2.walk_lds == gpu.WALK_WHOLE and st2.walk_rewalks == 1, (st2.walk_lds, st2.walk_rewalks)
+
+
+def test_large_walk_in_half_lds(gpu, cpu):
+    # 600 covering layers at orders up to 2,396: the large-frame walk, whose
+    # tiles fit half the LDS, rendered after a frame that needed the whole LDS
+    w, h = 24, 20
+    render_deep(gpu, 1500, 2, 78)
+    (ref,), _ = render_deep(cpu, 600, 4, 79)
+    (out, out2), g = render_deep(gpu, 600, 4, 79, frames=2)
+    assert_same_image(ref, out, w)
+    assert_same_image(ref, out2, w)
+    st = g.stats()
+    assert st.walk_lds == gpu.WALK_HALF and st.walk_rewalks == 0, (st.walk_lds, st.walk_rewalks)
 
 
 def test_pixel_segments_canonical_stream(gpu, cpu):
