@@ -13,7 +13,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmoldgpu.so")
 
-SOURCES = ["mg_kernels.hip", "mg_runtime.hip", "mg_path.cpp", "mg_api.cpp", "mg_server.cpp"]
+SOURCES = ["mg_kernels.hip", "mg_runtime.hip", "mg_scratch.hip", "mg_path.cpp", "mg_api.cpp", "mg_server.cpp"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-fvisibility=hidden",

@@ -17,6 +17,7 @@
 #include "mg_internal.h"
 #include "mg_path.h"
 #include "mg_runtime.h"
+#include "mg_scratch.h"
 #include "mg_shared.h"
 
 namespace {
@@ -867,6 +868,84 @@ MG_EXPORT mg_status mg_render_device(mg_composition *c, uint32_t width, uint32_t
     mg::FrameResult r;
     if ((s = render_frame(c, width, width, height, bg, crop, -1, nullptr, img, bytes, false, r))) return s;
     *device_rgba = img;
+    return MG_OK;
+}
+
+// mg_render_device with no device allocation: the frame engine of the
+// callers that cannot allocate takes every buffer from the static scratch
+// array of mg_scratch.hip (kNoAllocScratchBytes per device), the output is
+// the caller's device buffer, and the composition's lines are read in place
+// when they are resident and current, else copied into the scratch.
+MG_EXPORT mg_status mg_render_device_noalloc(mg_composition *c, uint32_t width, uint32_t height, const float bg[4],
+                                             const mg_rect *crop, void *device_rgba, size_t device_bytes) {
+    if (check_comp(c) || !bg || !device_rgba || width == 0 || height == 0)
+        return fail(MG_ERR_INVALID_ARG, "bad argument");
+    const size_t bytes = (size_t)width * height * 4;
+    if (device_bytes < bytes) return fail(MG_ERR_BUFFER, "device frame shorter than width * height * 4 bytes");
+    std::string err;
+    const int dev = mg::render_device();
+    if (dev < 0) return fail(MG_ERR_DEVICE, "no gfx950 device");
+    void *base = nullptr;
+    if (!mg::scratch_for_device(dev, &base, err)) return fail(MG_ERR_DEVICE, err);
+    std::shared_ptr<mg::Renderer> rdr = mg::Renderer::shared_arena(base, mg::kNoAllocScratchBytes, err);
+    if (!rdr) return fail(MG_ERR_DEVICE, err);
+    std::lock_guard<std::mutex> g(rdr->mu);
+    if (hipSetDevice(rdr->device()) != hipSuccess) return fail(MG_ERR_DEVICE, "hipSetDevice failed");
+    prepare_tables(c, -1);  // remove_disabled may compact the host lines first
+    const size_t n = c->n_lines();
+    rdr->arena_begin();
+    const float4 *pts = nullptr;
+    const uint16_t *ids = nullptr;
+    const bool resident = n > 0 && c->rdr && c->rdr->device() == rdr->device() && c->uploaded == c->ids.size() &&
+                          c->dlines.cap >= n;
+    if (resident) {
+        pts = c->dlines.points();
+        ids = c->dlines.layer_ids();
+    } else if (n) {
+        mg_status s = c->ensure_mirror();
+        if (s) return s;
+        void *p = rdr->arena_take(n * 16), *q = rdr->arena_take(n * 2);
+        if (!p || !q)
+            return fail(MG_ERR_CAPACITY, "the composition's lines do not fit the no-allocation scratch (" +
+                                             std::to_string(mg::kNoAllocScratchBytes >> 20) + " MiB)");
+        if (hipMemcpyAsync(p, c->pts.data(), n * 16, hipMemcpyHostToDevice, rdr->stream()) != hipSuccess ||
+            hipMemcpyAsync(q, c->ids.data(), n * 2, hipMemcpyHostToDevice, rdr->stream()) != hipSuccess)
+            return fail(MG_ERR_DEVICE, "upload of the lines failed");
+        pts = static_cast<const float4 *>(p);
+        ids = static_cast<const uint16_t *>(q);
+    }
+    mg::FrameSpec &f = c->spec;
+    f.scenes.clear();
+    f.scene_line_base.clear();
+    f.tables_owner = c->owner;
+    f.tables_version = c->version * 64;
+    mg::SceneDesc sd = scene_desc(width, width, height, bg, crop);
+    sd.n_lines = (uint32_t)n;
+    sd.n_layer_entries = (uint32_t)f.layers.size();
+    sd.n_props = (uint32_t)f.props.size();
+    sd.flags = c->spec_clips ? mg::SF_CLIPS : 0;
+    f.scenes.push_back(sd);
+    f.scene_line_base.push_back(0);
+    f.mode = 0;
+    f.image_bytes = bytes;
+    f.want_tile_ops = false;
+    f.want_coverage = false;
+    f.coverage_cap = 0;
+    f.radix_bits = c->radix_bits;
+    f.walk_threads = c->walk_threads;
+    f.walk_need_runs = c->walk_need_runs;
+    f.walk_need_ids = c->walk_need_ids;
+    mg::FrameResult r;
+    if (!rdr->run(f, pts, ids, n, static_cast<uint8_t *>(device_rgba), nullptr, r, err)) {
+        if (rdr->arena().short_by)
+            return fail(MG_ERR_CAPACITY, "the frame needs more device scratch than the no-allocation entry point's " +
+                                             std::to_string(mg::kNoAllocScratchBytes >> 20) + " MiB (at least " +
+                                             std::to_string(rdr->arena().short_by) + " more bytes)");
+        return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
+    }
+    c->walk_need_runs = r.walk_max_runs;
+    c->walk_need_ids = r.walk_max_ids;
+    fill_stats(c->stats, r, n);
     return MG_OK;
 }
 

@@ -17,10 +17,21 @@
 
 namespace mg {
 
-// Growable device allocation.
+// A fixed block of device memory handed out front to back; nothing is freed
+// until reset. An engine in arena mode takes every buffer from one.
+struct Arena {
+    unsigned char *base = nullptr;
+    size_t size = 0, used = 0;
+    size_t short_by = 0;  // bytes a refused request lacked (0: none refused)
+    void *take(size_t n);
+};
+
+// Growable device allocation (hipMalloc), or a block of an arena when
+// `arena` is set (no device allocation; the block is dropped, not freed).
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    Arena *arena = nullptr;
     bool reserve(size_t n);  // keeps contents only when not growing
     void release();
     template <class T>
@@ -115,6 +126,18 @@ public:
     ~Renderer();
     bool init(std::string &err);
     static std::shared_ptr<Renderer> shared(std::string &err);
+    // An engine whose buffers all come from `bytes` of device memory at
+    // `base` (no hipMalloc or hipFree, ever); its frames start with
+    // arena_begin(). One per device, shared by the callers that cannot
+    // allocate: `device` is the device of the shared engine.
+    static std::shared_ptr<Renderer> shared_arena(void *base, size_t bytes, std::string &err);
+    // Drops every buffer of the engine (arena mode): the next frame takes
+    // them from the arena's start. Cached tables are uploaded again.
+    void arena_begin();
+    // `bytes` of the arena for the caller's frame inputs (nullptr when the
+    // arena cannot hold them); valid until the next arena_begin().
+    void *arena_take(size_t bytes) { return arena_.take(bytes); }
+    const Arena &arena() const { return arena_; }
 
     std::mutex mu;
 
@@ -166,6 +189,9 @@ private:
                const uint64_t *sorted, uint64_t words_cap, uint8_t *image, int32_t *prev_layers, size_t tiles,
                FrameResult &res, std::string &err);
 
+    std::vector<DevBuf *> all_buffers();
+    Arena arena_;
+    bool arena_mode_ = false;
     int device_ = -1;
     int cus_ = 0;
     size_t lds_max_ = 65536;
@@ -200,5 +226,7 @@ private:
 
 // Number of usable gfx950 devices.
 int device_count();
+// The device an engine renders on: the first gfx950 device (-1: none).
+int render_device();
 
 }  // namespace mg

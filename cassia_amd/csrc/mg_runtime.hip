@@ -17,8 +17,26 @@ namespace mg {
         }                                                                   \
     } while (0)
 
+void *Arena::take(size_t n) {
+    const size_t want = (std::max<size_t>(n, 256) + 255) & ~size_t(255);
+    if (want > size - used) {
+        short_by = std::max(short_by, want - (size - used));
+        return nullptr;
+    }
+    void *p = base + used;
+    used += want;
+    return p;
+}
+
 bool DevBuf::reserve(size_t n) {
     if (n <= bytes) return true;
+    if (arena) {
+        void *p = arena->take(n);
+        if (!p) return false;
+        ptr = p;
+        bytes = (std::max<size_t>(n, 256) + 255) & ~size_t(255);
+        return true;
+    }
     release();
     size_t want = std::max<size_t>(n, 256);
     if (hipMalloc(&ptr, want) != hipSuccess) {
@@ -31,7 +49,7 @@ bool DevBuf::reserve(size_t n) {
 }
 
 void DevBuf::release() {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !arena) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
 }
@@ -51,7 +69,27 @@ int device_count() {
     return ok;
 }
 
+int render_device() {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return -1;
+    for (int d = 0; d < n; ++d)
+        if (is_gfx950(d)) return d;
+    return -1;
+}
+
 Renderer::Renderer() {}
+
+std::vector<DevBuf *> Renderer::all_buffers() {
+    return {&scenes_, &scene_line_base_, &layers_, &props_, &stops_,
+            &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
+            &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
+            &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
+            &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_,
+            &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
+            &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_,
+            &walk_scratch_, &part_kept_, &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_,
+            &deep_list_};
+}
 
 Renderer::~Renderer() {
     if (device_ >= 0) {
@@ -59,14 +97,7 @@ Renderer::~Renderer() {
         if (stream_) (void)hipStreamSynchronize(stream_);
     }
     drop_graph();
-    DevBuf *all[] = {&scenes_, &scene_line_base_, &layers_, &props_, &stops_,
-                     &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
-                     &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
-                     &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
-                     &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
-                     &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_, &walk_scratch_, &part_kept_,
-                     &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_, &deep_list_};
-    for (DevBuf *b : all) b->release();
+    for (DevBuf *b : all_buffers()) b->release();
     if (ev0_) (void)hipEventDestroy(ev0_);
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evs_)
@@ -80,8 +111,7 @@ bool Renderer::init(std::string &err) {
         err = "no HIP device";
         return false;
     }
-    for (int d = 0; d < n; ++d)
-        if (is_gfx950(d)) { device_ = d; break; }
+    device_ = render_device();
     if (device_ < 0) {
         err = "no gfx950 device";
         return false;
@@ -113,6 +143,37 @@ bool Renderer::init(std::string &err) {
         return false;
     }
     return true;
+}
+
+void Renderer::arena_begin() {
+    arena_.used = 0;
+    arena_.short_by = 0;
+    for (DevBuf *b : all_buffers()) b->release();
+    // tables, work lists and the record store are taken anew by the frame
+    uploaded_owner_ = uploaded_version_ = 0;
+    uploaded_scenes_.clear();
+    work_scenes_.clear();
+    n_work_ = n_runs_work_ = 0;
+    rec_cap_ = 0;
+    last_sorted_ = nullptr;
+    // the frame's fixed counters (their contents are cleared by every frame)
+    (void)(counters_.reserve(sizeof(FrameCounters)) && err_.reserve(16) && overflow_.reserve(64) &&
+           part_counters_.reserve(64 * sizeof(uint32_t)) && cov_count_.reserve(16));
+}
+
+std::shared_ptr<Renderer> Renderer::shared_arena(void *base, size_t bytes, std::string &err) {
+    static std::mutex m;
+    static std::shared_ptr<Renderer> r;  // lives until the process ends, as the arena does
+    std::lock_guard<std::mutex> g(m);
+    if (r) return r;
+    auto e = std::make_shared<Renderer>();
+    e->arena_mode_ = true;
+    e->arena_.base = static_cast<unsigned char *>(base);
+    e->arena_.size = bytes;
+    for (DevBuf *b : e->all_buffers()) b->arena = &e->arena_;
+    if (!e->init(err)) return nullptr;
+    r = e;
+    return r;
 }
 
 std::shared_ptr<Renderer> Renderer::shared(std::string &err) {

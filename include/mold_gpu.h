@@ -231,6 +231,26 @@ MG_EXPORT mg_status mg_render_device(mg_composition *comp, uint32_t width, uint3
                                      const float background[4], const mg_rect *crop,
                                      void **device_rgba);
 
+/* mg_render_device for callers that cannot allocate device memory (tight
+   serving loops, harnesses that forbid hipMalloc). No counterpart in the
+   reference. Renders `comp` into the
+   caller's device frame `device_rgba` (`device_bytes` >= width * height * 4;
+   stride = width; tiles outside `crop` keep their contents) and calls no
+   hipMalloc, hipFree or allocating library routine: every buffer of the
+   frame comes from a static device array of 64 MiB per device. The
+   composition's lines are read in place when they are resident on the device
+   and current (an earlier mg_render* call uploaded them), else copied into
+   that array. Frames that need more than it holds (roughly: 18 bytes per
+   line, 32 bytes per line-pixel, the layer and props tables and about 1.3
+   KB per tile) return MG_ERR_CAPACITY. Blocking: the frame is complete on
+   return. The array keeps the last call's contents; calls serialise on one
+   engine per device (calls that overlapped on one GPU would race on the
+   array). The first call on a device looks up the array's address and
+   creates the engine's stream and events. */
+MG_EXPORT mg_status mg_render_device_noalloc(mg_composition *comp, uint32_t width, uint32_t height,
+                                             const float background[4], const mg_rect *crop, void *device_rgba,
+                                             size_t device_bytes);
+
 /* mg_render_device through a captured hipGraph, for animations: while the
    composition's lines, the canvas and the layer count stay the same, each
    frame uploads its layer tables (transforms, orders, props) and replays the
