@@ -32,6 +32,9 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frames/s & pixel-segments/s batched vs plain on 1 MI355X, bit-exact; p99 ms"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
+# vector-ALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per
+# 4 cycles per SIMD (16 lanes), 2.4 GHz
+VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
 
 
 # kernel -> stats field with its per-frame time (hipEvents on the library's
@@ -59,25 +62,91 @@ def kernel_bytes(st, width, height, n_layers):
     }
 
 
-def cpu_baseline(scene, fraction, threads):
-    """The oracle's multi-threaded C restatement on a bounded sample of the
-    same scene: the lines of the first `fraction` of its layers."""
+def host_cpu():
+    """Host CPU facts for the baseline: logical CPUs, the CPUs this process may
+    run on, the cgroup CPU quota, and the model name."""
+    nproc = os.cpu_count() or 1
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = nproc
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = float(q) / float(period)
+    except (OSError, ValueError):
+        pass
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"nproc": nproc, "affinity": affinity, "cgroup_quota": quota, "model": model}
+
+
+def baseline_threads(info, share=16):
+    """Worker threads of the multi-threaded baseline: the CPUs this process
+    may use, bounded by the cgroup quota and by one GPU's share of the host
+    (`share` CPUs on the GPU box, whatever nproc shows)."""
+    n = info["affinity"]
+    if info["cgroup_quota"]:
+        n = min(n, max(1, int(info["cgroup_quota"])))
+    return max(1, min(n, share))
+
+
+def cpu_baseline(circles_scene, frame_scene, gpu_frame, runs=5):
+    """The CPU baseline and the headline frame's parity check, both on the
+    oracle (the C restatement of the reference CPU path, kind "port"):
+    - the plain 1024x1024 circles frame rendered single-threaded and on
+      `baseline_threads` threads, median of `runs` timed renders after one
+      warm-up each (render() from the inserted composition to RGBA8);
+    - the large frame rendered once on the same threads, compared byte for
+      byte with the GPU's timed frame (`gpu_frame`)."""
     from oracle import mold
-    from cassia_amd.workloads import Scene
-    n_layers = max(1, int(scene.n_layers * fraction))
-    keep = scene.layer_ids < n_layers
-    sample = Scene(scene.width, scene.height, scene.lines[keep], scene.layer_ids[keep],
-                   scene.props[:n_layers], scene.background, name=scene.name + "-sample")
-    comp = sample.build(mold)
-    buf = np.zeros((scene.width * scene.height, 4), np.uint8)
+    info = host_cpu()
+    threads = baseline_threads(info)
+    sc = circles_scene
+    comp = sc.build(mold)
+    res = {}
+    for label, th in (("single", 1), ("multi", threads)):
+        times, segs = [], 0
+        for r in range(runs + 1):
+            buf = np.zeros((sc.width * sc.height, 4), np.uint8)
+            t0 = time.perf_counter()
+            segs = comp.render_mt(mold.Buffer(buf, sc.width), sc.background, th)
+            dt = time.perf_counter() - t0
+            if r > 0:
+                times.append(dt)
+        med = float(np.median(times))
+        res[label] = {"threads": th, "median_s": med, "runs_s": [round(t, 4) for t in times],
+                      "frames_per_s": 1.0 / med, "pixel_segments_per_s": segs / med, "pixel_segments": int(segs)}
+        progress(f"cpu baseline {label} ({th} threads): {med * 1e3:.1f} ms per circles frame")
+    fs = frame_scene
+    fc = fs.build(mold)
+    ref = np.zeros((fs.width * fs.height, 4), np.uint8)
     t0 = time.perf_counter()
-    segs = comp.render_mt(mold.Buffer(buf, scene.width), scene.background, threads)
-    dt = time.perf_counter() - t0
-    return dict(value=segs / dt, unit="pixel-segments/s", cores=threads, kind="port",
-                sample=f"first {n_layers} of {scene.n_layers} layers of the same scene "
-                       f"({int(keep.sum())} lines, {segs} pixel segments, {dt:.2f} s, "
-                       f"{threads} threads, C restatement of the reference CPU path)",
-                seconds=dt, frames_per_s=1.0 / dt)
+    fsegs = fc.render_mt(mold.Buffer(ref, fs.width), fs.background, threads)
+    fdt = time.perf_counter() - t0
+    same = bool(np.array_equal(ref, gpu_frame))
+    diff = 0 if same else int((ref != gpu_frame).any(axis=1).sum())
+    progress(f"large frame on the CPU: {fdt:.2f} s, bit-identical to the GPU frame: {same}")
+    mt = res["multi"]
+    return {
+        "value": mt["pixel_segments_per_s"], "unit": "pixel-segments/s", "cores": threads, "kind": "port",
+        "sample": (f"plain 1024x1024 frame of 10,000 circles x 64 lines ({mt['pixel_segments']} pixel segments), "
+                   f"median of {runs} renders after a warm-up, {threads} threads; C restatement of the "
+                   f"reference CPU path (oracle/oracle.c)"),
+        "frames_per_s": mt["frames_per_s"],
+        "single_thread": res["single"], "multi_thread": mt,
+        "host": info,
+        "large_frame": {"seconds": fdt, "threads": threads, "pixel_segments": int(fsegs),
+                        "pixel_segments_per_s": fsegs / fdt, "bit_identical_to_gpu": same,
+                        "pixels_differing": diff},
+    }
 
 
 def progress(msg):
@@ -241,17 +310,25 @@ def bench_serving(cassia_amd, workloads, n_requests=160, fractions=(0.25, 0.5, 0
         done, el = run_stream(list(arrivals), picks)
         st = srv.stats(reset=True)
         lat = np.array([d[1] for d in done])
+        side = np.array([scenes[i].width for i in picks])
+        per_class = {}
+        for c in sorted(set(side.tolist())):
+            lc = lat[side == c]
+            per_class[f"{c}x{c}"] = {"requests": int(len(lc)), "p50_ms": float(np.percentile(lc, 50)),
+                                     "p99_ms": float(np.percentile(lc, 99)), "max_ms": float(lc.max())}
         rates.append({"arrival_rate": lam, "requests": n_requests, "requests_per_s": n_requests / el,
                       "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
                       "max_ms": float(lat.max()), "mean_batch": st.mean_batch,
                       "max_concurrent_batches": int(st.max_concurrent_batches),
+                      "per_canvas": per_class,
                       "identical": all(d[0] for d in done)})
         progress(f"serving at {lam:.0f}/s: p50 {rates[-1]['p50_ms']:.2f} ms, p99 {rates[-1]['p99_ms']:.2f} ms")
     srv.close()
     sides = [s.width for s in scenes]
     return {
-        "workload": ("Poisson request streams; request scene = canvas {64,256,1024,2048}^2 w.p. .5/.3/.15/.05, "
-                     "log-uniform 1e2..1e6 lines generated as the large frame's polygons; pool of 32 scenes"),
+        "workload": ("Poisson request streams; request scene = canvas {64,256,1024,2048}^2 in proportion "
+                     ".5/.3/.15/.05, log-uniform 1e2..1e6 lines generated as the large frame's polygons; "
+                     "pool of 32 scenes, each request one of them drawn uniformly"),
         "pool_lines": int(sum(len(s.lines) for s in scenes)),
         "pool_canvases": {str(c): sides.count(c) for c in sorted(set(sides))},
         "pool_alone_device_ms": round(sum(a["device_ms"] for a in alone), 3),
@@ -363,6 +440,39 @@ def measured_traffic(workload):
     return {}
 
 
+def measured_valu(workload):
+    """Wave-level VALU instructions per launch per kernel from the committed
+    rocprofv3 SQ counter summary of this workload, or {}."""
+    path = os.path.join(ROOT, "profiles", f"sq-instructions_{workload}.md")
+    if not os.path.exists(path):
+        return {}
+    for line in open(path):
+        if line.startswith("valu_per_launch:"):
+            return json.loads(line.split(":", 1)[1])
+    return {}
+
+
+def bench_circles(cassia_amd, workloads, steps, warmup):
+    """The plain configuration on the GPU: 1024x1024, 10,000 circles x 64
+    lines, one frame per step into a device-resident frame."""
+    sc = workloads.circles()
+    comp = sc.build(cassia_amd)
+    for _ in range(warmup):
+        comp.render_device(sc.width, sc.height, sc.background)
+    dev = 0.0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        comp.render_device(sc.width, sc.height, sc.background)
+        dev += comp.stats().device_ms
+    el = time.perf_counter() - t0
+    st = comp.stats()
+    progress(f"circles: {el * 1e3 / steps:.3f} ms per frame")
+    return {"workload": "plain method: 1024x1024, 10,000 circles x 64 lines, opaque, NonZero, Over",
+            "frames_per_s": steps / el, "ms_per_frame": el * 1e3 / steps, "device_ms_per_frame": dev / steps,
+            "pixel_segments": int(st.segments), "pixel_segments_per_s": st.segments * steps / el,
+            "lines": int(st.lines)}, sc
+
+
 def copy_calibration(nbytes):
     """One fill and one device-to-device copy of `nbytes`: the reference
     kernels whose FETCH_SIZE / WRITE_SIZE counters scale the renderer's (the
@@ -393,9 +503,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", choices=["polygons", "circles"], default="polygons")
     ap.add_argument("--paths", type=int, default=200_000)
-    ap.add_argument("--cpu-fraction", type=float, default=0.25)
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true",
+                    help="skip the CPU baseline and the oracle check of the large frame")
     ap.add_argument("--no-batch", action="store_true", help="skip the batched many-scenes measurement")
     ap.add_argument("--batch-scenes", type=int, default=4096)
     ap.add_argument("--no-frame", action="store_true",
@@ -458,10 +567,11 @@ def main():
     lat = []
     kms = {k: 0.0 for k in KERNELS}
     dev_ms = 0.0
+    addr = None
     t0 = time.perf_counter()
     for _ in range(args.steps):
         a = time.perf_counter()
-        comp.render_device(W, H, scene.background)
+        addr = comp.render_device(W, H, scene.background)
         lat.append((time.perf_counter() - a) * 1e3)
         st = comp.stats()
         for k, f in KERNELS.items():
@@ -469,6 +579,8 @@ def main():
         dev_ms += st.device_ms
     elapsed = time.perf_counter() - t0
     progress(f"{args.steps} frames in {elapsed:.2f} s")
+    # the last timed frame, for the oracle check in the CPU-baseline leg
+    gpu_frame = cassia_amd.device_frame(addr, W, H) if args.workload == "polygons" else None
     st = comp.stats()
     K = args.steps
     kms = {k: v / K for k, v in kms.items()}
@@ -483,12 +595,27 @@ def main():
         kernels[k] = {"ms_per_frame": round(ms, 4), "launches": n, "ms_per_launch": round(per_launch_ms, 4),
                       "bytes_per_launch": b,
                       "gbs": round(b / (per_launch_ms * 1e-3) / 1e9, 2) if per_launch_ms > 0 else None}
+    valu = measured_valu(profile_name) if profile_name else {}
+    for k, v in kernels.items():
+        v["hbm_frac"] = round(v["gbs"] / HBM_PEAK_GBS, 4) if v["gbs"] else None
+        if k in valu and v["ms_per_launch"] > 0:
+            v["valu_frac"] = round(valu[k] / (v["ms_per_launch"] * 1e-3) / VALU_PEAK_PER_S, 4)
     dominant = max(kms, key=kms.get)
     d = kernels[dominant]
     traffic = measured_traffic(profile_name).get(dominant) if profile_name else None
+    hbm_frac = d["gbs"] / HBM_PEAK_GBS
     roofline = {"bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(d["gbs"] / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": dominant,
+                "frac": round(hbm_frac, 4), "traffic": traffic, "kernel": dominant,
                 "ms_per_launch": d["ms_per_launch"], "bytes_per_launch": d["bytes_per_launch"]}
+    if dominant in valu:
+        # vector-ALU issue: the kernel's wave-level VALU instructions (SQ
+        # counters of the committed profile) per second of its measured time
+        rate = valu[dominant] / (d["ms_per_launch"] * 1e-3)
+        roofline["valu"] = {"achieved": rate, "peak": VALU_PEAK_PER_S, "unit": "wave64 VALU instructions/s",
+                            "frac": round(rate / VALU_PEAK_PER_S, 4), "instructions_per_launch": valu[dominant],
+                            "source": f"profiles/sq-instructions_{profile_name}.md"}
+        if rate / VALU_PEAK_PER_S > hbm_frac:
+            roofline["bound"] = "valu"
     out = {
         "metric": METRIC,
         "value": segs * fps,
@@ -530,8 +657,9 @@ def main():
         comp.render(cassia_amd.Buffer(hb, W), scene.background, None)
         host_ms.append((time.perf_counter() - a) * 1e3)
     out["host_buffer_ms_per_frame"] = round(float(np.median(host_ms)), 3)
+    del comp
+    out["circles"], circles_scene = bench_circles(cassia_amd, workloads, args.steps, args.warmup)
     if not args.no_batch:
-        del comp
         out["batched"] = bench_batched(cassia_amd, workloads, args.steps, args.warmup, args.batch_scenes)
     if not args.no_anim:
         out["animation"] = bench_animation(cassia_amd, workloads)
@@ -539,10 +667,12 @@ def main():
         out["flatten"] = bench_flatten(cassia_amd)
     if not args.no_serve:
         out["serving"] = bench_serving(cassia_amd, workloads)
-    if not args.no_cpu:
-        cb = cpu_baseline(scene, args.cpu_fraction, args.cpu_threads)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+    if not args.no_cpu and gpu_frame is not None:
+        cb = cpu_baseline(circles_scene, scene, gpu_frame)
+        out["bit_identical"] = cb["large_frame"]["bit_identical_to_gpu"]
+        out["cpu_baseline"] = cb
     else:
+        out["bit_identical"] = None
         out["cpu_baseline"] = None
     print(json.dumps(out))
 

@@ -830,8 +830,15 @@ MG_EXPORT mg_status mg_render(mg_composition *c, const mg_buffer *buf, const flo
     const size_t bytes = len * 4;
     uint8_t *img = c->frame_image(bytes);
     if (!img) return fail(MG_ERR_OOM, "out of device memory (frame)");
-    // the caller's pixels go first: tiles the painter skips keep them
-    if (hipMemcpyAsync(img, buf->rgba, bytes, hipMemcpyHostToDevice, c->rdr->stream()) != hipSuccess)
+    // The caller's pixels go first where the frame leaves some of them as
+    // they are: tiles the layer cache skips, tiles outside the crop, and
+    // pixels beyond `width` in each row (stride padding, a longer last row).
+    // Without those, the painter writes every pixel and nothing is uploaded.
+    const mg::SceneDesc cover = scene_desc(width, stride, rows_px, bg, crop);
+    const bool writes_all = cache_id < 0 && stride == width && len == rows_px * width && cover.crop_h0 == 0 &&
+                            cover.crop_v0 == 0 && cover.crop_h1 == row_len && cover.crop_v1 == tile_rows;
+    if (!writes_all &&
+        hipMemcpyAsync(img, buf->rgba, bytes, hipMemcpyHostToDevice, c->rdr->stream()) != hipSuccess)
         return fail(MG_ERR_DEVICE, "upload of the frame buffer failed");
     mg::FrameResult r;
     if ((s = render_frame(c, width, stride, rows_px, bg, crop, cache_id, prev, img, bytes, buf->flusher != nullptr, r)))

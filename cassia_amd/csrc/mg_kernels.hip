@@ -1705,7 +1705,7 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
         if (col > h1) return true;
         const uint32_t nr = nr_l[col];
         if (nr > rcap) {
-            max_runs = nr > max_runs ? nr : max_runs;
+            max_runs = (uint32_t)__builtin_amdgcn_readfirstlane((int)nr);
             if (tid == 0) atomicOr(pp.overflow, 1u);
             publish_max();
             return false;
@@ -1729,8 +1729,13 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
     auto step = [&](uint32_t colp, uint32_t buf) -> bool {
         const bool fold = colp <= h0;
         const uint32_t nr = nr_l[colp];
-        max_runs = nr > max_runs ? nr : max_runs;
-        max_ids = nq + nr > max_ids ? nq + nr : max_ids;
+        {
+            // uniform over the work group: kept in scalar registers
+            const uint32_t r = (uint32_t)__builtin_amdgcn_readfirstlane((int)nr);
+            const uint32_t t = (uint32_t)__builtin_amdgcn_readfirstlane((int)(nq + nr));
+            max_runs = r > max_runs ? r : max_runs;
+            max_ids = t > max_ids ? t : max_ids;
+        }
         if (nq + nr > cap) {
             if (tid == 0) atomicOr(pp.overflow, 1u);
             publish_max();

@@ -251,17 +251,35 @@ def glyph_batch(seed=3, n_scenes=4096, gamma=3.0, min_side=32, max_side=512, pat
 SERVE_CANVASES = ((64, 0.5), (256, 0.3), (1024, 0.15), (2048, 0.05))
 
 
+def class_counts(n, probs):
+    """`n` items split over classes in proportion to `probs` (largest
+    remainder), every class at least once when n allows it."""
+    probs = np.asarray(probs, dtype=np.float64) / np.sum(probs)
+    exact = n * probs
+    counts = np.floor(exact).astype(np.int64)
+    for k in np.argsort(-(exact - counts), kind="stable")[: n - int(counts.sum())]:
+        counts[k] += 1
+    if n >= len(probs):
+        while (counts == 0).any():
+            counts[int(np.argmin(counts))] += 1
+            counts[int(np.argmax(counts))] -= 1
+    return counts
+
+
 def serve_pool(seed=5, n=32, lines=(1e2, 1e6), canvases=SERVE_CANVASES):
-    """Request scenes of the serving workload: square canvas drawn from
-    `canvases` (side, probability), line count log-uniform in `lines`,
-    generated as the single large frame's polygons (5-50 edges, box 4-512 px,
-    affine, mixed fill rules, blend modes and alpha) on that canvas."""
+    """Request scenes of the serving workload: square canvases of the sides
+    in `canvases` (side, probability) in proportion to the probabilities
+    (every side present, class_counts), in seeded random order; line count
+    log-uniform in `lines`, generated as the single large frame's polygons
+    (5-50 edges, box 4-512 px, affine, mixed fill rules, blend modes and
+    alpha) on that canvas."""
     rng = np.random.default_rng(seed)
-    sides = np.array([c for c, _ in canvases])
-    probs = np.array([p for _, p in canvases], dtype=np.float64)
+    counts = class_counts(n, [p for _, p in canvases])
+    sides = np.repeat(np.array([c for c, _ in canvases]), counts)
+    rng.shuffle(sides)
     out = []
     for i in range(n):
-        side = int(rng.choice(sides, p=probs / probs.sum()))
+        side = int(sides[i])
         n_lines = float(np.exp(rng.uniform(np.log(lines[0]), np.log(lines[1]))))
         count = max(1, int(round(n_lines / 27.5)))  # 27.5 edges per path on average
         out.append(polygons(seed=seed * 1000 + i, width=side, height=side, count=count))
