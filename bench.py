@@ -230,7 +230,11 @@ def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes, plain=True):
         "radix_bits": int(st.radix_bits),
         "sort_passes": int(st.sort_passes),
         "autotune": {"candidates": {int(tn.radix_bits[k]): round(tn.device_ms[k], 3) for k in range(tn.n_candidates)},
-                     "chosen": int(tn.chosen_radix_bits), "seconds": round(tune_s, 3)},
+                     "chosen": int(tn.chosen_radix_bits),
+                     "walk_candidates": {int(tn.walk_threads[k]): round(tn.walk_ms[k], 3)
+                                         for k in range(tn.n_walk_candidates)},
+                     "chosen_walk_threads": int(tn.chosen_walk_threads), "seconds": round(tune_s, 3)},
+        "walk_threads": int(st.walk_threads), "walk_lds": int(st.walk_lds),
         "stage_ms": {k: round(getattr(st, k), 3) for k in ("setup_ms", "emit_ms", "hist_ms", "sort_ms", "runs_ms",
                                                              "walk_ms", "paint_ms")},
         "scene_build_s": round(t_build, 2),

@@ -54,7 +54,8 @@ struct PaintParams {
     uint2 *rec_hdr;
     uint32_t *rec_used;
     uint64_t rec_cap;
-    uint32_t *n_runs, *n_records;  // statistics: runs and records of the frame
+    uint32_t *n_runs;            // statistics: runs of the frame
+    uint32_t *n_records;         // statistics: records of the frame, in kStatSlices counters
     const uint2 *work;           // (global row, tile column) per paint work group
 };
 
@@ -64,7 +65,7 @@ __global__ void k_setup(const float4 *pts, const uint16_t *ids, const LayerEntry
                         const SceneDesc *scenes, const uint32_t *scene_line_base, FrameDims dims,
                         LineSetup *out, uint32_t *lengths, uint32_t *err, uint32_t *bucket_hist);
 __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *status,
-                           uint32_t *stall, uint64_t *total);
+                           uint32_t *stall, uint64_t *total, uint32_t spread_rows);
 // Per-partition slot counts written by the emission: pass 0's digit counts
 // (digit-major, one column per partition) and slices of the other totals.
 struct SlotCounts {
@@ -77,8 +78,8 @@ __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const Sc
                        FrameDims dims, const FrameCounters *counters, uint64_t *out, const uint32_t *part_line,
                        SlotCounts sc);
 __global__ void k_order_scatter(const LineSetup *setup, const uint32_t *lengths, uint64_t n_lines,
-                                const SceneDesc *scenes, uint32_t *base, LineSetup *setup_perm,
-                                uint32_t *lengths_perm);
+                                const SceneDesc *scenes, uint32_t *base, uint32_t bucket_rows,
+                                LineSetup *setup_perm, uint32_t *lengths_perm);
 __global__ void k_path_counts(const PathCommand *cmds, const flat::PathDesc *paths, uint32_t n_paths,
                               uint32_t *counts, uint32_t *err);
 __global__ void k_path_emit(const PathCommand *cmds, const flat::PathDesc *paths, uint32_t n_paths,
@@ -112,6 +113,7 @@ size_t paint_deep_lds_bytes_host();
 size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride, uint32_t run_bufs);
 size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 
+constexpr uint32_t kStatSlices = 32;
 constexpr uint32_t kEmitTileHost = 2048;
 constexpr uint32_t kSortTileHost = 4096;
 constexpr uint32_t kScanTileHost = 2048;

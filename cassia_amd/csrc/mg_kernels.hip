@@ -301,7 +301,7 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
     }
     out[i] = r;
     lengths[i] = length;
-    bucket = line_bucket(S, r.packed);
+    bucket = bucket_slot(line_bucket(S, r.packed), dims.bucket_rows);
     }
     if (bucket_hist) bucket_count(valid, bucket, bucket_hist);
 }
@@ -350,9 +350,10 @@ __global__ __launch_bounds__(256) void k_path_emit(const PathCommand *__restrict
 
 // Single-pass exclusive scan of u32 values (decoupled look-back), 2048
 // values per partition. out may alias in. total receives the sum (u64).
+// Element e lives at bucket_slot(e, spread_rows) (spread_rows 0: at e).
 __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n,
                                                   uint64_t *status, uint32_t *stall,
-                                                  uint64_t *total) {
+                                                  uint64_t *total, uint32_t spread_rows) {
     __shared__ uint32_t wsum[4];
     __shared__ uint64_t s_excl;
     constexpr uint32_t IPT = 8;
@@ -360,7 +361,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
     const uint64_t base = (uint64_t)part * 256 * IPT + (uint64_t)threadIdx.x * IPT;
     uint32_t v[IPT], sum = 0;
     for (uint32_t k = 0; k < IPT; ++k) {
-        v[k] = base + k < n ? in[base + k] : 0u;
+        v[k] = base + k < n ? in[bucket_slot((uint32_t)(base + k), spread_rows)] : 0u;
         sum += v[k];
     }
     uint32_t agg;
@@ -377,7 +378,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
     __syncthreads();
     uint64_t run = s_excl + ex;
     for (uint32_t k = 0; k < IPT; ++k) {
-        if (base + k < n) out[base + k] = (uint32_t)run;
+        if (base + k < n) out[bucket_slot((uint32_t)(base + k), spread_rows)] = (uint32_t)run;
         run += v[k];
     }
 }
@@ -774,14 +775,16 @@ __global__ __launch_bounds__(256) void k_count_reduce(const uint32_t *__restrict
 __global__ __launch_bounds__(256) void k_order_scatter(const LineSetup *__restrict__ setup,
                                                        const uint32_t *__restrict__ lengths, uint64_t n_lines,
                                                        const SceneDesc *__restrict__ scenes,
-                                                       uint32_t *__restrict__ base,
+                                                       uint32_t *__restrict__ base, uint32_t bucket_rows,
                                                        LineSetup *__restrict__ setup_perm,
                                                        uint32_t *__restrict__ lengths_perm) {
     const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     const bool valid = i < n_lines;
     LineSetup L;
     if (valid) L = setup[i];
-    const uint32_t order = valid ? line_bucket(scenes[L.packed >> 19], L.packed) : 0xFFFFFFFFu;
+    // the bucket's counter slot (base holds each bucket's next position there)
+    const uint32_t order = valid ? bucket_slot(line_bucket(scenes[L.packed >> 19], L.packed), bucket_rows)
+                                 : 0xFFFFFFFFu;
     uint64_t heads;
     const uint32_t head = run_head_lane(valid, order, &heads);
     const uint32_t lane = lane_id();
@@ -1692,13 +1695,17 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
     };
     // The runs of column col into run arrays `buf`; false when they exceed
     // the arrays (the work group must stop).
-    // largest runs of a tile and queue + runs met by this row (thread 0),
-    // published at the row's end or when it stops
+    // largest runs of a tile and queue + runs met by this row, published at
+    // the row's end or when it stops; only the large configuration (two sets
+    // of run arrays) reads them, and a row publishes only what raises them
+    // (frames of many rows would otherwise queue on the two addresses)
     uint32_t max_runs = 0, max_ids = 0;
     auto publish_max = [&]() {
-        if (tid == 0) {
-            atomicMax(pp.walk_max, max_runs);
-            atomicMax(pp.walk_max + 1, max_ids);
+        if (tid == 0 && pp.walk_run_bufs == 2) {
+            if (max_runs > __hip_atomic_load(pp.walk_max, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(pp.walk_max, max_runs);
+            if (max_ids > __hip_atomic_load(pp.walk_max + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT))
+                atomicMax(pp.walk_max + 1, max_ids);
         }
     };
     auto store = [&](const Prefetch &f, uint32_t col, uint32_t buf) -> bool {
@@ -1923,7 +1930,9 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
         c1 = c3;
         c2 = c4;
     }
-    if (tid == 0 && row_records) atomicAdd(pp.n_records, row_records);
+    // (one of kStatSlices counters per row: rows of a large batch would
+    // queue on one address)
+    if (tid == 0 && row_records) atomicAdd(pp.n_records + (blockIdx.x % kStatSlices), row_records);
     publish_max();
 }
 template __global__ void k_row_walk<64, false>(PaintParams);

@@ -8,6 +8,11 @@
 
 namespace mg {
 
+// words of the paint flags buffer: overflow flags, record store use, run
+// count, deep tiles and the walk's maxima at 0..6, record counter slices
+// from kStatSlices on
+constexpr uint32_t kOverflowWords = 2 * kStatSlices;
+
 #define MG_HIP(call, what)                                                  \
     do {                                                                    \
         hipError_t e_ = (call);                                             \
@@ -137,7 +142,7 @@ bool Renderer::init(std::string &err) {
     MG_HIP(hipEventCreate(&ev0_), "hipEventCreate");
     MG_HIP(hipEventCreate(&ev1_), "hipEventCreate");
     for (hipEvent_t &e : evs_) MG_HIP(hipEventCreate(&e), "hipEventCreate");
-    if (!counters_.reserve(sizeof(FrameCounters)) || !err_.reserve(16) || !overflow_.reserve(64) ||
+    if (!counters_.reserve(sizeof(FrameCounters)) || !err_.reserve(16) || !overflow_.reserve(kOverflowWords * 4) ||
         !part_counters_.reserve(64 * sizeof(uint32_t)) || !cov_count_.reserve(16)) {
         err = "out of device memory";
         return false;
@@ -157,7 +162,7 @@ void Renderer::arena_begin() {
     rec_cap_ = 0;
     last_sorted_ = nullptr;
     // the frame's fixed counters (their contents are cleared by every frame)
-    (void)(counters_.reserve(sizeof(FrameCounters)) && err_.reserve(16) && overflow_.reserve(64) &&
+    (void)(counters_.reserve(sizeof(FrameCounters)) && err_.reserve(16) && overflow_.reserve(kOverflowWords * 4) &&
            part_counters_.reserve(64 * sizeof(uint32_t)) && cov_count_.reserve(16));
 }
 
@@ -470,13 +475,17 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         // the bucket counts taken by the setup kernel
         const uint64_t n_buckets = std::max<size_t>(spec.props.size(), 1);
         const uint64_t oparts = (n_buckets + kScanTileHost - 1) / kScanTileHost;
+        // bucket counters in the interleaved layout of bucket_slot
+        const uint32_t brows = (uint32_t)((n_buckets + kBucketSpread - 1) / kBucketSpread);
+        const uint64_t n_slots = (uint64_t)brows * kBucketSpread;
+        dims.bucket_rows = brows;
         if (order_lines) {
             if (!setup_perm_.reserve(L * sizeof(LineSetup)) || !lengths_perm_.reserve(L * 4) ||
-                !order_hist_.reserve(n_buckets * 4) || !order_status_.reserve(oparts * 8)) {
+                !order_hist_.reserve(n_slots * 4) || !order_status_.reserve(oparts * 8)) {
                 err = "out of device memory (line order)";
                 return false;
             }
-            fill32(order_hist_.as<uint32_t>(), 0, n_buckets, stream_);
+            fill32(order_hist_.as<uint32_t>(), 0, n_slots, stream_);
             fill64(order_status_.as<uint64_t>(), 0, oparts, stream_);
         }
         k_setup<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
@@ -489,15 +498,15 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         if (order_lines) {
             k_scan_u32<<<(uint32_t)oparts, 256, 0, stream_>>>(order_hist_.as<uint32_t>(), order_hist_.as<uint32_t>(),
                                                             n_buckets, order_status_.as<uint64_t>(), stall,
-                                                            &ctr->order_total);
+                                                            &ctr->order_total, brows);
             k_order_scatter<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
                 setup_.as<LineSetup>(), lengths_.as<uint32_t>(), L, scenes_.as<SceneDesc>(),
-                order_hist_.as<uint32_t>(), setup_perm_.as<LineSetup>(), lengths_perm_.as<uint32_t>());
+                order_hist_.as<uint32_t>(), brows, setup_perm_.as<LineSetup>(), lengths_perm_.as<uint32_t>());
             scan_in = lengths_perm_.as<uint32_t>();
         }
         fill64(scan_status_.as<uint64_t>(), 0, nparts, stream_);
         k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(scan_in, offsets_.as<uint32_t>(), L,
-                                                        scan_status_.as<uint64_t>(), stall, &ctr->line_pixels);
+                                                        scan_status_.as<uint64_t>(), stall, &ctr->line_pixels, 0u);
     }
     MG_HIP(hipGetLastError(), "setup/scan launch");
     FrameCounters hc{};
@@ -574,7 +583,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         k_count_reduce<<<rest_rows, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), dims, hist_.as<uint32_t>(), ctr);
         fill64(cnt_status_.as<uint64_t>(), 0, cnt_parts, stream_);
         k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
-                                                           cnt_status_.as<uint64_t>(), stall, &ctr->order_total);
+                                                           cnt_status_.as<uint64_t>(), stall, &ctr->order_total, 0u);
     }
     k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
     // look-back status words of every digit pass, cleared outside the sort's timing
@@ -709,7 +718,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.cov_cap = spec.coverage_cap;
     }
     pp.n_runs = overflow_.as<uint32_t>() + 3;
-    pp.n_records = overflow_.as<uint32_t>() + 2;
+    pp.n_records = overflow_.as<uint32_t>() + kStatSlices;
     fill32(overflow_.as<uint32_t>() + 3, 0, 1, stream_);
     // Tiles of 8,192 records or more go to the four-wave deep painter (at most
     // two per CU; more stay with the one-wave painter). A tile holds one wave
@@ -837,6 +846,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
         fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
         fill32(overflow_.as<uint32_t>() + 4, 0, 3, stream_);
+        fill32(overflow_.as<uint32_t>() + kStatSlices, 0, kStatSlices, stream_);
         MG_HIP(mark(evs_[5]), "event");
         const size_t lds = global_walk ? 0 : lds_w;
         if (walk_threads == 64)
@@ -857,13 +867,14 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         MG_HIP(mark(evs_[7]), "event");
         MG_HIP(hipGetLastError(), "paint launch");
         if (capture_P_) return true;  // one attempt; the replay reads the flags
-        uint32_t ov[7] = {0, 0, 0, 0, 0, 0, 0};
+        uint32_t ov[kOverflowWords] = {};
         MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, sizeof ov, hipMemcpyDeviceToHost, stream_), "read overflow");
         MG_HIP(hipStreamSynchronize(stream_), "paint");
         res.walk_max_runs = ov[5];
         res.walk_max_ids = ov[6];
         if (!ov[0]) {
-            n_records_ = ov[2];
+            n_records_ = 0;
+            for (uint32_t k = 0; k < kStatSlices; ++k) n_records_ += ov[kStatSlices + k];
             n_runs_ = ov[3];
             n_deep_ = std::min(ov[4], pp.deep_cap);
             return true;
@@ -994,10 +1005,10 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
             MG_HIP(hipGraphLaunch(graph_exec_, stream_), "graph launch");
             MG_HIP(hipEventRecord(ev1_, stream_), "event");
             FrameCounters hc{};
-            uint32_t herr[3] = {0, 0, 0}, ov[4] = {0, 0, 0, 0};
+            uint32_t herr[3] = {0, 0, 0}, ov[kOverflowWords] = {};
             MG_HIP(hipMemcpyAsync(&hc, counters_.ptr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
             MG_HIP(hipMemcpyAsync(herr, err_.ptr, 12, hipMemcpyDeviceToHost, stream_), "read flags");
-            MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, 16, hipMemcpyDeviceToHost, stream_), "read overflow");
+            MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, sizeof ov, hipMemcpyDeviceToHost, stream_), "read overflow");
             MG_HIP(hipStreamSynchronize(stream_), "graph frame");
             if (!herr[0] && !herr[1] && !herr[2] && !ov[0]) {
                 float ms = 0.0f;
@@ -1007,7 +1018,8 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
                 res.segments = hc.segments;
                 res.kept = hc.kept;
                 res.device_ms = ms;
-                res.records = ov[2];
+                res.records = 0;
+                for (uint32_t k = 0; k < kStatSlices; ++k) res.records += ov[kStatSlices + k];
                 res.runs = ov[3];
                 for (double &v : res.stage_ms) v = 0.0;
                 replayed = true;
@@ -1097,7 +1109,7 @@ bool Renderer::flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat:
                                              (uint32_t)n_paths, flat_counts_.as<uint32_t>(), err_.as<uint32_t>());
     k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(flat_counts_.as<uint32_t>(), flat_offsets_.as<uint32_t>(),
                                                     n_paths, flat_status_.as<uint64_t>(), err_.as<uint32_t>() + 1,
-                                                    &ctr->line_pixels);
+                                                    &ctr->line_pixels, 0u);
     MG_HIP(hipGetLastError(), "flatten count launch");
     uint64_t total = 0;
     uint32_t herr[2] = {0, 0};

@@ -86,7 +86,18 @@ struct FrameDims {
     uint32_t radix_bits, passes;
     uint32_t mode;        // 0: render (cull + render key), 1: canonical full stream, 2: ingest
     uint32_t sort_shift;  // bit of the first sort digit
+    uint32_t bucket_rows; // line-order bucket counters: rows of the interleaved layout (bucket_slot)
 };
+
+// Counter slot of line-order bucket b: the counters are laid out as a
+// 32 x rows matrix (bucket b in column b / 32 of row b mod 32), so that
+// consecutive buckets, which concurrent waves count and place at once (the
+// lines of consecutive paths), fall in different 128-byte lines instead of
+// contending for the same few.
+constexpr uint32_t kBucketSpread = 32;
+__host__ __device__ inline uint32_t bucket_slot(uint32_t b, uint32_t rows) {
+    return rows ? (b % kBucketSpread) * rows + b / kBucketSpread : b;
+}
 
 // Device-side counters of a frame.
 struct FrameCounters {
