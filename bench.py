@@ -647,6 +647,8 @@ def main():
         "tiles": int(st.tiles),
         "tile_runs": int(st.runs),
         "tile_records": int(st.records),
+        "walk_threads": int(st.walk_threads),
+        "walk_lds": ["share", "half", "whole", "global"][int(st.walk_lds)],
         "setup_hist_ms": round(st.setup_ms + st.hist_ms, 4),
         "kernels": kernels,
         "roofline": roofline,

@@ -3,6 +3,7 @@
 // every pixel operation runs on the GPU (mg_runtime.hip, mg_kernels.hip).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <cmath>
 #include <cstring>
@@ -229,11 +230,39 @@ struct mg_composition {
         std::string err;
         if (uploaded > ids.size()) uploaded = 0;
         if (uploaded < ids.size()) {
+            if (uploaded == 0) dlines.ranked = 0;  // the lines are uploaded anew: rank them anew
             if (hipSetDevice(rdr->device()) != hipSuccess ||
                 !dlines.upload(&pts[4 * uploaded], &ids[uploaded], uploaded, ids.size() - uploaded, rdr->stream(), err))
                 return fail(MG_ERR_OOM, err.empty() ? "hipSetDevice failed" : err);
             uploaded = ids.size();
         }
+        return MG_OK;
+    }
+
+    // Ranks of the resident lines among the lines of their layers (k_setup
+    // places line i of a layer at the layer's base + its rank), computed on
+    // the device for the lines not ranked yet: once per change of the lines.
+    // Called with the engine locked, after sync_lines.
+    mg_status ensure_ranks() {
+        std::string err;
+        const size_t n = n_lines();
+        const uint32_t n_ids = layers.empty() ? 1u : (uint32_t)layers.rbegin()->first + 1u;
+        bool reset = dlines.ranked > n;
+        if (n_ids > dlines.count_ids) {
+            if (!dlines.counts.reserve((size_t)n_ids * 4)) return fail(MG_ERR_OOM, "out of device memory (line ranks)");
+            dlines.count_ids = n_ids;
+            reset = true;
+        }
+        if (dlines.rank.bytes < n * 4) {
+            if (!dlines.rank.reserve(std::max(n, dlines.cap) * 4)) return fail(MG_ERR_OOM, "out of device memory (line ranks)");
+            reset = true;
+        }
+        if (reset) dlines.ranked = 0;
+        if (dlines.ranked == n) return MG_OK;
+        if (!rdr->rank_lines(dlines.layer_ids(), dlines.counts.as<uint32_t>(), dlines.count_ids,
+                             dlines.rank.as<uint32_t>(), dlines.ranked, n, dlines.ranked == 0, err))
+            return fail(MG_ERR_DEVICE, err);
+        dlines.ranked = n;
         return MG_OK;
     }
 
@@ -309,7 +338,24 @@ struct mg_composition {
             if (l.enabled) max_order = std::max<uint32_t>(max_order, kv.first);
         }
         f.max_order = std::max<uint32_t>(max_order, n_props - 1);
+        // where render frames place each layer's lines: enabled layers in
+        // their render order (ties by id), then disabled layers (whose lines
+        // set up with no pixels); a layer's lines start at its base
+        std::vector<std::pair<uint32_t, uint16_t>> on, off;
+        for (auto &kv : layers) {
+            const Layer &l = kv.second;
+            (l.enabled ? on : off).push_back({l.has_order ? (uint32_t)l.order : (uint32_t)kv.first, kv.first});
+        }
+        std::sort(on.begin(), on.end());
+        uint64_t base = 0;
+        for (const auto *v : {&on, &off})
+            for (const auto &p : *v) {
+                f.layers[p.second].base = (uint32_t)base;
+                base += layers[p.second].len;
+            }
+        lines_placed = base;
     }
+    size_t lines_placed = 0;  // lines the last tables placed (equals n_lines() while consistent)
 };
 
 static mg_status check_comp(const mg_composition *c) {
@@ -773,11 +819,14 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     f.walk_need_runs = c->walk_need_runs;
     f.walk_need_ids = c->walk_need_ids;
     mg_status s;
-    if ((s = c->sync_lines())) return s;
+    if ((s = c->sync_lines()) || (s = c->ensure_ranks())) return s;
+    if (c->lines_placed != c->n_lines()) return fail(MG_ERR_DEVICE, "layer line counts disagree with the lines");
+    const uint32_t *ranks = c->dlines.rank.as<uint32_t>();
     const bool ok = graph_replayed
-                        ? c->rdr->run_graph(f, c->dlines.points(), c->dlines.layer_ids(), c->n_lines(), img, r,
+                        ? c->rdr->run_graph(f, c->dlines.points(), c->dlines.layer_ids(), ranks, c->n_lines(), img, r,
                                             *graph_replayed, err)
-                        : c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->n_lines(), img, prev, r, err);
+                        : c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), ranks, c->n_lines(), img, prev, r,
+                                      err);
     if (!ok) return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
     c->walk_need_runs = r.walk_max_runs;
     c->walk_need_ids = r.walk_max_ids;
@@ -901,8 +950,10 @@ MG_EXPORT mg_status mg_render_device_noalloc(mg_composition *c, uint32_t width, 
     prepare_tables(c, -1);  // remove_disabled may compact the host lines first
     const size_t n = c->n_lines();
     rdr->arena_begin();
+    if (c->lines_placed != n) return fail(MG_ERR_DEVICE, "layer line counts disagree with the lines");
     const float4 *pts = nullptr;
     const uint16_t *ids = nullptr;
+    uint32_t *ranks = nullptr;
     const bool resident = n > 0 && c->rdr && c->rdr->device() == rdr->device() && c->uploaded == c->ids.size() &&
                           c->dlines.cap >= n;
     if (resident) {
@@ -920,6 +971,16 @@ MG_EXPORT mg_status mg_render_device_noalloc(mg_composition *c, uint32_t width, 
             return fail(MG_ERR_DEVICE, "upload of the lines failed");
         pts = static_cast<const float4 *>(p);
         ids = static_cast<const uint16_t *>(q);
+    }
+    if (n) {
+        // the lines' ranks among their layers' lines, computed in the scratch
+        const uint32_t n_ids = c->layers.empty() ? 1u : (uint32_t)c->layers.rbegin()->first + 1u;
+        uint32_t *counts = static_cast<uint32_t *>(rdr->arena_take((size_t)n_ids * 4));
+        ranks = static_cast<uint32_t *>(rdr->arena_take(n * 4));
+        if (!counts || !ranks)
+            return fail(MG_ERR_CAPACITY, "the composition's lines do not fit the no-allocation scratch (" +
+                                             std::to_string(mg::kNoAllocScratchBytes >> 20) + " MiB)");
+        if (!rdr->rank_lines(ids, counts, n_ids, ranks, 0, n, true, err)) return fail(MG_ERR_DEVICE, err);
     }
     mg::FrameSpec &f = c->spec;
     f.scenes.clear();
@@ -943,7 +1004,7 @@ MG_EXPORT mg_status mg_render_device_noalloc(mg_composition *c, uint32_t width, 
     f.walk_need_runs = c->walk_need_runs;
     f.walk_need_ids = c->walk_need_ids;
     mg::FrameResult r;
-    if (!rdr->run(f, pts, ids, n, static_cast<uint8_t *>(device_rgba), nullptr, r, err)) {
+    if (!rdr->run(f, pts, ids, ranks, n, static_cast<uint8_t *>(device_rgba), nullptr, r, err)) {
         if (rdr->arena().short_by)
             return fail(MG_ERR_CAPACITY, "the frame needs more device scratch than the no-allocation entry point's " +
                                              std::to_string(mg::kNoAllocScratchBytes >> 20) + " MiB (at least " +
@@ -1031,7 +1092,7 @@ MG_EXPORT mg_status mg_pixel_segments(mg_composition *c, const uint64_t **out, s
     if ((s = c->sync_lines())) return s;
     std::string err;
     mg::FrameResult r;
-    if (!c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), c->n_lines(), nullptr, nullptr, r, err))
+    if (!c->rdr->run(f, c->dlines.points(), c->dlines.layer_ids(), nullptr, c->n_lines(), nullptr, nullptr, r, err))
         return fail(MG_ERR_DEVICE, err);
     c->pixsegs.assign(2 * r.line_pixels, mg::kNone);
     if (!c->rdr->download_sorted(c->pixsegs.data(), r.kept, err)) return fail(MG_ERR_DEVICE, err);
@@ -1113,9 +1174,16 @@ mg_status prepare_for_batch(mg_composition *c, const mg::Renderer *held) {
     mg_status s = c->ensure_renderer();
     if (s) return s;
     prepare_tables(c, -1);
-    if (c->rdr.get() == held) return c->sync_lines();
+    if (c->lines_placed != c->n_lines()) return fail(MG_ERR_DEVICE, "layer line counts disagree with the lines");
+    if (c->rdr.get() == held) {
+        if ((s = c->sync_lines())) return s;
+        return c->ensure_ranks();  // on the batch's own stream, before its frame
+    }
     std::lock_guard<std::mutex> g(c->rdr->mu);
-    return c->sync_lines();
+    if ((s = c->sync_lines()) || (s = c->ensure_ranks())) return s;
+    // the ranks are read by another engine's stream
+    if (hipStreamSynchronize(c->rdr->stream()) != hipSuccess) return fail(MG_ERR_DEVICE, "line ranks failed");
+    return MG_OK;
 }
 
 // Packs the compositions into one frame: scene descriptors with their line,
@@ -1166,10 +1234,14 @@ mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_f
         sd.tile_base = (uint32_t)tile_base;
         sd.lines_pts = (uint64_t)(uintptr_t)c->dlines.points();
         sd.lines_ids = (uint64_t)(uintptr_t)c->dlines.layer_ids();
+        sd.lines_rank = (uint64_t)(uintptr_t)c->dlines.rank.as<uint32_t>();
         sd.flags = c->spec_clips ? mg::SF_CLIPS : 0;
         sd.out_offset = out;
         const uint32_t stops_base = (uint32_t)f.stops.size();
-        f.layers.insert(f.layers.end(), cs.layers.begin(), cs.layers.end());
+        for (mg::LayerEntry e : cs.layers) {
+            e.base += (uint32_t)line_base;  // the scene's lines follow the earlier scenes'
+            f.layers.push_back(e);
+        }
         for (mg::PropsEntry e : cs.props) {
             e.stops_offset += stops_base;
             f.props.push_back(e);
@@ -1288,7 +1360,7 @@ mg_status batch_render(mg_batch *b, mg_composition *const *comps, const mg_batch
         f.walk_threads = walk;
         f.walk_need_runs = b->walk_need_runs;
         f.walk_need_ids = b->walk_need_ids;
-        if (!b->rdr->run(f, nullptr, nullptr, b->n_lines, b->frame.as<uint8_t>(), nullptr, r, err))
+        if (!b->rdr->run(f, nullptr, nullptr, nullptr, b->n_lines, b->frame.as<uint8_t>(), nullptr, r, err))
             return fail(err.find("capacity") != std::string::npos ? MG_ERR_CAPACITY : MG_ERR_DEVICE, err);
         b->walk_need_runs = r.walk_max_runs;
         b->walk_need_ids = r.walk_max_ids;
@@ -1490,7 +1562,7 @@ MG_EXPORT void cassia_render(const uint64_t *psegments, size_t n, const CassiaSt
     f.scene_line_base.push_back(0);
     f.image_bytes = (size_t)st->width * st->height * 4;
     mg::FrameResult r;
-    if (!st->rdr->run(f, nullptr, nullptr, 0, st->image, nullptr, r, err)) g_error = err;
+    if (!st->rdr->run(f, nullptr, nullptr, nullptr, 0, st->image, nullptr, r, err)) g_error = err;
 }
 
 MG_EXPORT void cassia_shutdown(void) {

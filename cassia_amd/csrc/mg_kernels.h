@@ -61,11 +61,12 @@ struct PaintParams {
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n);
 __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n);
-__global__ void k_setup(const float4 *pts, const uint16_t *ids, const LayerEntry *layers,
+__global__ void k_setup(const float4 *pts, const uint16_t *ids, const uint32_t *ranks, const LayerEntry *layers,
                         const SceneDesc *scenes, const uint32_t *scene_line_base, FrameDims dims,
-                        LineSetup *out, uint32_t *lengths, uint32_t *err, uint32_t *bucket_hist);
+                        LineSetup *out, uint32_t *lengths, uint32_t *err, uint32_t order_lines);
+__global__ void k_line_ranks(const uint16_t *ids, uint64_t first, uint64_t n, uint32_t *counts, uint32_t *rank);
 __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *status,
-                           uint32_t *stall, uint64_t *total, uint32_t spread_rows);
+                           uint32_t *stall, uint64_t *total);
 // Per-partition slot counts written by the emission: pass 0's digit counts
 // (digit-major, one column per partition) and slices of the other totals.
 struct SlotCounts {
@@ -77,9 +78,6 @@ constexpr uint32_t kCountSlicesHost = 256;
 __global__ void k_emit(const LineSetup *lines, const uint32_t *offsets, const SceneDesc *scenes,
                        FrameDims dims, const FrameCounters *counters, uint64_t *out, const uint32_t *part_line,
                        SlotCounts sc);
-__global__ void k_order_scatter(const LineSetup *setup, const uint32_t *lengths, uint64_t n_lines,
-                                const SceneDesc *scenes, uint32_t *base, uint32_t bucket_rows,
-                                LineSetup *setup_perm, uint32_t *lengths_perm);
 __global__ void k_path_counts(const PathCommand *cmds, const flat::PathDesc *paths, uint32_t n_paths,
                               uint32_t *counts, uint32_t *err);
 __global__ void k_path_emit(const PathCommand *cmds, const flat::PathDesc *paths, uint32_t n_paths,

@@ -193,14 +193,6 @@ __device__ __forceinline__ uint32_t scene_of_line(const uint32_t *scene_line_bas
     return lo;
 }
 
-// Render frames order their lines by (scene, layer order) with a counting
-// sort, so slot order inside every tile (one scene's) is layer order. Bucket
-// of a line: the scene's props range, where every layer order has an entry.
-__device__ __forceinline__ uint32_t line_bucket(const SceneDesc &S, uint32_t packed) {
-    const uint32_t order = packed & 0xFFFFu, last = S.n_props ? S.n_props - 1 : 0u;
-    return S.props_base + (order < last ? order : last);
-}
-
 // Head lane of each run of equal keys among a wave's valid lanes (a prefix
 // of the wave); returns the head of this lane's run.
 __device__ __forceinline__ uint32_t run_head_lane(bool valid, uint32_t key, uint64_t *heads_out) {
@@ -213,37 +205,52 @@ __device__ __forceinline__ uint32_t run_head_lane(bool valid, uint32_t key, uint
     return le ? 63u - (uint32_t)__clzll((long long)le) : 0u;
 }
 
-// hist[key] += 1 for every valid lane: one atomic per run of equal keys.
-__device__ __forceinline__ void bucket_count(bool valid, uint32_t key, uint32_t *hist) {
+// Rank of each line among the lines of its layer id: lines [first, n) are
+// ranked after those counted already (counts[id]: lines of id ranked so
+// far), one returning atomic per run of equal ids in a wave. Ranks inside a
+// layer come in no fixed order: the segments of one layer in one tile are
+// summed (wrapping integer adds), so their order never shows in a frame.
+__global__ __launch_bounds__(256) void k_line_ranks(const uint16_t *__restrict__ ids, uint64_t first, uint64_t n,
+                                                    uint32_t *__restrict__ counts, uint32_t *__restrict__ rank) {
+    const uint64_t i = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    const bool valid = i < n;
+    const uint32_t key = valid ? ids[i] : 0xFFFFFFFFu;
     uint64_t heads;
-    run_head_lane(valid, key, &heads);
+    const uint32_t head = run_head_lane(valid, key, &heads);
     const uint32_t lane = lane_id();
-    const uint32_t vn = (uint32_t)__popcll(__ballot(valid));
+    const uint32_t vn = (uint32_t)__popcll(__ballot(valid));  // valid lanes are a prefix
+    uint32_t at = 0;
     if ((heads >> lane) & 1ull) {
         const uint64_t after = heads & ~((2ull << lane) - 1ull);
         const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
         const uint32_t end = next < vn ? next : vn;
-        atomicAdd(&hist[key], end - lane);
+        at = atomicAdd(&counts[key], end - lane);
     }
+    const uint32_t r = __shfl(at, (int)head) + (lane - head);
+    if (valid) rank[i] = r;
 }
 
-// LinesBuilder::build for one line (segment.rs:107-219).
+// LinesBuilder::build for one line (segment.rs:107-219). Render frames
+// (order_lines) write line i to its layer's base + its rank, which orders
+// the frame's lines by (scene, layer order): slot order inside every tile is
+// then layer order, and the segment sort covers only the row and column
+// bits. Other frames keep the line order.
 __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, const uint16_t *__restrict__ ids,
+                                               const uint32_t *__restrict__ ranks,
                                                const LayerEntry *__restrict__ layers,
                                                const SceneDesc *__restrict__ scenes,
                                                const uint32_t *__restrict__ scene_line_base,
                                                FrameDims dims, LineSetup *__restrict__ out,
                                                uint32_t *__restrict__ lengths, uint32_t *__restrict__ err,
-                                               uint32_t *__restrict__ bucket_hist) {
+                                               uint32_t order_lines) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool valid = i < dims.n_lines;
-    uint32_t bucket = 0xFFFFFFFFu;
-    if (valid) {
+    if (i >= dims.n_lines) return;
     uint32_t sc = dims.n_scenes == 1 ? 0 : scene_of_line(scene_line_base, dims.n_scenes, i);
     const SceneDesc &S = scenes[sc];
     // the scene's own resident lines, or the frame's line arrays
     const float4 *P = S.lines_pts ? (const float4 *)S.lines_pts : pts;
     const uint16_t *I = S.lines_ids ? (const uint16_t *)S.lines_ids : ids;
+    const uint32_t *Rk = S.lines_pts ? (const uint32_t *)S.lines_rank : ranks;
     const uint32_t li = S.lines_pts ? i - S.line_base : i;
     uint32_t id = I[li];
     LayerEntry le;
@@ -255,6 +262,15 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
     r.packed = sc << 19;
     uint32_t length = 0;
     bool exists = le.flags & LF_EXISTS;
+    uint32_t pos = i;
+    if (order_lines) {
+        // every line of a render frame has a layer entry (its position)
+        pos = exists ? le.base + Rk[li] : 0xFFFFFFFFu;
+        if (pos >= dims.n_lines) {
+            atomicOr(err, 2u);
+            return;
+        }
+    }
     if (!(exists && !(le.flags & LF_ENABLED))) {
         uint32_t order = (exists && (le.flags & LF_ORDER)) ? le.order : id;
         float4 p = P[li];
@@ -299,11 +315,8 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
             r.packed = (order & 0xFFFFu) | (octant << 16) | (sc << 19);
         }
     }
-    out[i] = r;
-    lengths[i] = length;
-    bucket = bucket_slot(line_bucket(S, r.packed), dims.bucket_rows);
-    }
-    if (bucket_hist) bucket_count(valid, bucket, bucket_hist);
+    out[pos] = r;
+    lengths[pos] = length;
 }
 
 // ------------------------------------------------------- path flattening
@@ -350,10 +363,9 @@ __global__ __launch_bounds__(256) void k_path_emit(const PathCommand *__restrict
 
 // Single-pass exclusive scan of u32 values (decoupled look-back), 2048
 // values per partition. out may alias in. total receives the sum (u64).
-// Element e lives at bucket_slot(e, spread_rows) (spread_rows 0: at e).
 __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n,
                                                   uint64_t *status, uint32_t *stall,
-                                                  uint64_t *total, uint32_t spread_rows) {
+                                                  uint64_t *total) {
     __shared__ uint32_t wsum[4];
     __shared__ uint64_t s_excl;
     constexpr uint32_t IPT = 8;
@@ -361,7 +373,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
     const uint64_t base = (uint64_t)part * 256 * IPT + (uint64_t)threadIdx.x * IPT;
     uint32_t v[IPT], sum = 0;
     for (uint32_t k = 0; k < IPT; ++k) {
-        v[k] = base + k < n ? in[bucket_slot((uint32_t)(base + k), spread_rows)] : 0u;
+        v[k] = base + k < n ? in[base + k] : 0u;
         sum += v[k];
     }
     uint32_t agg;
@@ -378,7 +390,7 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
     __syncthreads();
     uint64_t run = s_excl + ex;
     for (uint32_t k = 0; k < IPT; ++k) {
-        if (base + k < n) out[bucket_slot((uint32_t)(base + k), spread_rows)] = (uint32_t)run;
+        if (base + k < n) out[base + k] = (uint32_t)run;
         run += v[k];
     }
 }
@@ -762,44 +774,6 @@ __global__ __launch_bounds__(256) void k_count_reduce(const uint32_t *__restrict
         atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)all);
     } else {
         atomicAdd((unsigned long long *)&counters->segments, (unsigned long long)all);
-    }
-}
-
-// Lines grouped by layer order (a counting sort; the order inside one layer
-// is arbitrary): histogram of the orders, exclusive scan, scatter of the
-// line setup records and lengths. Consecutive lines mostly share an order
-// (the lines of one path), so each run of equal orders inside a wave takes
-// one atomic, issued by the run's first lane.
-// Stable scatter of the lines to their bucket's range (bases = exclusive
-// scan of the bucket counts), one atomic per run of equal buckets.
-__global__ __launch_bounds__(256) void k_order_scatter(const LineSetup *__restrict__ setup,
-                                                       const uint32_t *__restrict__ lengths, uint64_t n_lines,
-                                                       const SceneDesc *__restrict__ scenes,
-                                                       uint32_t *__restrict__ base, uint32_t bucket_rows,
-                                                       LineSetup *__restrict__ setup_perm,
-                                                       uint32_t *__restrict__ lengths_perm) {
-    const uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
-    const bool valid = i < n_lines;
-    LineSetup L;
-    if (valid) L = setup[i];
-    // the bucket's counter slot (base holds each bucket's next position there)
-    const uint32_t order = valid ? bucket_slot(line_bucket(scenes[L.packed >> 19], L.packed), bucket_rows)
-                                 : 0xFFFFFFFFu;
-    uint64_t heads;
-    const uint32_t head = run_head_lane(valid, order, &heads);
-    const uint32_t lane = lane_id();
-    const uint32_t vn = (uint32_t)__popcll(__ballot(valid));  // valid lanes are a prefix
-    uint32_t first = 0;
-    if ((heads >> lane) & 1ull) {
-        const uint64_t after = heads & ~((2ull << lane) - 1ull);
-        const uint32_t next = after ? (uint32_t)__ffsll((long long)after) - 1u : 64u;
-        const uint32_t end = next < vn ? next : vn;
-        first = atomicAdd(&base[order], end - lane);
-    }
-    const uint32_t pos = __shfl(first, (int)head) + (lane - head);
-    if (valid) {
-        setup_perm[pos] = L;
-        lengths_perm[pos] = lengths[i];
     }
 }
 

@@ -39,10 +39,15 @@ struct DevBuf {
 };
 
 // Device-resident lines of one composition or batch: float4 points and u16
-// layer ids, appended as the caller inserts them.
+// layer ids, appended as the caller inserts them; and each line's rank among
+// the lines of its layer id (the first `ranked` lines ranked; `counts`: lines
+// ranked per layer id, `count_ids` entries).
 struct LineStore {
     DevBuf pts, ids;
     size_t cap = 0;
+    DevBuf rank, counts;
+    size_t ranked = 0;
+    uint32_t count_ids = 0;
     bool upload(const float *pts4, const uint16_t *ids, size_t first, size_t n, hipStream_t s, std::string &err);
     // Capacity for `need` lines, keeping the first `keep`.
     bool grow(size_t need, size_t keep, hipStream_t s, std::string &err);
@@ -143,15 +148,20 @@ public:
 
     // Frame over `n_lines` device-resident lines (float4 points, u16 layer
     // ids; unused by ingest frames) into the device image `image`.
-    bool run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
-             int32_t *prev_layers, FrameResult &res, std::string &err);
+    bool run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, const uint32_t *ranks, size_t n_lines,
+             uint8_t *image, int32_t *prev_layers, FrameResult &res, std::string &err);
     // A render frame through a captured hipGraph: replayed while everything
     // its kernels take as arguments (lines, image, canvas, key layout) is
     // unchanged, with the frame's tables uploaded beside it. The first frame
     // of a shape, and any frame beyond the captured capacities, runs eagerly
     // and captures anew (`replayed` false).
-    bool run_graph(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
-                   FrameResult &res, bool &replayed, std::string &err);
+    bool run_graph(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, const uint32_t *ranks,
+                   size_t n_lines, uint8_t *image, FrameResult &res, bool &replayed, std::string &err);
+    // Ranks of lines [first, n) among the lines of their layer ids (ids
+    // below n_ids): counts[id] holds the lines of id ranked so far, cleared
+    // first when `reset`.
+    bool rank_lines(const uint16_t *ids, uint32_t *counts, uint32_t n_ids, uint32_t *rank, size_t first, size_t n,
+                    bool reset, std::string &err);
     // Path flattening on the device: the paths' lines appended to `ls` after
     // its first `base` lines, path after path; `counts` receives each
     // path's number of lines.
@@ -167,14 +177,14 @@ public:
 
 private:
     struct GraphKey {
-        const void *pts = nullptr, *ids = nullptr, *image = nullptr;
+        const void *pts = nullptr, *ids = nullptr, *ranks = nullptr, *image = nullptr;
         size_t n_lines = 0;
         uint64_t image_bytes = 0;
         uint32_t radix_bits = 0, max_order = 0, mode = 0;
         size_t n_props = 0;
         std::vector<SceneDesc> scenes;
         bool operator==(const GraphKey &o) const {
-            return pts == o.pts && ids == o.ids && image == o.image && n_lines == o.n_lines &&
+            return pts == o.pts && ids == o.ids && ranks == o.ranks && image == o.image && n_lines == o.n_lines &&
                    image_bytes == o.image_bytes && radix_bits == o.radix_bits && max_order == o.max_order &&
                    mode == o.mode && n_props == o.n_props && scenes.size() == o.scenes.size() &&
                    std::memcmp(scenes.data(), o.scenes.data(), scenes.size() * sizeof(SceneDesc)) == 0;
@@ -205,7 +215,7 @@ private:
     DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
     DevBuf row_count_, row_start_, row_scene_, row_status_;
     DevBuf overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
-    DevBuf part_line_, setup_perm_, lengths_perm_, order_hist_, order_status_;
+    DevBuf part_line_;
     DevBuf cnt0_, cnt_acc_, cnt_status_;
     DevBuf flat_cmds_, flat_paths_, flat_counts_, flat_offsets_, flat_status_;
     DevBuf work_, runs_work_, tile_nr_, runs_status_, walk_scratch_, tile_start_, tile_run0_, run_key_, run_beg_, run_cov_, rec_, rec_len_, rec_hdr_;

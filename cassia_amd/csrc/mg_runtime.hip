@@ -89,8 +89,7 @@ std::vector<DevBuf *> Renderer::all_buffers() {
             &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
             &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
             &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
-            &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &setup_perm_, &lengths_perm_,
-            &order_hist_, &order_status_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
+            &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
             &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_,
             &walk_scratch_, &part_kept_, &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_,
             &deep_list_};
@@ -249,8 +248,14 @@ bool LineStore::grow(size_t need, size_t keep, hipStream_t s, std::string &err) 
 void LineStore::release() {
     pts.release();
     ids.release();
+    rank.release();
+    counts.release();
     cap = 0;
+    ranked = 0;
+    count_ids = 0;
 }
+
+
 
 static void fill32(uint32_t *p, uint32_t v, uint64_t n, hipStream_t s) {
     if (!n) return;
@@ -262,6 +267,16 @@ static void fill64(uint64_t *p, uint64_t v, uint64_t n, hipStream_t s) {
     if (!n) return;
     uint32_t blocks = (uint32_t)std::min<uint64_t>((n + 255) / 256, 4096);
     k_fill_u64<<<blocks, 256, 0, s>>>(p, v, n);
+}
+
+bool Renderer::rank_lines(const uint16_t *ids, uint32_t *counts, uint32_t n_ids, uint32_t *rank, size_t first,
+                          size_t n, bool reset, std::string &err) {
+    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+    if (reset) fill32(counts, 0, n_ids, stream_);
+    if (n > first)
+        k_line_ranks<<<(uint32_t)((n - first + 255) / 256), 256, 0, stream_>>>(ids, first, n, counts, rank);
+    MG_HIP(hipGetLastError(), "line ranks launch");
+    return true;
 }
 
 int32_t *LayerCaches::array(int id, size_t n_tiles, hipStream_t s, bool &resized, std::string &err) {
@@ -407,8 +422,8 @@ bool Renderer::upload_tables(const FrameSpec &spec, uint32_t global_rows, std::s
     return true;
 }
 
-bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines, uint8_t *image,
-                   int32_t *prev_layers, FrameResult &res, std::string &err) {
+bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, const uint32_t *ranks,
+                   size_t n_lines, uint8_t *image, int32_t *prev_layers, FrameResult &res, std::string &err) {
     MG_HIP(hipSetDevice(device_), "hipSetDevice");
     const uint32_t n_scenes = (uint32_t)spec.scenes.size();
     if (n_scenes == 0 || n_scenes > 8192) {  // LineSetup::packed holds a 13-bit scene index
@@ -471,42 +486,24 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     fill32(pc, 0, 64, stream_);
     const bool ingest = spec.mode == 2;
     if (L && !ingest) {
-        // render frames: counting sort of the lines by (scene, layer order),
-        // the bucket counts taken by the setup kernel
-        const uint64_t n_buckets = std::max<size_t>(spec.props.size(), 1);
-        const uint64_t oparts = (n_buckets + kScanTileHost - 1) / kScanTileHost;
-        // bucket counters in the interleaved layout of bucket_slot
-        const uint32_t brows = (uint32_t)((n_buckets + kBucketSpread - 1) / kBucketSpread);
-        const uint64_t n_slots = (uint64_t)brows * kBucketSpread;
-        dims.bucket_rows = brows;
+        // render frames place each line at its layer's base plus its rank
+        // (lines ordered by scene and layer order); a length left unwritten
+        // would only come from inconsistent tables, and stays 0
         if (order_lines) {
-            if (!setup_perm_.reserve(L * sizeof(LineSetup)) || !lengths_perm_.reserve(L * 4) ||
-                !order_hist_.reserve(n_slots * 4) || !order_status_.reserve(oparts * 8)) {
-                err = "out of device memory (line order)";
+            if (!ranks && !spec.scenes[0].lines_rank) {
+                err = "render frame without line ranks";
                 return false;
             }
-            fill32(order_hist_.as<uint32_t>(), 0, n_slots, stream_);
-            fill64(order_status_.as<uint64_t>(), 0, oparts, stream_);
+            fill32(lengths_.as<uint32_t>(), 0, L, stream_);
         }
         k_setup<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
-            pts, ids, layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(), scene_line_base_.as<uint32_t>(), dims,
-            setup_.as<LineSetup>(), lengths_.as<uint32_t>(), err_.as<uint32_t>(),
-            order_lines ? order_hist_.as<uint32_t>() : nullptr);
+            pts, ids, ranks, layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(), scene_line_base_.as<uint32_t>(), dims,
+            setup_.as<LineSetup>(), lengths_.as<uint32_t>(), err_.as<uint32_t>(), order_lines ? 1u : 0u);
         const uint64_t nparts = (L + kScanTileHost - 1) / kScanTileHost;
         if (!scan_status_.reserve(nparts * 8)) { err = "out of device memory"; return false; }
-        const uint32_t *scan_in = lengths_.as<uint32_t>();
-        if (order_lines) {
-            k_scan_u32<<<(uint32_t)oparts, 256, 0, stream_>>>(order_hist_.as<uint32_t>(), order_hist_.as<uint32_t>(),
-                                                            n_buckets, order_status_.as<uint64_t>(), stall,
-                                                            &ctr->order_total, brows);
-            k_order_scatter<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
-                setup_.as<LineSetup>(), lengths_.as<uint32_t>(), L, scenes_.as<SceneDesc>(),
-                order_hist_.as<uint32_t>(), brows, setup_perm_.as<LineSetup>(), lengths_perm_.as<uint32_t>());
-            scan_in = lengths_perm_.as<uint32_t>();
-        }
         fill64(scan_status_.as<uint64_t>(), 0, nparts, stream_);
-        k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(scan_in, offsets_.as<uint32_t>(), L,
-                                                        scan_status_.as<uint64_t>(), stall, &ctr->line_pixels, 0u);
+        k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(lengths_.as<uint32_t>(), offsets_.as<uint32_t>(), L,
+                                                        scan_status_.as<uint64_t>(), stall, &ctr->line_pixels);
     }
     MG_HIP(hipGetLastError(), "setup/scan launch");
     FrameCounters hc{};
@@ -521,7 +518,8 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         MG_HIP(hipMemcpyAsync(&herr, err_.ptr, 4, hipMemcpyDeviceToHost, stream_), "read err");
         MG_HIP(hipStreamSynchronize(stream_), "setup/scan");
         if (herr) {
-            err = "a line spans more pixels than an i32 can index";
+            err = (herr & 2u) ? "line ranks and layer bases of the frame are inconsistent"
+                              : "a line spans more pixels than an i32 can index";
             return false;
         }
         P = hc.line_pixels;
@@ -567,12 +565,10 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
                                                                    dims, ctr, segs_a_.as<uint64_t>(), counts);
     } else if (P) {
         if (!part_line_.reserve(nparts_emit * 4)) { err = "out of device memory"; return false; }
-        const uint32_t *lens = order_lines ? lengths_perm_.as<uint32_t>() : lengths_.as<uint32_t>();
-        k_part_lines<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(offsets_.as<uint32_t>(), lens, L,
+        k_part_lines<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(offsets_.as<uint32_t>(), lengths_.as<uint32_t>(), L,
                                                                       kEmitTileHost, part_line_.as<uint32_t>(),
                                                                       nparts_emit);
-        const LineSetup *setup = order_lines ? setup_perm_.as<LineSetup>() : setup_.as<LineSetup>();
-        k_emit<<<(uint32_t)nparts_emit, 256, count_lds, stream_>>>(setup, offsets_.as<uint32_t>(),
+        k_emit<<<(uint32_t)nparts_emit, 256, count_lds, stream_>>>(setup_.as<LineSetup>(), offsets_.as<uint32_t>(),
                                                                  scenes_.as<SceneDesc>(), dims, ctr,
                                                                  segs_a_.as<uint64_t>(), part_line_.as<uint32_t>(),
                                                                  counts);
@@ -583,7 +579,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         k_count_reduce<<<rest_rows, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), dims, hist_.as<uint32_t>(), ctr);
         fill64(cnt_status_.as<uint64_t>(), 0, cnt_parts, stream_);
         k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
-                                                           cnt_status_.as<uint64_t>(), stall, &ctr->order_total, 0u);
+                                                           cnt_status_.as<uint64_t>(), stall, &ctr->order_total);
     }
     k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
     // look-back status words of every digit pass, cleared outside the sort's timing
@@ -963,7 +959,7 @@ std::vector<const void *> Renderer::frame_buffers() const {
     const DevBuf *all[] = {&scenes_, &scene_line_base_, &layers_, &props_, &stops_, &row_scene_, &setup_,
                            &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_, &segs_b_, &hist_,
                            &bases_, &sort_status_, &part_counters_, &row_start_, &overflow_, &part_line_,
-                           &setup_perm_, &lengths_perm_, &order_hist_, &order_status_, &cnt0_, &cnt_acc_,
+                           &cnt0_, &cnt_acc_,
                            &cnt_status_, &work_, &runs_work_, &tile_nr_, &runs_status_, &tile_start_, &tile_run0_, &run_key_, &run_beg_,
                            &run_cov_, &rec_, &rec_len_, &rec_hdr_, &deep_list_, &part_kept_};
     std::vector<const void *> v;
@@ -978,7 +974,8 @@ void Renderer::drop_graph() {
     graph_ = nullptr;
 }
 
-bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, size_t n_lines,
+bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_t *ids, const uint32_t *ranks,
+                         size_t n_lines,
                          uint8_t *image, FrameResult &res, bool &replayed, std::string &err) {
     replayed = false;
     MG_HIP(hipSetDevice(device_), "hipSetDevice");
@@ -986,6 +983,7 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
     GraphKey key;
     key.pts = pts;
     key.ids = ids;
+    key.ranks = ranks;
     key.image = image;
     key.n_lines = n_lines;
     key.image_bytes = spec.image_bytes;
@@ -1030,7 +1028,7 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
         drop_graph();
     }
     drop_graph();
-    if (!run(spec, pts, ids, n_lines, image, nullptr, res, err)) return false;
+    if (!run(spec, pts, ids, ranks, n_lines, image, nullptr, res, err)) return false;
     if (spec.mode != 0 || spec.want_tile_ops || spec.want_coverage) return true;
     // capture the frame with 25% headroom over this one: a run with the
     // capacity sizes first (it sizes every buffer, so the capture allocates
@@ -1043,7 +1041,7 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
     rec_cap_ += rec_cap_ / 4;
     FrameResult tmp;
     std::string cerr;
-    bool ok = run(spec, pts, ids, n_lines, image, nullptr, tmp, cerr);
+    bool ok = run(spec, pts, ids, ranks, n_lines, image, nullptr, tmp, cerr);
     {
         // the capacity run must complete like the eager one (same image)
         uint32_t herr[3] = {0, 0, 0}, ov[4] = {0, 0, 0, 0};
@@ -1054,13 +1052,13 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
     if (!ok) {
         capture_P_ = 0;
         (void)hipGetLastError();
-        return run(spec, pts, ids, n_lines, image, nullptr, res, err);  // stays eager
+        return run(spec, pts, ids, ranks, n_lines, image, nullptr, res, err);  // stays eager
     }
     hipGraph_t g = nullptr;
     if (ok) {
         ok = hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed) == hipSuccess;
         if (ok) {
-            const bool enq = run(spec, pts, ids, n_lines, image, nullptr, tmp, cerr);
+            const bool enq = run(spec, pts, ids, ranks, n_lines, image, nullptr, tmp, cerr);
             ok = hipStreamEndCapture(stream_, &g) == hipSuccess && enq;
         }
     }
@@ -1109,7 +1107,7 @@ bool Renderer::flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat:
                                              (uint32_t)n_paths, flat_counts_.as<uint32_t>(), err_.as<uint32_t>());
     k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(flat_counts_.as<uint32_t>(), flat_offsets_.as<uint32_t>(),
                                                     n_paths, flat_status_.as<uint64_t>(), err_.as<uint32_t>() + 1,
-                                                    &ctr->line_pixels, 0u);
+                                                    &ctr->line_pixels);
     MG_HIP(hipGetLastError(), "flatten count launch");
     uint64_t total = 0;
     uint32_t herr[2] = {0, 0};

@@ -25,6 +25,11 @@ enum : uint32_t { LF_EXISTS = 1, LF_ENABLED = 2, LF_AFFINE = 4, LF_ORDER = 8 };
 struct LayerEntry {
     uint32_t flags;
     uint32_t order;
+    // render frames: position of the layer's first line when the frame's
+    // lines are ordered by (scene, layer order); line i of the layer goes to
+    // base + its rank among the layer's lines
+    uint32_t base;
+    uint32_t _pad;
     float affine[6];
 };
 
@@ -68,6 +73,8 @@ struct SceneDesc {
     // layer ids), line i of the frame being their line i - line_base; 0: the
     // frame's line arrays hold them at line_base
     uint64_t lines_pts, lines_ids;
+    // and each line's rank among the lines of its layer id (u32)
+    uint64_t lines_rank;
 };
 
 // Per-line setup record written by the setup kernel (segment.rs:144-219).
@@ -86,18 +93,7 @@ struct FrameDims {
     uint32_t radix_bits, passes;
     uint32_t mode;        // 0: render (cull + render key), 1: canonical full stream, 2: ingest
     uint32_t sort_shift;  // bit of the first sort digit
-    uint32_t bucket_rows; // line-order bucket counters: rows of the interleaved layout (bucket_slot)
 };
-
-// Counter slot of line-order bucket b: the counters are laid out as a
-// 32 x rows matrix (bucket b in column b / 32 of row b mod 32), so that
-// consecutive buckets, which concurrent waves count and place at once (the
-// lines of consecutive paths), fall in different 128-byte lines instead of
-// contending for the same few.
-constexpr uint32_t kBucketSpread = 32;
-__host__ __device__ inline uint32_t bucket_slot(uint32_t b, uint32_t rows) {
-    return rows ? (b % kBucketSpread) * rows + b / kBucketSpread : b;
-}
 
 // Device-side counters of a frame.
 struct FrameCounters {
