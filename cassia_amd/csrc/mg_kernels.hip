@@ -1145,6 +1145,7 @@ template __global__ void k_sort_pass<9>(const uint64_t *, uint64_t *, const Fram
 constexpr uint32_t kMaskBit = 1u << 30, kSkipClipBit = 1u << 31;
 constexpr uint32_t kRunEvenOdd = 1u << 16;                    // run key: layer | even_odd << 16
 constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flags
+constexpr uint32_t kRecEvenOdd = 1u << 18;                     // record key: the layer's fill rule is even-odd
 constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
 constexpr uint32_t kPaintWaveLds = kStageWin * 8 + 2 * 64 * 4;  // stage window + cells of one wave
 constexpr uint32_t kDeepFlag = 1u << 31;                      // record header: tile of the deep list
@@ -1825,7 +1826,7 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
         // lay_eo: layer | kRunEvenOdd bit (the queue entry's key)
         auto put = [&](uint32_t lay_eo, int32_t qi, int32_t ri, uint64_t v, bool keep) {
             if (write_rec) {
-                uint32_t key = lay_eo & 0xFFFFu, beg = 0, len = 0;
+                uint32_t key = (lay_eo & 0xFFFFu) | ((lay_eo & kRunEvenOdd) ? kRecEvenOdd : 0u), beg = 0, len = 0;
                 uint64_t cv = 0;
                 if (ri >= 0) {
                     key |= kRecRun;
@@ -2146,6 +2147,17 @@ __device__ __forceinline__ void blend_apply(float &c0, float &c1, float &c2, flo
     fma_in_place(ca, inv, a);
 }
 
+// The props table seen as constant memory: read at a wave-uniform index, its
+// entries come through scalar loads (the table is not written while a frame
+// paints), so a record's flags and solid colour need no lane reads.
+typedef uint32_t PropsHead __attribute__((ext_vector_type(8)));  // flags, clip_layers, color[4], grad_start
+static_assert(offsetof(PropsEntry, color) == 8 && sizeof(PropsEntry) % 16 == 0, "props entry layout");
+typedef const __attribute__((address_space(4))) PropsHead *ScalarPropsHead;
+// The first 32 bytes of entry `lay` (flags and solid colour), one scalar load.
+__device__ __forceinline__ PropsHead props_head(const PropsEntry *props, uint32_t lay) {
+    return *(ScalarPropsHead)((uintptr_t)(props + lay));
+}
+
 // The painter over records [skipped, n) of a tile (painter/mod.rs:260-400):
 // coverage, clip masks (only in scenes with clips), column skipping and
 // blending into this lane's pixel colour.
@@ -2165,12 +2177,18 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
         if (Staged) stage_batch(st, segs, b);
         cnt = b.cnt;
         for (uint32_t q = 0; q < cnt; ++q) {
-            const uint32_t f = rlane(b.flags, q);
+            // with clips the flags are the clip pass's (per record); without,
+            // the layer's, read with its colour through scalar loads, which
+            // the coverage computation below leaves time to arrive
+            const uint32_t key = rlane(b.key, q);
+            const uint32_t lay = key & 0xFFFFu;
+            PropsHead ph;
+            if (!Clips) ph = props_head(props, lay);
+            const uint32_t f = Clips ? rlane(b.flags, q) : ph[0];
             if (Clips && !(f & kMaskBit)) continue;
-            const uint32_t lay = rlane(b.key, q) & 0xFFFFu;
             if (Clips && has_clip && clip_last < lay) has_clip = false;
             const int32_t A = area(b, k0, q);
-            const float cov = coverage_from_area(A, f & PF_EVENODD);
+            const float cov = coverage_from_area(A, (key & kRecEvenOdd) != 0);
             if (Clips && (f & PF_CLIP)) {
                 // clip_at (painter/mod.rs:402-416)
                 if (!has_clip) {
@@ -2196,10 +2214,13 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
                 if (apply_clip) a *= clip_mask;
                 blend_terms(mode, c0, c1, c2, g[0], g[1], g[2], a, t0, t1, t2);
             } else {
-                a = rlane_f(b.color.w, q) * cov;
+                const float s0 = Clips ? rlane_f(b.color.x, q) : __uint_as_float(ph[2]);
+                const float s1 = Clips ? rlane_f(b.color.y, q) : __uint_as_float(ph[3]);
+                const float s2 = Clips ? rlane_f(b.color.z, q) : __uint_as_float(ph[4]);
+                const float s3 = Clips ? rlane_f(b.color.w, q) : __uint_as_float(ph[5]);
+                a = s3 * cov;
                 if (apply_clip) a *= clip_mask;
-                blend_terms_uniform(mode, c0, c1, c2, rlane_f(b.color.x, q), rlane_f(b.color.y, q),
-                                    rlane_f(b.color.z, q), a, t0, t1, t2);
+                blend_terms_uniform(mode, c0, c1, c2, s0, s1, s2, a, t0, t1, t2);
             }
             blend_apply(c0, c1, c2, ca, t0, t1, t2, a);
         }
