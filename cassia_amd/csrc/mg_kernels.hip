@@ -167,6 +167,17 @@ __device__ V block_excl_scan(V v, V *wsum /* W */, V *total) {
     return base + x - v;
 }
 
+// LDS hand-off between the lanes of one wave: the wave's LDS operations
+// complete in issue order, so ordering the compiler (and waiting for the
+// wave's own outstanding LDS operations) is all it takes. Waves that work
+// independently (a deep tile's stagers, tile-run waves) use it instead of a
+// work-group barrier.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 // ------------------------------------------------------------ fill kernels
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n) {
@@ -1450,57 +1461,32 @@ __global__ __launch_bounds__(256) void k_run_cov(PaintParams pp, const FrameCoun
     }
 }
 
-// Segmented inclusive scan over the 64 lanes of a wave (all active) of
-// eight row sums: (f, s) then (f', s') gives (f | f', f' ? s' : s + s').
-// The rows are 16-bit fields of four words (row 2i in the low half of word
-// i, row 2i + 1 in the high half) holding byte values; 64 of them sum to at
-// most 64 * 255 < 2^16, so plain word adds never carry between fields and
-// the low byte of a field is the byte-wise wrapping sum. The DPP steps of
-// wave_incl_scan; a lane whose source is outside the row or the enabled
-// rows combines with (0, 0), the identity.
-// The words move in every row (reading zero from outside the row,
-// bound_ctrl) and are masked off in lanes with f set and in the rows the
-// step does not combine, so that the move and the mask are one DPP AND.
-template <int Ctrl, int Rows>
-__device__ __forceinline__ void seg_scan_step(uint32_t &f, uint32_t (&s)[4]) {
-    const uint32_t fp = (uint32_t)__builtin_amdgcn_update_dpp(0, (int)f, Ctrl, Rows, 0xF, false);
-    const uint32_t row = lane_id() >> 4;
-    const bool on = Rows == 0xF || ((Rows >> row) & 1);
-    uint32_t m = (f == 0u && on) ? ~0u : 0u;
-    asm("" : "+v"(m));  // kept as a mask: an AND that takes the move as its DPP operand
-#pragma unroll
-    for (int i = 0; i < 4; ++i) s[i] += (uint32_t)__builtin_amdgcn_mov_dpp((int)s[i], Ctrl, 0xF, 0xF, true) & m;
-    f |= fp;
-}
-__device__ __forceinline__ void wave_seg_scan_rows(uint32_t &f, uint32_t (&s)[4]) {
-    seg_scan_step<0x111, 0xF>(f, s);  // row_shr:1
-    seg_scan_step<0x112, 0xF>(f, s);  // row_shr:2
-    seg_scan_step<0x114, 0xF>(f, s);  // row_shr:4
-    seg_scan_step<0x118, 0xF>(f, s);  // row_shr:8
-    seg_scan_step<0x142, 0xA>(f, s);  // row_bcast:15
-    seg_scan_step<0x143, 0xC>(f, s);  // row_bcast:31
-}
-// Eight packed bytes (byte r = row r) to and from the 16-bit fields.
-__device__ __forceinline__ void bytes_to_fields(uint64_t b, uint32_t (&s)[4]) {
-    const uint32_t lo = (uint32_t)b, hi = (uint32_t)(b >> 32);
-    // v_perm_b32 selector byte 0x0C yields zero
-    s[0] = __builtin_amdgcn_perm(0u, lo, 0x0C010C00u);
-    s[1] = __builtin_amdgcn_perm(0u, lo, 0x0C030C02u);
-    s[2] = __builtin_amdgcn_perm(0u, hi, 0x0C010C00u);
-    s[3] = __builtin_amdgcn_perm(0u, hi, 0x0C030C02u);
-}
-__device__ __forceinline__ uint64_t fields_to_bytes(const uint32_t (&s)[4]) {
-    const uint32_t lo = __builtin_amdgcn_perm(s[1], s[0], 0x06040200u);
-    const uint32_t hi = __builtin_amdgcn_perm(s[3], s[2], 0x06040200u);
-    return (uint64_t)lo | ((uint64_t)hi << 32);
-}
 
 // Per-tile form of the runs, for frames with many tiles of moderate depth:
 // one wave per tile column, the tile's runs stored from its first word's
-// index on (tile_run0 = tile_start).
+// index on (tile_run0 = tile_start). Each chunk of 64 words adds its covers
+// into the wave's LDS cells, one u32 per (run slot, pixel row): slot 0 takes
+// the run left open by the earlier chunks, slot s the s-th run starting in
+// the chunk; the low byte of a cell is the wrapping i8 sum. One lane per slot
+// then packs the slot's 8 row sums into bytes (and clears its cells).
+constexpr uint32_t kRunSlots = 65;  // the open run and up to 64 run heads of a chunk
+__device__ __forceinline__ uint64_t pack_run_cells(uint32_t *c) {
+    const uint4 a = *(const uint4 *)c, b = *(const uint4 *)(c + 4);
+    *(uint4 *)c = make_uint4(0, 0, 0, 0);
+    *(uint4 *)(c + 4) = make_uint4(0, 0, 0, 0);
+    // v_perm_b32 byte selectors 0, 4 of each source pair: low bytes
+    const uint32_t lo = __builtin_amdgcn_perm(__builtin_amdgcn_perm(a.w, a.z, 0x0c0c0400u),
+                                              __builtin_amdgcn_perm(a.y, a.x, 0x0c0c0400u), 0x05040100u);
+    const uint32_t hi = __builtin_amdgcn_perm(__builtin_amdgcn_perm(b.w, b.z, 0x0c0c0400u),
+                                              __builtin_amdgcn_perm(b.y, b.x, 0x0c0c0400u), 0x05040100u);
+    return (uint64_t)lo | ((uint64_t)hi << 32);
+}
+
 // Runs of tile column colp (col + 1; 0 gathers every col < 0) of row grow,
-// by one wave. Returns the number of runs.
-__device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_t grow, uint32_t colp) {
+// by one wave; `cells` is the wave's kRunSlots x 8 LDS cells, all zero.
+// Returns the number of runs.
+__device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_t grow, uint32_t colp,
+                                                  uint32_t *cells) {
     const uint32_t lane = lane_id();
     const SceneDesc &S = pp.scenes[pp.row_scene[grow]];
     const uint32_t *ts_row = pp.tile_start + (uint64_t)grow * pp.tstride;
@@ -1508,7 +1494,7 @@ __device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_
     const PropsEntry *props = pp.props + S.props_base;
     const uint32_t lmask = (1u << pp.layer_bits) - 1u;
     uint32_t nr = 0, prev_layer = 0xFFFFFFFFu;
-    uint64_t open = 0;  // cover sums of the run still open at the previous chunk's end
+    uint64_t open = 0;  // row cover sums of the run still open after the earlier chunks
     for (uint32_t p = ts; p < te; p += 256) {
         uint64_t w[4];
 #pragma unroll
@@ -1527,33 +1513,35 @@ __device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_
             if (lane == 0) prevl = prev_layer;
             const bool head = valid && layer != prevl;
             const uint64_t heads = __ballot(head);
-            // the open run ended exactly at the previous chunk's end
-            if (lane == 0 && head && nr) pp.run_cov[ts + nr - 1] = open;
-            uint64_t s0 = 0;
+            const uint32_t nh = (uint32_t)__popcll(heads);
+            const uint32_t slot = (uint32_t)__popcll(heads & ((2ull << lane) - 1ull));
             if (valid) {
                 const int32_t cover = (int32_t)((int64_t)(w[b] << 58) >> 58);
                 const uint32_t ty = (uint32_t)(w[b] >> 19) & 7u;
-                s0 = (uint64_t)((uint32_t)cover & 0xFFu) << (8 * ty);
+                atomicAdd(&cells[slot * 8 + ty], (uint32_t)cover);
             }
-            // segmented inclusive scan: (f, s) (+) (f', s') = (f | f', f' ? s' : s + s')
-            uint32_t f = head ? 1u : 0u, sf[4], of[4];
-            bytes_to_fields(s0, sf);
-            wave_seg_scan_rows(f, sf);
-            bytes_to_fields(open, of);
-#pragma unroll
-            for (int i = 0; i < 4; ++i) sf[i] += f ? 0u : of[i];
-            const uint64_t s = fields_to_bytes(sf);
-            const uint32_t r = nr - 1u + (uint32_t)__popcll(heads & ((2ull << lane) - 1ull));
             if (head) {
+                const uint32_t r = nr + slot - 1;
                 const bool eo = props[layer].flags & PF_EVENODD;
                 pp.run_key[ts + r] = layer | (eo ? kRunEvenOdd : 0u);
                 pp.run_beg[ts + r] = base + lane;
             }
-            // a run ending inside this chunk is complete; the last one stays open
-            if (valid && lane + 1 < nvalid && ((heads >> lane) & 2ull)) pp.run_cov[ts + r] = s;
-            nr += (uint32_t)__popcll(heads);
-            open = rlane64(s, nvalid - 1);
+            wave_lds_sync();
+            const uint64_t pk = lane <= nh ? pack_run_cells(cells + lane * 8) : 0ull;
+            const uint64_t pk64 = (lane == 0 && nh == 64) ? pack_run_cells(cells + 64 * 8) : 0ull;
+            const uint64_t cont = add_bytes(open, rlane64(pk, 0));  // the open run with this chunk's words
+            if (nh == 0) {
+                open = cont;
+            } else {
+                // the open run closed at the chunk's first head; runs 1 .. nh - 1
+                // of the chunk are complete, the last stays open
+                if (lane == 0 && nr) pp.run_cov[ts + nr - 1] = cont;
+                if (lane >= 1 && lane < nh) pp.run_cov[ts + nr + lane - 1] = pk;
+                open = nh < 64 ? rlane64(pk, nh) : rlane64(pk64, 0);
+            }
+            nr += nh;
             prev_layer = rlane(layer, nvalid - 1);
+            wave_lds_sync();
         }
     }
     if (lane == 0) {
@@ -1566,11 +1554,15 @@ __device__ __forceinline__ uint32_t tile_runs_one(const PaintParams &pp, uint32_
 // Tile runs of every (row, column) of the work list: each wave takes entries
 // wave, wave + waves, ...; waves never synchronise.
 __global__ __launch_bounds__(256) void k_tile_runs(PaintParams pp, const uint2 *__restrict__ work, uint32_t n_work) {
+    __shared__ __align__(16) uint32_t s_cells[4][kRunSlots * 8];
+    uint32_t *cells = s_cells[threadIdx.x >> 6];
+    for (uint32_t k = lane_id(); k < kRunSlots * 8; k += 64) cells[k] = 0;
+    wave_lds_sync();
     const uint32_t waves = gridDim.x * 4;
     uint32_t runs = 0;
     for (uint32_t t = blockIdx.x * 4 + (threadIdx.x >> 6); t < n_work; t += waves) {
         const uint2 wk = work[t];
-        runs += tile_runs_one(pp, wk.x, wk.y);
+        runs += tile_runs_one(pp, wk.x, wk.y, cells);
     }
     if (lane_id() == 0 && runs) atomicAdd(pp.n_runs, runs);
 }
@@ -1925,15 +1917,6 @@ struct Stage {
     uint32_t base, end;
 };
 
-// LDS hand-off between the lanes of one wave: the wave's LDS operations
-// complete in issue order, so ordering the compiler (and waiting for the
-// wave's own outstanding LDS operations) is all it takes. Waves of a deep
-// tile stage independently, so no work-group barrier may be used here.
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 __device__ __forceinline__ void stage_load(Stage &st, const uint64_t *segs, uint32_t from, uint32_t to) {
     const uint32_t lane = lane_id();
