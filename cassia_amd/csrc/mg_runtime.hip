@@ -764,18 +764,19 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     // entry per order, so the walk never needs more than runs = orders and
     // queue + runs = 2 orders. Frames whose arrays sized that way fit in a
     // LDS share walk each row with 256 threads (several rows per CU when the
-    // share is small); others use 1024 threads and half the LDS, or the whole
+    // share is small); others use 512 threads and half the LDS, or the whole
     // LDS for deeper tiles (measured on MI355X: 256 threads are 12% faster for a
-    // 2,000-order frame, 1024 threads 34% faster for a 65,536-order one). The
-    // work-group size is a result-invariant tuning knob (FrameSpec::walk_threads).
+    // 2,000-order frame; a 65,536-order one below). The work-group size is a
+    // result-invariant tuning knob (FrameSpec::walk_threads).
     const uint64_t orders = (uint64_t)spec.max_order + 1;
     const uint32_t rcap_fit = (uint32_t)((orders + 63) & ~63ull);
     const bool fits = orders <= 2048 && walk_lds_bytes_host(2 * rcap_fit, rcap_fit, tstride, 1) + 256 <= lds_max;
     uint32_t walk_threads = spec.walk_threads;
     if (walk_threads != 64 && walk_threads != 256 && walk_threads != 512 && walk_threads != 1024)
-        // (512 threads: the walk of a 3840x2160 frame of 200,000 polygons is
-        // 0.08 ms faster, its paint as much slower)
-        walk_threads = fits ? 256 : 1024;
+        // (a 3840x2160 frame of 200,000 polygons, about 700 queue entries
+        // and runs per tile: 1024 threads 2.16 ms, 512 threads 1.95 ms, 256
+        // threads 4.9 ms, which need the whole LDS)
+        walk_threads = fits ? 256 : 512;
     const bool small_walk = fits;
     // The large configuration keeps two sets of run arrays so that the
     // next column's runs are stored while the current column is merged (one

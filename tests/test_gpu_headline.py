@@ -51,7 +51,7 @@ def test_polygons_full_frame_rgba8(gpu, cpu, polygons):
     # passes over the 18 row and column bits, hundreds of millions of words
     assert st.sort_passes == 3 and st.radix_bits == 6, (st.sort_passes, st.radix_bits)
     assert st.segments > 3e8 and st.records > 5e7, (st.segments, st.records)
-    assert st.walk_threads == 1024 and st.walk_lds in (gpu.WALK_HALF, gpu.WALK_WHOLE)
+    assert st.walk_threads == 512 and st.walk_lds in (gpu.WALK_HALF, gpu.WALK_WHOLE)
     c = sc.build(cpu)
     ref = np.zeros((W * H, 4), np.uint8)
     n = c.render_mt(cpu.Buffer(ref, W), sc.background, THREADS)
