@@ -1042,6 +1042,18 @@ MG_EXPORT mg_status mg_copy_to_host(void *dst, const void *device_src, size_t by
     return MG_OK;
 }
 
+MG_EXPORT mg_status mg_stream_copy(void *dst, const void *device_src, size_t bytes, uint32_t repeats,
+                                   double *ms_per_copy) {
+    if (!dst || !device_src || !ms_per_copy || bytes == 0 || bytes % 16 || repeats == 0)
+        return fail(MG_ERR_INVALID_ARG, "bad argument (bytes: a nonzero multiple of 16)");
+    std::string err;
+    auto r = mg::Renderer::shared(err);
+    if (!r) return fail(MG_ERR_DEVICE, err);
+    std::lock_guard<std::mutex> g(r->mu);
+    if (!r->stream_copy(dst, device_src, bytes, repeats, *ms_per_copy, err)) return fail(MG_ERR_DEVICE, err);
+    return MG_OK;
+}
+
 MG_EXPORT mg_status mg_render_coverage(mg_composition *c, uint32_t width, uint32_t height, const float bg[4],
                                        const mg_rect *crop, int32_t *a_out, uint32_t *key_out, size_t cap,
                                        size_t *n) {

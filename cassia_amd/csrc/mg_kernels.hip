@@ -192,6 +192,27 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
         p[i] = v;
 }
 
+// Streaming copy of n 16-byte words: four non-temporal 16-byte loads in
+// flight per thread, then their stores. The device-copy rate it reaches is
+// the HBM ceiling the bandwidth-bound kernels are measured against.
+__global__ __launch_bounds__(256) void k_copy_stream(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                     uint64_t n) {
+    typedef uint32_t V4 __attribute__((ext_vector_type(4)));
+    constexpr uint32_t U = 4;
+    const V4 *a = (const V4 *)src;
+    V4 *b = (V4 *)dst;
+    const uint64_t stride = (uint64_t)gridDim.x * 256 * U;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256 * U + threadIdx.x; i < n; i += stride) {
+        V4 v[U];
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k)
+            if (i + k * 256 < n) v[k] = __builtin_nontemporal_load(&a[i + k * 256]);
+#pragma unroll
+        for (uint32_t k = 0; k < U; ++k)
+            if (i + k * 256 < n) __builtin_nontemporal_store(v[k], &b[i + k * 256]);
+    }
+}
+
 // ----------------------------------------------------------------- setup
 
 __device__ __forceinline__ uint32_t scene_of_line(const uint32_t *scene_line_base, uint32_t n_scenes,

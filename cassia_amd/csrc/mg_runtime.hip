@@ -1131,6 +1131,24 @@ bool Renderer::flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat:
     return true;
 }
 
+bool Renderer::stream_copy(void *dst, const void *src, size_t bytes, uint32_t repeats, double &ms,
+                           std::string &err) {
+    MG_HIP(hipSetDevice(device_), "hipSetDevice");
+    const uint64_t n = bytes / 16;
+    // 16,384 work groups (64 per CU) of 4 x 16 B per thread and pass
+    const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16384, (n + 1023) / 1024));
+    MG_HIP(hipEventRecord(ev0_, stream_), "event");
+    for (uint32_t r = 0; r < repeats; ++r)
+        k_copy_stream<<<grid, 256, 0, stream_>>>((const uint4 *)src, (uint4 *)dst, n);
+    MG_HIP(hipGetLastError(), "copy launch");
+    MG_HIP(hipEventRecord(ev1_, stream_), "event");
+    MG_HIP(hipEventSynchronize(ev1_), "copy");
+    float t = 0.0f;
+    MG_HIP(hipEventElapsedTime(&t, ev0_, ev1_), "elapsed");
+    ms = repeats ? (double)t / repeats : 0.0;
+    return true;
+}
+
 bool Renderer::download_sorted(uint64_t *dst, size_t n, std::string &err) {
     if (!n) return true;
     MG_HIP(hipMemcpyAsync(dst, last_sorted_, n * 8, hipMemcpyDeviceToHost, stream_), "download sorted");

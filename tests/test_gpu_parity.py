@@ -348,3 +348,49 @@ def test_line_changes_between_frames(gpu, cpu):
         for t, o in zip(targets, (70, 71, 72)):
             c.get_mut(t).set_order(o)
     frame("paths flattened on the device")
+
+
+def far_coordinate_lines():
+    """Lines whose sub-pixel coordinates (16 v + 0.5) leave i32 and saturate
+    to i32::MAX / MIN (to_sub_pixel's `as i32`, surpass/src/segment.rs:13-16),
+    lines whose tile indices wrap the 12- and 13-bit PixelSegment fields, and
+    lines with NaN (converted to 0) or infinite endpoints. Every line spans at
+    most ~60 pixels after conversion."""
+    nan, inf = float("nan"), float("inf")
+    L = []
+    for x in (1.4e8, 2.0e9, 3.0e38, -1.4e8, -2.0e9, -3.0e38):  # x saturates: y-major lines
+        L += [[x, 3.0, x + 4.0, 40.0], [x, 41.5, x - 7.0, 9.25]]
+    for y in (1.4e8, -2.0e9):  # y saturates at both ends: no pixels
+        L += [[2.0, y, 50.0, y + 6.0]]
+    for x, y in ((5.0e5, 10.0), (3.0e6, -4.0), (-3.0e6, 20.0), (1.0e7, 7.5), (12.0, 2.5e5), (30.0, -3.0e6)):
+        L += [[x, y, x + 37.0, y + 11.0], [x + 3.0, y + 20.0, x - 9.0, y - 25.0]]  # field wraps
+    L += [[nan, 5.0, 10.0, 30.0], [5.0, nan, 20.0, 40.0], [nan, 2.0, nan, 33.0],
+          [inf, 3.0, inf, 30.0], [-inf, 8.0, -inf, 12.0]]
+    return np.array(L, np.float32)
+
+
+def test_far_and_non_finite_coordinates(gpu, cpu):
+    """Canonical PixelSegment stream and RGBA8 against the oracle for lines
+    far outside any canvas (saturated sub-pixel coordinates, wrapped tile
+    fields, NaN and infinite endpoints) beside an ordinary polygon, also
+    through a rotating layer transform."""
+    lines = far_coordinate_lines()
+    w, h = 48, 40
+    out = {}
+    for m in (cpu, gpu):
+        comp = m.Composition()
+        a, b, c = comp.create_layer(), comp.create_layer(), comp.create_layer()
+        comp.insert_lines(a, lines)
+        comp.insert_lines(b, scenegen.random_polygon(np.random.default_rng(5), w, h, 0))
+        comp.insert_lines(c, lines[:14])
+        for lid, col in ((a, [0.9, 0.2, 0.1, 0.7]), (b, [0.1, 0.5, 0.9, 0.8]), (c, [0.3, 0.9, 0.3, 0.6])):
+            comp.get_mut(lid).set_props(m.Props(fill_rule=m.FillRule.EvenOdd,
+                                                func=m.Func.Draw(m.Style(fill=m.Fill.Solid(col)))))
+        comp.get_mut(c).set_transform([0.6, -0.8, 0.8, 0.6, 5.0, 7.0])
+        segs = comp.pixel_segments()
+        buf = np.zeros((w * h, 4), np.uint8)
+        comp.render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
+        out[m] = (np.asarray(segs), buf)
+    assert len(out[cpu][0]) > 500
+    assert np.array_equal(out[cpu][0], out[gpu][0])
+    assert_same_image(out[cpu][1], out[gpu][1], w)
