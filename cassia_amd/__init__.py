@@ -176,7 +176,7 @@ _sig("mg_layers_set_order", C.c_int, _vp, C.POINTER(C.c_uint16), C.POINTER(C.c_u
 _sig("mg_render_animated", C.c_int, _vp, C.c_uint32, C.c_uint32, _f, C.POINTER(_Rect), C.POINTER(_vp),
      C.POINTER(C.c_int))
 _sig("mg_copy_to_host", C.c_int, _vp, _vp, C.c_size_t)
-_sig("mg_stream_copy", C.c_int, _vp, _vp, C.c_size_t, C.c_uint32, C.POINTER(C.c_double))
+_sig("mg_stream_copy", C.c_int, C.c_size_t, C.c_uint32, C.POINTER(C.c_double), C.POINTER(C.c_uint64))
 _sig("mg_remove_disabled", C.c_int, _vp)
 _sig("mg_lines_len", C.c_int, _vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t))
 _sig("mg_create_buffer_layer_cache", C.c_int, _vp, C.POINTER(C.c_int32))
@@ -773,12 +773,13 @@ def device_frame(address, width, height):
     return out
 
 
-def stream_copy(dst_address, src_address, nbytes, repeats=1):
-    """`repeats` streaming device-to-device copies of `nbytes` (mg_stream_copy);
-    returns the average milliseconds of one copy."""
-    ms = C.c_double()
-    _check(_lib.mg_stream_copy(C.c_void_p(dst_address), C.c_void_p(src_address), nbytes, repeats, C.byref(ms)))
-    return ms.value
+def stream_copy(nbytes, repeats=1):
+    """`repeats` streaming device-to-device copies of `nbytes` between buffers
+    the library allocates (mg_stream_copy): (average milliseconds of one copy,
+    32-bit words found wrong afterwards)."""
+    ms, bad = C.c_double(), C.c_uint64()
+    _check(_lib.mg_stream_copy(nbytes, repeats, C.byref(ms), C.byref(bad)))
+    return ms.value, bad.value
 
 
 class Batch:

@@ -1042,15 +1042,14 @@ MG_EXPORT mg_status mg_copy_to_host(void *dst, const void *device_src, size_t by
     return MG_OK;
 }
 
-MG_EXPORT mg_status mg_stream_copy(void *dst, const void *device_src, size_t bytes, uint32_t repeats,
-                                   double *ms_per_copy) {
-    if (!dst || !device_src || !ms_per_copy || bytes == 0 || bytes % 16 || repeats == 0)
+MG_EXPORT mg_status mg_stream_copy(size_t bytes, uint32_t repeats, double *ms_per_copy, uint64_t *mismatches) {
+    if (!ms_per_copy || !mismatches || bytes == 0 || bytes % 16 || repeats == 0)
         return fail(MG_ERR_INVALID_ARG, "bad argument (bytes: a nonzero multiple of 16)");
     std::string err;
     auto r = mg::Renderer::shared(err);
     if (!r) return fail(MG_ERR_DEVICE, err);
     std::lock_guard<std::mutex> g(r->mu);
-    if (!r->stream_copy(dst, device_src, bytes, repeats, *ms_per_copy, err)) return fail(MG_ERR_DEVICE, err);
+    if (!r->stream_copy(bytes, repeats, *ms_per_copy, *mismatches, err)) return fail(MG_ERR_DEVICE, err);
     return MG_OK;
 }
 

@@ -54,6 +54,18 @@ __host__ __device__ __forceinline__ int32_t sub_pixel(float v) {
     return f2i_sat(__builtin_fmaf(v, 16.0f, __builtin_copysignf(0.5f, v)));
 }
 
+// `f32 as i32` in one instruction: v_cvt_i32_f32 truncates toward zero,
+// saturates out-of-range values (infinities included) and converts NaN to 0,
+// which is exactly Rust's `as` (f2i_sat spells the same out with selects).
+__device__ __forceinline__ int32_t f2i_sat_hw(float v) {
+    int32_t r;
+    asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+    return r;
+}
+__device__ __forceinline__ int32_t sub_pixel_hw(float v) {
+    return f2i_sat_hw(__builtin_fmaf(v, 16.0f, __builtin_copysignf(0.5f, v)));
+}
+
 __host__ __device__ __forceinline__ int32_t wrap_add(int32_t a, int32_t b) {
     return (int32_t)((uint32_t)a + (uint32_t)b);
 }

@@ -62,6 +62,8 @@ struct PaintParams {
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n);
 __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n);
 __global__ void k_copy_stream(const uint4 *src, uint4 *dst, uint64_t n);
+__global__ void k_copy_pattern(uint32_t *p, uint64_t n, uint32_t seed);
+__global__ void k_copy_check(const uint32_t *b, uint64_t first, uint64_t n, uint32_t seed, unsigned long long *bad);
 __global__ void k_setup(const float4 *pts, const uint16_t *ids, const uint32_t *ranks, const LayerEntry *layers,
                         const SceneDesc *scenes, const uint32_t *scene_line_base, FrameDims dims,
                         LineSetup *out, uint32_t *lengths, uint32_t *err, uint32_t order_lines);

@@ -213,6 +213,27 @@ __global__ __launch_bounds__(256) void k_copy_stream(const uint4 *__restrict__ s
     }
 }
 
+// Word i of a copy check's source (a 32-bit mix of i and the seed), and the
+// check: words of `b` differing from that pattern are counted into *bad.
+__device__ __forceinline__ uint32_t copy_pattern(uint64_t i, uint32_t seed) {
+    uint32_t h = (uint32_t)i * 0x9E3779B1u ^ (uint32_t)(i >> 32) ^ seed;
+    h ^= h >> 15;
+    h *= 0x85EBCA77u;
+    return h ^ (h >> 13);
+}
+__global__ void k_copy_pattern(uint32_t *p, uint64_t n, uint32_t seed) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = copy_pattern(i, seed);
+}
+__global__ void k_copy_check(const uint32_t *b, uint64_t first, uint64_t n, uint32_t seed,
+                             unsigned long long *bad) {
+    uint32_t local = 0;
+    for (uint64_t i = first + blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * blockDim.x)
+        local += b[i] != copy_pattern(i, seed) ? 1u : 0u;
+    if (local) atomicAdd(bad, (unsigned long long)local);
+}
+
 // ----------------------------------------------------------------- setup
 
 __device__ __forceinline__ uint32_t scene_of_line(const uint32_t *scene_line_base, uint32_t n_scenes,
@@ -315,7 +336,7 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
             float b1 = __builtin_fmaf(t[2], p1x, __builtin_fmaf(t[3], p1y, t[5]));
             p0x = a0; p0y = b0; p1x = a1; p1y = b1;
         }
-        int32_t x0 = sub_pixel(p0x), y0 = sub_pixel(p0y), x1 = sub_pixel(p1x), y1 = sub_pixel(p1y);
+        int32_t x0 = sub_pixel_hw(p0x), y0 = sub_pixel_hw(p0y), x1 = sub_pixel_hw(p1x), y1 = sub_pixel_hw(p1y);
         if (y0 != y1) {
             int32_t dx = wrap_sub(x1, x0), dy = wrap_sub(y1, y0);
             int32_t adx = dx < 0 ? wrap_sub(0, dx) : dx, ady = dy < 0 ? wrap_sub(0, dy) : dy;
@@ -477,12 +498,12 @@ __device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, u
     const float x1 = fmin_rs((float)x1i, l.end_f32);
     const float y0 = __builtin_fmaf(l.slope, x0, l.py_slope_px);
     const float y1 = __builtin_fmaf(l.slope, x1, l.py_slope_px);
-    const int32_t x0s = sub_pixel(x0), x1s = sub_pixel(x1), y0s = sub_pixel(y0), y1s = sub_pixel(y1);
+    const int32_t x0s = sub_pixel_hw(x0), x1s = sub_pixel_hw(x1), y0s = sub_pixel_hw(y0), y1s = sub_pixel_hw(y1);
     const bool split = (y0s & 15) != 0 && (y1s & 15) != 0 && (y0s >> 4) != (y1s >> 4);
     const int32_t y = (y0s > y1s ? y0s : y1s) >> 4;
     const int32_t ys = (int32_t)((uint32_t)y << 4);
     const float x = __builtin_fmaf(l.slope_recip, (float)y, l.px_slope_recip_py);
-    const int32_t xs = sub_pixel(x);
+    const int32_t xs = sub_pixel_hw(x);
     const int32_t ex = split ? xs : x1s, ey = split ? ys : y1s;
     out[0] = pack_segment(layer, x0s, ex, y0s, ey, x0i, (ey < y0s ? ey : y0s) >> 4, octant);
     out[1] = pack_segment(layer, xs, x1s, ys, y1s, x0i, (ys < y1s ? ys : y1s) >> 4, octant);

@@ -167,9 +167,11 @@ public:
     // path's number of lines.
     bool flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat::PathDesc *paths, size_t n_paths,
                        LineStore &ls, size_t base, std::vector<uint32_t> &counts, std::string &err);
-    // `repeats` streaming copies of `bytes` (a multiple of 16) on the
-    // engine's stream; `ms` receives the average time of one, from events.
-    bool stream_copy(void *dst, const void *src, size_t bytes, uint32_t repeats, double &ms, std::string &err);
+    // `repeats` streaming copies of `bytes` (a multiple of 16) between two
+    // buffers of its own on the engine's stream: `ms` receives the average
+    // time of one (events), `mismatches` the 32-bit words of the result that
+    // differ from the source, or past the copy from what they held before.
+    bool stream_copy(size_t bytes, uint32_t repeats, double &ms, uint64_t &mismatches, std::string &err);
     bool download_sorted(uint64_t *dst, size_t n, std::string &err);
     bool download_tile_ops(int8_t *dst, size_t n, std::string &err);
     bool download_coverage(std::vector<int32_t> &a, std::vector<uint32_t> &key, uint32_t &count,

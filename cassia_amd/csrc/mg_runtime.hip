@@ -1131,21 +1131,43 @@ bool Renderer::flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat:
     return true;
 }
 
-bool Renderer::stream_copy(void *dst, const void *src, size_t bytes, uint32_t repeats, double &ms,
-                           std::string &err) {
+bool Renderer::stream_copy(size_t bytes, uint32_t repeats, double &ms, uint64_t &mismatches, std::string &err) {
     MG_HIP(hipSetDevice(device_), "hipSetDevice");
-    const uint64_t n = bytes / 16;
+    DevBuf src, dst, bad;
+    struct Release {
+        DevBuf *b[3];
+        ~Release() {
+            for (DevBuf *x : b) x->release();
+        }
+    } release{{&src, &dst, &bad}};
+    constexpr uint64_t kGuardWords = 64;  // past the copy in dst: must keep their own pattern
+    const uint64_t n = bytes / 16, words = bytes / 4;
+    if (!src.reserve(bytes) || !dst.reserve(bytes + kGuardWords * 4) || !bad.reserve(8)) {
+        err = "out of device memory";
+        return false;
+    }
+    const uint32_t seed = 0x5EEDu;
+    k_copy_pattern<<<2048, 256, 0, stream_>>>(src.as<uint32_t>(), words, seed);
+    k_copy_pattern<<<2048, 256, 0, stream_>>>(dst.as<uint32_t>(), words + kGuardWords, seed ^ 1u);
+    fill64(bad.as<uint64_t>(), 0, 1, stream_);
     // 16,384 work groups (64 per CU) of 4 x 16 B per thread and pass
     const uint32_t grid = (uint32_t)std::max<uint64_t>(1, std::min<uint64_t>(16384, (n + 1023) / 1024));
     MG_HIP(hipEventRecord(ev0_, stream_), "event");
     for (uint32_t r = 0; r < repeats; ++r)
-        k_copy_stream<<<grid, 256, 0, stream_>>>((const uint4 *)src, (uint4 *)dst, n);
+        k_copy_stream<<<grid, 256, 0, stream_>>>(src.as<uint4>(), dst.as<uint4>(), n);
     MG_HIP(hipGetLastError(), "copy launch");
     MG_HIP(hipEventRecord(ev1_, stream_), "event");
-    MG_HIP(hipEventSynchronize(ev1_), "copy");
+    k_copy_check<<<2048, 256, 0, stream_>>>(dst.as<uint32_t>(), 0, words, seed, bad.as<unsigned long long>());
+    k_copy_check<<<1, 256, 0, stream_>>>(dst.as<uint32_t>(), words, words + kGuardWords, seed ^ 1u,
+                                         bad.as<unsigned long long>());
+    MG_HIP(hipGetLastError(), "check launch");
+    unsigned long long nbad = 0;
+    MG_HIP(hipMemcpyAsync(&nbad, bad.ptr, 8, hipMemcpyDeviceToHost, stream_), "read check");
+    MG_HIP(hipStreamSynchronize(stream_), "copy");
     float t = 0.0f;
     MG_HIP(hipEventElapsedTime(&t, ev0_, ev1_), "elapsed");
-    ms = repeats ? (double)t / repeats : 0.0;
+    ms = (double)t / repeats;
+    mismatches = nbad;
     return true;
 }
 

@@ -267,13 +267,14 @@ MG_EXPORT mg_status mg_render_animated(mg_composition *comp, uint32_t width, uin
    mg_render_animated) to host memory. */
 MG_EXPORT mg_status mg_copy_to_host(void *dst, const void *device_src, size_t bytes);
 
-/* No counterpart in the reference: the measurement harness's HBM ceiling.
-   Copies `bytes` (a nonzero multiple of 16) from device memory to device
-   memory `repeats` times with a streaming (non-temporal) copy kernel on the
-   device's engine stream, and stores the average milliseconds of one copy,
-   timed with events on that stream. The caller's buffers must be ready. */
-MG_EXPORT mg_status mg_stream_copy(void *dst, const void *device_src, size_t bytes, uint32_t repeats,
-                                   double *ms_per_copy);
+/* No counterpart in the reference: the measured HBM ceiling that the
+   bandwidth-bound kernels are compared with. Allocates two device buffers of
+   `bytes` (a nonzero multiple of 16), copies one to the other `repeats` times
+   with the library's streaming (non-temporal) copy kernel on the device's
+   engine stream, stores the average milliseconds of one copy (events on that
+   stream) and the number of 32-bit words the check found wrong (0 for an
+   exact copy that wrote nothing past its end), and frees the buffers. */
+MG_EXPORT mg_status mg_stream_copy(size_t bytes, uint32_t repeats, double *ms_per_copy, uint64_t *mismatches);
 
 /* Composition::pixel_segments (composition.rs:153-188): the canonically
    sorted PixelSegment stream (stable by `seg >> 22` over generation order,
