@@ -61,6 +61,13 @@ struct PaintParams {
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, uint64_t n);
 __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n);
+constexpr uint32_t kZeroRanges = 6;
+struct ZeroRanges {
+    uint32_t *p[kZeroRanges];
+    uint64_t n[kZeroRanges];  // 32-bit words of each range
+    uint32_t count;
+};
+__global__ void k_zero(ZeroRanges z);
 __global__ void k_copy_stream(const uint4 *src, uint4 *dst, uint64_t n);
 __global__ void k_copy_pattern(uint32_t *p, uint64_t n, uint32_t seed);
 __global__ void k_copy_check(const uint32_t *b, uint64_t first, uint64_t n, uint32_t seed, unsigned long long *bad);

@@ -192,6 +192,14 @@ __global__ void k_fill_u64(uint64_t *p, uint64_t v, uint64_t n) {
         p[i] = v;
 }
 
+// Clears up to kZeroRanges ranges of 32-bit words in one launch (the frame's
+// counters, flags, look-back words and histograms).
+__global__ void k_zero(ZeroRanges z) {
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x, step = (uint64_t)gridDim.x * blockDim.x;
+    for (uint32_t r = 0; r < z.count; ++r)
+        for (uint64_t i = t; i < z.n[r]; i += step) z.p[r][i] = 0u;
+}
+
 // Streaming copy of n 16-byte words: four non-temporal 16-byte loads in
 // flight per thread, then their stores. The device-copy rate it reaches is
 // the HBM ceiling the bandwidth-bound kernels are measured against.

@@ -216,10 +216,18 @@ private:
     uint64_t uploaded_owner_ = 0, uploaded_version_ = 0;
     std::vector<SceneDesc> uploaded_scenes_;
     DevBuf scenes_, scene_line_base_, layers_, props_, stops_;
-    DevBuf setup_, lengths_, offsets_, scan_status_, counters_, err_;
-    DevBuf segs_a_, segs_b_, emit_status_, hist_, bases_, sort_status_, part_counters_;
-    DevBuf row_count_, row_start_, row_scene_, row_status_;
-    DevBuf overflow_, tile_ops_, cov_a_, cov_key_, cov_count_;
+    DevBuf setup_, lengths_, offsets_, scan_status_;
+    // the frame's status block (counters, error flags, paint flags), its host
+    // copy (pinned memory; in arena mode the engine's own `status_local_`) and
+    // whether the host copy is of the frame's end
+    DevBuf status_;
+    void *status_host_ = nullptr;
+    bool status_pinned_ = false, status_final_ = false;
+    alignas(16) unsigned char status_local_[512] = {};
+    bool read_status(size_t bytes, std::string &err);
+    DevBuf segs_a_, segs_b_, hist_, bases_, sort_status_;
+    DevBuf row_start_, row_scene_;
+    DevBuf tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf part_line_;
     DevBuf cnt0_, cnt_acc_, cnt_status_;
     DevBuf flat_cmds_, flat_paths_, flat_counts_, flat_offsets_, flat_status_;

@@ -8,10 +8,39 @@
 
 namespace mg {
 
-// words of the paint flags buffer: overflow flags, record store use, run
-// count, deep tiles and the walk's maxima at 0..6, record counter slices
-// from kStatSlices on
+// words of the paint flags: overflow flags, record store use, run count,
+// deep tiles and the walk's maxima at 0..6, record counter slices from
+// kStatSlices on
 constexpr uint32_t kOverflowWords = 2 * kStatSlices;
+
+// The frame's status block on the device, cleared by the frame's first
+// kernel and read back by one copy at each host synchronisation: counters;
+// error flags (0: a line longer than i32 pixels, or line ranks inconsistent
+// with the layer bases; 1: a look-back wait gave up; 2: the frame exceeded a
+// replayed graph's line-pixel capacity); paint flags.
+struct FrameStatus {
+    FrameCounters ctr;
+    uint32_t err[4];
+    uint32_t ov[kOverflowWords];
+};
+static_assert(sizeof(FrameStatus) % 4 == 0 && sizeof(FrameStatus) <= 512, "status block");
+constexpr size_t kStatusHead = offsetof(FrameStatus, ov);  // counters and error flags
+
+// One launch clearing the given ranges of 32-bit words (empty ones skipped).
+static void zero_words(hipStream_t s, uint32_t cus, std::initializer_list<std::pair<void *, uint64_t>> ranges) {
+    ZeroRanges z{};
+    uint64_t most = 0;
+    for (const auto &r : ranges) {
+        if (!r.first || !r.second) continue;
+        z.p[z.count] = static_cast<uint32_t *>(r.first);
+        z.n[z.count] = r.second;
+        most = std::max(most, r.second);
+        ++z.count;
+    }
+    if (!z.count) return;
+    const uint64_t blocks = std::min<uint64_t>((most + 255) / 256, (uint64_t)std::max(cus, 1u) * 8);
+    k_zero<<<(uint32_t)std::max<uint64_t>(blocks, 1), 256, 0, s>>>(z);
+}
 
 #define MG_HIP(call, what)                                                  \
     do {                                                                    \
@@ -86,9 +115,9 @@ Renderer::Renderer() {}
 
 std::vector<DevBuf *> Renderer::all_buffers() {
     return {&scenes_, &scene_line_base_, &layers_, &props_, &stops_,
-            &setup_, &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_,
-            &segs_b_, &emit_status_, &hist_, &bases_, &sort_status_, &part_counters_,
-            &row_count_, &row_start_, &row_scene_, &row_status_, &overflow_,
+            &setup_, &lengths_, &offsets_, &scan_status_, &status_, &segs_a_,
+            &segs_b_, &hist_, &bases_, &sort_status_,
+            &row_start_, &row_scene_,
             &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
             &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_,
             &walk_scratch_, &part_kept_, &flat_cmds_, &flat_paths_, &flat_counts_, &flat_offsets_, &flat_status_,
@@ -107,6 +136,15 @@ Renderer::~Renderer() {
     for (hipEvent_t e : evs_)
         if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
+    if (status_pinned_) (void)hipHostFree(status_host_);
+}
+
+// Copies the first `bytes` of the status block to its host copy and waits
+// for the stream.
+bool Renderer::read_status(size_t bytes, std::string &err) {
+    MG_HIP(hipMemcpyAsync(status_host_, status_.ptr, bytes, hipMemcpyDeviceToHost, stream_), "read status");
+    MG_HIP(hipStreamSynchronize(stream_), "frame");
+    return true;
 }
 
 bool Renderer::init(std::string &err) {
@@ -141,11 +179,17 @@ bool Renderer::init(std::string &err) {
     MG_HIP(hipEventCreate(&ev0_), "hipEventCreate");
     MG_HIP(hipEventCreate(&ev1_), "hipEventCreate");
     for (hipEvent_t &e : evs_) MG_HIP(hipEventCreate(&e), "hipEventCreate");
-    if (!counters_.reserve(sizeof(FrameCounters)) || !err_.reserve(16) || !overflow_.reserve(kOverflowWords * 4) ||
-        !part_counters_.reserve(64 * sizeof(uint32_t)) || !cov_count_.reserve(16)) {
+    if (!status_.reserve(sizeof(FrameStatus)) || !cov_count_.reserve(16)) {
         err = "out of device memory";
         return false;
     }
+    // the status block's host copy: pinned, except for an engine that must
+    // not allocate (arena mode)
+    status_host_ = status_local_;
+    if (!arena_mode_ && hipHostMalloc(&status_host_, sizeof(FrameStatus), hipHostMallocDefault) == hipSuccess)
+        status_pinned_ = true;
+    else
+        status_host_ = status_local_;
     return true;
 }
 
@@ -161,8 +205,7 @@ void Renderer::arena_begin() {
     rec_cap_ = 0;
     last_sorted_ = nullptr;
     // the frame's fixed counters (their contents are cleared by every frame)
-    (void)(counters_.reserve(sizeof(FrameCounters)) && err_.reserve(16) && overflow_.reserve(kOverflowWords * 4) &&
-           part_counters_.reserve(64 * sizeof(uint32_t)) && cov_count_.reserve(16));
+    (void)(status_.reserve(sizeof(FrameStatus)) && cov_count_.reserve(16));
 }
 
 std::shared_ptr<Renderer> Renderer::shared_arena(void *base, size_t bytes, std::string &err) {
@@ -474,55 +517,56 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         err = "out of device memory (lines)";
         return false;
     }
-    FrameCounters *ctr = counters_.as<FrameCounters>();
-    uint32_t *pc = part_counters_.as<uint32_t>();
+    FrameStatus *fs = status_.as<FrameStatus>();
+    FrameCounters *ctr = &fs->ctr;
+    uint32_t *stall = fs->err + 1;
+    status_final_ = false;
+    const bool ingest = spec.mode == 2;
+    const bool lines = L && !ingest;
+    // render frames place each line at its layer's base plus its rank (lines
+    // ordered by scene and layer order); a length left unwritten would only
+    // come from inconsistent tables, and stays 0
+    if (lines && order_lines && !ranks && !spec.scenes[0].lines_rank) {
+        err = "render frame without line ranks";
+        return false;
+    }
+    const uint64_t nparts_scan = (L + kScanTileHost - 1) / kScanTileHost;
+    const uint32_t count_rows = dims.passes * radix + 2, rest_rows = count_rows - radix;
+    if (!scan_status_.reserve(std::max<uint64_t>(nparts_scan, 1) * 8) || !hist_.reserve(16 * radix * 4) ||
+        !cnt_acc_.reserve((uint64_t)kCountSlicesHost * rest_rows * 4)) {
+        err = "out of device memory";
+        return false;
+    }
     MG_HIP(mark(ev0_), "event");
     MG_HIP(mark(evs_[0]), "event");
-    fill64((uint64_t *)ctr, 0, sizeof(FrameCounters) / 8, stream_);
-    // err_[0]: a line longer than i32 pixels; err_[1]: a look-back wait gave up
-    // err_[2]: the frame exceeded a replayed graph's line-pixel capacity
-    fill32(err_.as<uint32_t>(), 0, 3, stream_);
-    uint32_t *stall = err_.as<uint32_t>() + 1;
-    fill32(pc, 0, 64, stream_);
-    const bool ingest = spec.mode == 2;
-    if (L && !ingest) {
-        // render frames place each line at its layer's base plus its rank
-        // (lines ordered by scene and layer order); a length left unwritten
-        // would only come from inconsistent tables, and stays 0
-        if (order_lines) {
-            if (!ranks && !spec.scenes[0].lines_rank) {
-                err = "render frame without line ranks";
-                return false;
-            }
-            fill32(lengths_.as<uint32_t>(), 0, L, stream_);
-        }
+    // the status block, the lines' lengths, the scan's look-back words, the
+    // digit histograms and the slot-count slices the emission adds into
+    zero_words(stream_, (uint32_t)cus_,
+               {{fs, sizeof(FrameStatus) / 4}, {lengths_.ptr, lines && order_lines ? L : 0},
+                {scan_status_.ptr, lines ? 2 * nparts_scan : 0}, {hist_.ptr, (uint64_t)dims.passes * radix},
+                {cnt_acc_.ptr, (uint64_t)kCountSlicesHost * rest_rows}});
+    if (lines) {
         k_setup<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
             pts, ids, ranks, layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(), scene_line_base_.as<uint32_t>(), dims,
-            setup_.as<LineSetup>(), lengths_.as<uint32_t>(), err_.as<uint32_t>(), order_lines ? 1u : 0u);
-        const uint64_t nparts = (L + kScanTileHost - 1) / kScanTileHost;
-        if (!scan_status_.reserve(nparts * 8)) { err = "out of device memory"; return false; }
-        fill64(scan_status_.as<uint64_t>(), 0, nparts, stream_);
-        k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(lengths_.as<uint32_t>(), offsets_.as<uint32_t>(), L,
-                                                        scan_status_.as<uint64_t>(), stall, &ctr->line_pixels);
+            setup_.as<LineSetup>(), lengths_.as<uint32_t>(), fs->err, order_lines ? 1u : 0u);
+        k_scan_u32<<<(uint32_t)nparts_scan, 256, 0, stream_>>>(lengths_.as<uint32_t>(), offsets_.as<uint32_t>(), L,
+                                                             scan_status_.as<uint64_t>(), stall, &ctr->line_pixels);
     }
     MG_HIP(hipGetLastError(), "setup/scan launch");
-    FrameCounters hc{};
-    uint32_t herr = 0;
+    const FrameStatus &hs = *static_cast<const FrameStatus *>(status_host_);
     uint64_t P = 0;
     if (!ingest && capture_P_) {
         // frame of a graph: buffers and grids for the capacity, no host read
-        k_clamp_line_pixels<<<1, 64, 0, stream_>>>(ctr, capture_P_, err_.as<uint32_t>() + 2);
+        k_clamp_line_pixels<<<1, 64, 0, stream_>>>(ctr, capture_P_, fs->err + 2);
         P = capture_P_;
     } else if (!ingest) {
-        MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read P");
-        MG_HIP(hipMemcpyAsync(&herr, err_.ptr, 4, hipMemcpyDeviceToHost, stream_), "read err");
-        MG_HIP(hipStreamSynchronize(stream_), "setup/scan");
-        if (herr) {
-            err = (herr & 2u) ? "line ranks and layer bases of the frame are inconsistent"
-                              : "a line spans more pixels than an i32 can index";
+        if (!read_status(kStatusHead, err)) return false;
+        if (hs.err[0]) {
+            err = (hs.err[0] & 2u) ? "line ranks and layer bases of the frame are inconsistent"
+                                   : "a line spans more pixels than an i32 can index";
             return false;
         }
-        P = hc.line_pixels;
+        P = hs.ctr.line_pixels;
     } else {
         P = (spec.n_in + 1) / 2;  // capacity: n_in segments
     }
@@ -535,25 +579,20 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     const uint64_t cap_segs = std::max<uint64_t>(2 * P, 1);
     const uint64_t nparts_emit = (P + kEmitTileHost - 1) / kEmitTileHost;
     const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
-    if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) || !hist_.reserve(16 * radix * 4) ||
-        !bases_.reserve(16 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4 * dims.passes) ||
+    if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) || !bases_.reserve(16 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4 * dims.passes) ||
         !row_start_.reserve(((size_t)global_rows + 1) * 4)) {
         err = "out of device memory (segments)";
         return false;
     }
-    fill32(hist_.as<uint32_t>(), 0, dims.passes * radix, stream_);
     // Per-partition slot counts: pass 0 digits digit-major (scanned in place
     // into the first pass's offsets), the rest in kCountSlicesHost slices.
-    const uint32_t count_rows = dims.passes * radix + 2, rest_rows = count_rows - radix;
     const uint64_t n_cnt0 = (uint64_t)radix * nparts_sort;
     const uint64_t cnt_parts = (n_cnt0 + kScanTileHost - 1) / kScanTileHost;
-    if (!cnt0_.reserve(std::max<uint64_t>(n_cnt0, 1) * 4) || !cnt_acc_.reserve((uint64_t)kCountSlicesHost * rest_rows * 4) ||
-        !cnt_status_.reserve(std::max<uint64_t>(cnt_parts, 1) * 8) ||
+    if (!cnt0_.reserve(std::max<uint64_t>(n_cnt0, 1) * 4) || !cnt_status_.reserve(std::max<uint64_t>(cnt_parts, 1) * 8) ||
         !part_kept_.reserve(std::max<uint64_t>(nparts_sort, 1) * 16)) {
         err = "out of device memory (slot counts)";
         return false;
     }
-    fill32(cnt_acc_.as<uint32_t>(), 0, (uint64_t)kCountSlicesHost * rest_rows, stream_);
     // a capacity frame has partitions beyond the frame's: their counts are zero
     if (capture_P_) fill32(cnt0_.as<uint32_t>(), 0, n_cnt0, stream_);
     SlotCounts counts{cnt0_.as<uint32_t>(), cnt_acc_.as<uint32_t>(), part_kept_.as<uint32_t>()};
@@ -575,15 +614,16 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     }
     MG_HIP(hipGetLastError(), "emit launch");
     MG_HIP(mark(evs_[2]), "event");
+    // look-back words of the slot-count scan and of every digit pass,
+    // cleared outside the sort's timing
+    zero_words(stream_, (uint32_t)cus_,
+               {{cnt_status_.ptr, P ? 2 * cnt_parts : 0}, {sort_status_.ptr, nparts_sort * radix * dims.passes}});
     if (P) {
         k_count_reduce<<<rest_rows, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), dims, hist_.as<uint32_t>(), ctr);
-        fill64(cnt_status_.as<uint64_t>(), 0, cnt_parts, stream_);
         k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
                                                            cnt_status_.as<uint64_t>(), stall, &ctr->order_total);
     }
     k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
-    // look-back status words of every digit pass, cleared outside the sort's timing
-    fill32(sort_status_.as<uint32_t>(), 0, nparts_sort * radix * dims.passes, stream_);
     MG_HIP(hipGetLastError(), "histogram launch");
     MG_HIP(mark(evs_[3]), "event");
     uint64_t *in = segs_a_.as<uint64_t>(), *out = segs_b_.as<uint64_t>();
@@ -634,13 +674,16 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
                    err))
             return false;
     }
-    MG_HIP(mark(ev1_), "event");
-    if (capture_P_) return true;  // the replay reads the results
-    MG_HIP(hipMemcpyAsync(&hc, ctr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
-    uint32_t hstall = 0;
-    MG_HIP(hipMemcpyAsync(&hstall, stall, 4, hipMemcpyDeviceToHost, stream_), "read stall");
-    MG_HIP(hipStreamSynchronize(stream_), "frame");
-    if (hstall) {
+    if (capture_P_) {
+        MG_HIP(mark(ev1_), "event");
+        return true;  // the replay reads the results
+    }
+    // (a painted frame's last read was the paint's)
+    if (!status_final_) {
+        MG_HIP(mark(ev1_), "event");
+        if (!read_status(sizeof(FrameStatus), err)) return false;
+    }
+    if (hs.err[1]) {
         err = "a decoupled look-back wait timed out; the frame is invalid";
         return false;
     }
@@ -651,8 +694,8 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         MG_HIP(hipEventElapsedTime(&sm, evs_[k], evs_[k + 1]), "elapsed");
         res.stage_ms[k] = sm;
     }
-    res.segments = hc.segments;
-    res.kept = hc.kept;
+    res.segments = hs.ctr.segments;
+    res.kept = hs.ctr.kept;
     res.device_ms = ms;
     res.runs = n_runs_;
     res.records = n_records_;
@@ -695,7 +738,9 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.out = image;
     pp.col_bits = col_bits;
     pp.layer_bits = layer_bits;
-    pp.overflow = overflow_.as<uint32_t>();
+    FrameStatus *fs = status_.as<FrameStatus>();
+    uint32_t *ov_dev = fs->ov;
+    pp.overflow = ov_dev;
     pp.tile_ops = spec.want_tile_ops ? tile_ops_.as<int8_t>() : nullptr;
     pp.tile_start = tile_start_.as<uint32_t>();
     pp.tile_run0 = tile_run0_.as<uint32_t>();
@@ -705,7 +750,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     pp.run_beg = run_beg_.as<uint32_t>();
     pp.run_cov = run_cov_.as<uint64_t>();
     pp.rec_hdr = rec_hdr_.as<uint2>();
-    pp.rec_used = overflow_.as<uint32_t>() + 1;
+    pp.rec_used = ov_dev + 1;
     pp.work = work_.as<uint2>();
     if (spec.want_coverage) {
         pp.cov_a = cov_a_.as<int32_t>();
@@ -713,17 +758,16 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.cov_count = cov_count_.as<uint32_t>();
         pp.cov_cap = spec.coverage_cap;
     }
-    pp.n_runs = overflow_.as<uint32_t>() + 3;
-    pp.n_records = overflow_.as<uint32_t>() + kStatSlices;
-    fill32(overflow_.as<uint32_t>() + 3, 0, 1, stream_);
+    pp.n_runs = ov_dev + 3;
+    pp.n_records = ov_dev + kStatSlices;
     // Tiles of 8,192 records or more go to the four-wave deep painter (at most
     // two per CU; more stay with the one-wave painter). A tile holds one wave
     // of pixels, so a frame of few deep tiles would otherwise leave the GPU
     // waiting on the serial record loops of a few waves.
     pp.deep_min = spec.want_coverage ? 0u : 8192u;
     pp.deep_cap = 2 * cus_;
-    pp.deep_count = overflow_.as<uint32_t>() + 4;
-    pp.walk_max = overflow_.as<uint32_t>() + 5;
+    pp.deep_count = ov_dev + 4;
+    pp.walk_max = ov_dev + 5;
     if (!deep_list_.reserve((size_t)pp.deep_cap * 8)) {
         err = "out of device memory (deep tile list)";
         return false;
@@ -746,11 +790,10 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         // status words (8 B) then the open-run index (4 B) of each partition
         uint32_t *open = (uint32_t *)(runs_status_.as<uint64_t>() + std::max<uint64_t>(nparts, 1));
         if (nparts)
-            k_runs_scan<<<(uint32_t)nparts, 256, 0, stream_>>>(pp, counters_.as<FrameCounters>(),
-                                                               runs_status_.as<uint64_t>(), err_.as<uint32_t>() + 1,
-                                                               open);
+            k_runs_scan<<<(uint32_t)nparts, 256, 0, stream_>>>(pp, &fs->ctr, runs_status_.as<uint64_t>(),
+                                                               fs->err + 1, open);
         k_run_cov<<<(uint32_t)std::max<uint64_t>(std::min<uint64_t>((nparts + 255) / 256, (uint64_t)cus_ * 4), 1), 256,
-                    0, stream_>>>(pp, counters_.as<FrameCounters>(), open);
+                    0, stream_>>>(pp, &fs->ctr, open);
         k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows, 1u);
         k_tile_nr<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows);
     }
@@ -840,10 +883,12 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.rec = rec_.as<uint4>();
         pp.rec_len = rec_len_.as<uint32_t>();
         pp.rec_cap = rec_cap_;
-        if (attempt > 0 && spec.want_coverage) fill32(cov_count_.as<uint32_t>(), 0, 1, stream_);
-        fill32(overflow_.as<uint32_t>(), 0, 3, stream_);
-        fill32(overflow_.as<uint32_t>() + 4, 0, 3, stream_);
-        fill32(overflow_.as<uint32_t>() + kStatSlices, 0, kStatSlices, stream_);
+        // (the frame's first kernel cleared the flags; a walk again restarts
+        // them, the record store and its counters)
+        if (attempt > 0)
+            zero_words(stream_, (uint32_t)cus_,
+                       {{ov_dev, 3}, {ov_dev + 4, 3}, {ov_dev + kStatSlices, kStatSlices},
+                        {spec.want_coverage ? cov_count_.ptr : nullptr, 1}});
         MG_HIP(mark(evs_[5]), "event");
         const size_t lds = global_walk ? 0 : lds_w;
         if (walk_threads == 64)
@@ -864,9 +909,9 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         MG_HIP(mark(evs_[7]), "event");
         MG_HIP(hipGetLastError(), "paint launch");
         if (capture_P_) return true;  // one attempt; the replay reads the flags
-        uint32_t ov[kOverflowWords] = {};
-        MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, sizeof ov, hipMemcpyDeviceToHost, stream_), "read overflow");
-        MG_HIP(hipStreamSynchronize(stream_), "paint");
+        MG_HIP(mark(ev1_), "event");
+        if (!read_status(sizeof(FrameStatus), err)) return false;
+        const uint32_t *ov = static_cast<const FrameStatus *>(status_host_)->ov;
         res.walk_max_runs = ov[5];
         res.walk_max_ids = ov[6];
         if (!ov[0]) {
@@ -874,6 +919,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
             for (uint32_t k = 0; k < kStatSlices; ++k) n_records_ += ov[kStatSlices + k];
             n_runs_ = ov[3];
             n_deep_ = std::min(ov[4], pp.deep_cap);
+            status_final_ = true;
             return true;
         }
         if (ov[0] & 1u) {
@@ -958,8 +1004,8 @@ bool Renderer::upload_work(const FrameSpec &spec, std::string &err) {
 // while none of them moved.
 std::vector<const void *> Renderer::frame_buffers() const {
     const DevBuf *all[] = {&scenes_, &scene_line_base_, &layers_, &props_, &stops_, &row_scene_, &setup_,
-                           &lengths_, &offsets_, &scan_status_, &counters_, &err_, &segs_a_, &segs_b_, &hist_,
-                           &bases_, &sort_status_, &part_counters_, &row_start_, &overflow_, &part_line_,
+                           &lengths_, &offsets_, &scan_status_, &status_, &segs_a_, &segs_b_, &hist_,
+                           &bases_, &sort_status_, &row_start_, &part_line_,
                            &cnt0_, &cnt_acc_,
                            &cnt_status_, &work_, &runs_work_, &tile_nr_, &runs_status_, &tile_start_, &tile_run0_, &run_key_, &run_beg_,
                            &run_cov_, &rec_, &rec_len_, &rec_hdr_, &deep_list_, &part_kept_};
@@ -1003,12 +1049,10 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
             MG_HIP(hipEventRecord(ev0_, stream_), "event");
             MG_HIP(hipGraphLaunch(graph_exec_, stream_), "graph launch");
             MG_HIP(hipEventRecord(ev1_, stream_), "event");
-            FrameCounters hc{};
-            uint32_t herr[3] = {0, 0, 0}, ov[kOverflowWords] = {};
-            MG_HIP(hipMemcpyAsync(&hc, counters_.ptr, sizeof hc, hipMemcpyDeviceToHost, stream_), "read counters");
-            MG_HIP(hipMemcpyAsync(herr, err_.ptr, 12, hipMemcpyDeviceToHost, stream_), "read flags");
-            MG_HIP(hipMemcpyAsync(ov, overflow_.ptr, sizeof ov, hipMemcpyDeviceToHost, stream_), "read overflow");
-            MG_HIP(hipStreamSynchronize(stream_), "graph frame");
+            if (!read_status(sizeof(FrameStatus), err)) return false;
+            const FrameStatus &hs = *static_cast<const FrameStatus *>(status_host_);
+            const FrameCounters &hc = hs.ctr;
+            const uint32_t *herr = hs.err, *ov = hs.ov;
             if (!herr[0] && !herr[1] && !herr[2] && !ov[0]) {
                 float ms = 0.0f;
                 MG_HIP(hipEventElapsedTime(&ms, ev0_, ev1_), "elapsed");
@@ -1045,10 +1089,9 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
     bool ok = run(spec, pts, ids, ranks, n_lines, image, nullptr, tmp, cerr);
     {
         // the capacity run must complete like the eager one (same image)
-        uint32_t herr[3] = {0, 0, 0}, ov[4] = {0, 0, 0, 0};
-        ok = ok && hipMemcpyAsync(herr, err_.ptr, 12, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
-             hipMemcpyAsync(ov, overflow_.ptr, 16, hipMemcpyDeviceToHost, stream_) == hipSuccess &&
-             hipStreamSynchronize(stream_) == hipSuccess && !herr[0] && !herr[1] && !herr[2] && !ov[0];
+        std::string rerr;
+        const FrameStatus &hs = *static_cast<const FrameStatus *>(status_host_);
+        ok = ok && read_status(sizeof(FrameStatus), rerr) && !hs.err[0] && !hs.err[1] && !hs.err[2] && !hs.ov[0];
     }
     if (!ok) {
         capture_P_ = 0;
@@ -1099,22 +1142,20 @@ bool Renderer::flatten_paths(const PathCommand *cmds, size_t n_cmds, const flat:
                "upload commands");
     MG_HIP(hipMemcpyAsync(flat_paths_.ptr, paths, n_paths * sizeof(flat::PathDesc), hipMemcpyHostToDevice, stream_),
            "upload paths");
-    FrameCounters *ctr = counters_.as<FrameCounters>();
-    fill64((uint64_t *)ctr, 0, sizeof(FrameCounters) / 8, stream_);
-    fill32(err_.as<uint32_t>(), 0, 3, stream_);
-    fill64(flat_status_.as<uint64_t>(), 0, nparts, stream_);
+    FrameStatus *fs = status_.as<FrameStatus>();
+    FrameCounters *ctr = &fs->ctr;
+    zero_words(stream_, (uint32_t)cus_, {{fs, kStatusHead / 4}, {flat_status_.ptr, 2 * (uint64_t)nparts}});
     const uint32_t grid = (uint32_t)((n_paths + 255) / 256);
     k_path_counts<<<grid, 256, 0, stream_>>>(flat_cmds_.as<PathCommand>(), flat_paths_.as<flat::PathDesc>(),
-                                             (uint32_t)n_paths, flat_counts_.as<uint32_t>(), err_.as<uint32_t>());
+                                             (uint32_t)n_paths, flat_counts_.as<uint32_t>(), fs->err);
     k_scan_u32<<<(uint32_t)nparts, 256, 0, stream_>>>(flat_counts_.as<uint32_t>(), flat_offsets_.as<uint32_t>(),
-                                                    n_paths, flat_status_.as<uint64_t>(), err_.as<uint32_t>() + 1,
+                                                    n_paths, flat_status_.as<uint64_t>(), fs->err + 1,
                                                     &ctr->line_pixels);
     MG_HIP(hipGetLastError(), "flatten count launch");
-    uint64_t total = 0;
-    uint32_t herr[2] = {0, 0};
-    MG_HIP(hipMemcpyAsync(&total, &ctr->line_pixels, 8, hipMemcpyDeviceToHost, stream_), "read line count");
-    MG_HIP(hipMemcpyAsync(herr, err_.ptr, 8, hipMemcpyDeviceToHost, stream_), "read flags");
-    MG_HIP(hipStreamSynchronize(stream_), "flatten count");
+    if (!read_status(kStatusHead, err)) return false;
+    const FrameStatus &hs = *static_cast<const FrameStatus *>(status_host_);
+    const uint64_t total = hs.ctr.line_pixels;
+    const uint32_t herr[2] = {hs.err[0], hs.err[1]};
     if (herr[0] || herr[1] || base + total > 0xFFFFFFFFull) {
         err = herr[1] ? "a look-back wait timed out" : "too many lines";
         return false;
