@@ -148,8 +148,11 @@ struct mg_composition {
     uint64_t spec_version = 0;
     int32_t spec_cache_id = -2;
     bool spec_clips = false;
+    bool spec_opaque = true;  // some layer of the tables has PF_OPAQUE
 
     size_t n_lines() const { return ids.size() + dev_extra; }
+
+    uint32_t scene_flags() const { return (spec_clips ? mg::SF_CLIPS : 0) | (spec_opaque ? 0 : mg::SF_NO_OPAQUE); }
 
     // Copies device-appended lines into the host arrays, for host work on
     // them (inserting host lines after them, compaction, estimates).
@@ -301,6 +304,7 @@ struct mg_composition {
         f.props.assign(n_props, mg::PropsEntry{});
         f.stops.clear();
         has_clips = false;
+        spec_opaque = false;
         uint32_t max_order = 0;
         for (auto &kv : orders_to_layers) {
             auto it = layers.find(kv.second);
@@ -331,6 +335,7 @@ struct mg_composition {
                     f.stops.insert(f.stops.end(), l.props.stops.begin(), l.props.stops.end());
                 } else if (p.blend_mode == MG_BLEND_OVER && p.color[3] == 1.0f) {
                     fl |= mg::PF_OPAQUE;
+                    spec_opaque = true;
                 }
             }
             if (cache_id >= 0 && ((l.is_unchanged >> cache_id) & 1u)) fl |= mg::PF_UNCHANGED;
@@ -791,7 +796,6 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     std::string err;
     prepare_tables(c, cache_id);
     mg::FrameSpec &f = c->spec;
-    const bool has_clips = c->spec_clips;
     f.scenes.clear();
     f.scene_line_base.clear();
     f.tables_owner = c->owner;
@@ -805,7 +809,7 @@ static mg_status render_frame(mg_composition *c, size_t width, size_t stride, si
     sd.n_props = (uint32_t)f.props.size();
     sd.row_base = 0;
     sd.tile_base = 0;
-    sd.flags = (has_clips ? mg::SF_CLIPS : 0) | (cache_id >= 0 ? mg::SF_CACHE : 0);
+    sd.flags = c->scene_flags() | (cache_id >= 0 ? mg::SF_CACHE : 0);
     sd.out_offset = 0;
     f.scenes.push_back(sd);
     f.scene_line_base.push_back(0);
@@ -991,7 +995,7 @@ MG_EXPORT mg_status mg_render_device_noalloc(mg_composition *c, uint32_t width, 
     sd.n_lines = (uint32_t)n;
     sd.n_layer_entries = (uint32_t)f.layers.size();
     sd.n_props = (uint32_t)f.props.size();
-    sd.flags = c->spec_clips ? mg::SF_CLIPS : 0;
+    sd.flags = c->scene_flags();
     f.scenes.push_back(sd);
     f.scene_line_base.push_back(0);
     f.mode = 0;
@@ -1246,7 +1250,7 @@ mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_f
         sd.lines_pts = (uint64_t)(uintptr_t)c->dlines.points();
         sd.lines_ids = (uint64_t)(uintptr_t)c->dlines.layer_ids();
         sd.lines_rank = (uint64_t)(uintptr_t)c->dlines.rank.as<uint32_t>();
-        sd.flags = c->spec_clips ? mg::SF_CLIPS : 0;
+        sd.flags = c->scene_flags();
         sd.out_offset = out;
         const uint32_t stops_base = (uint32_t)f.stops.size();
         for (mg::LayerEntry e : cs.layers) {

@@ -2437,6 +2437,9 @@ __device__ __forceinline__ void paint_tile(const PaintParams &pp, uint32_t grow,
         // skip_fully_covered_layers (layer_workbench.rs:310-388)
         int first = 0;  // 0 unset, 1 opaque, 2 incomplete
         int32_t k_opaque = -1;
+        // without opaque layers in the scene, the search only decides
+        // whether the tile is solid: it ends at the first incomplete id
+        const bool no_opaque = (S.flags & SF_NO_OPAQUE) != 0;
         for (int32_t top = (int32_t)n - 1; top >= 0; top -= 64) {
             const int32_t k = top - (int32_t)lane;
             bool pI = false, pO = false;
@@ -2452,6 +2455,7 @@ __device__ __forceinline__ void paint_tile(const PaintParams &pp, uint32_t grow,
             const uint64_t bI = __ballot(pI), bO = __ballot(pO);
             const uint64_t before = bO ? ((1ull << (__ffsll((long long)bO) - 1)) - 1ull) : ~0ull;
             if (first == 0 && (bI & before)) first = 2;
+            if (no_opaque && first == 2) break;
             if (bO) {
                 const uint32_t l = (uint32_t)__ffsll((long long)bO) - 1;
                 k_opaque = top - (int32_t)l;

@@ -56,7 +56,9 @@ struct PropsEntry {
 };
 
 // One scene of a frame (a plain render is a batch of one scene).
-enum : uint32_t { SF_CLIPS = 1, SF_CACHE = 2, SF_GRADIENTS = 4 };
+// SF_NO_OPAQUE: no layer of the scene has PF_OPAQUE (the painter's search for
+// the topmost opaque layer of a tile can stop early)
+enum : uint32_t { SF_CLIPS = 1, SF_CACHE = 2, SF_GRADIENTS = 4, SF_NO_OPAQUE = 8 };
 struct SceneDesc {
     uint32_t line_base, n_lines;
     uint32_t layer_base, n_layer_entries;
