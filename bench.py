@@ -235,6 +235,9 @@ def bench_batched(cassia_amd, workloads, steps, warmup, n_scenes, plain=True):
                                          for k in range(tn.n_walk_candidates)},
                      "chosen_walk_threads": int(tn.chosen_walk_threads), "seconds": round(tune_s, 3)},
         "walk_threads": int(st.walk_threads), "walk_lds": int(st.walk_lds),
+        # dispatches of one batched frame (kernels, clears and status reads),
+        # from the committed kernel trace of this workload
+        "dispatches_per_frame": profile_value("kernel-trace_glyphs-4096-scenes", "dispatches_per_frame"),
         "stage_ms": {k: round(getattr(st, k), 3) for k in ("setup_ms", "emit_ms", "hist_ms", "sort_ms", "runs_ms",
                                                              "walk_ms", "paint_ms")},
         "scene_build_s": round(t_build, 2),
@@ -431,29 +434,29 @@ def bench_flatten(cassia_amd, n_paths=100_000, seed=12):
             **out, "speedup": out["host"]["ms"] / out["device"]["ms"], "identical": same}
 
 
+def profile_value(summary, key, default=None):
+    """The JSON value of the `key:` line of a committed rocprofv3 summary
+    (profiles/<summary>.md), or `default` when there is none."""
+    path = os.path.join(ROOT, "profiles", f"{summary}.md")
+    if not os.path.exists(path):
+        return default
+    for line in open(path):
+        if line.startswith(key + ":"):
+            return json.loads(line.split(":", 1)[1])
+    return default
+
+
 def measured_traffic(workload):
     """HBM bytes per launch per kernel from the committed rocprofv3 PMC
     summary of this workload (FETCH_SIZE and WRITE_SIZE runs, each scaled by
     the counter's reading for a 1 GiB device copy), or {} if there is none."""
-    path = os.path.join(ROOT, "profiles", f"hbm-traffic_{workload}.md")
-    if not os.path.exists(path):
-        return {}
-    for line in open(path):
-        if line.startswith("traffic_bytes_per_launch:"):
-            return json.loads(line.split(":", 1)[1])
-    return {}
+    return profile_value(f"hbm-traffic_{workload}", "traffic_bytes_per_launch", {})
 
 
 def measured_valu(workload):
     """Wave-level VALU instructions per launch per kernel from the committed
     rocprofv3 SQ counter summary of this workload, or {}."""
-    path = os.path.join(ROOT, "profiles", f"sq-instructions_{workload}.md")
-    if not os.path.exists(path):
-        return {}
-    for line in open(path):
-        if line.startswith("valu_per_launch:"):
-            return json.loads(line.split(":", 1)[1])
-    return {}
+    return profile_value(f"sq-instructions_{workload}", "valu_per_launch", {})
 
 
 def bench_circles(cassia_amd, workloads, steps, warmup):
@@ -601,6 +604,9 @@ def main():
     roofline = {"bound": "hbm", "achieved": d["gbs"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(hbm_frac, 4), "traffic": traffic, "kernel": dominant,
                 "ms_per_launch": d["ms_per_launch"], "bytes_per_launch": d["bytes_per_launch"]}
+    if profile_name:
+        # the streaming-copy rate measured on this device (mg_stream_copy)
+        roofline["copy_ceiling_gbs"] = profile_value(f"hbm-traffic_{profile_name}", "copy_ceiling_gbs")
     if dominant in valu:
         # vector-ALU issue: the kernel's wave-level VALU instructions (SQ
         # counters of the committed profile) per second of its measured time
@@ -640,6 +646,10 @@ def main():
         "walk_threads": int(st.walk_threads),
         "walk_lds": ["share", "half", "whole", "global"][int(st.walk_lds)],
         "setup_hist_ms": round(st.setup_ms + st.hist_ms, 4),
+        # dispatches of one frame (kernels, clears and status reads), from
+        # the committed kernel trace of this workload
+        "dispatches_per_frame": (profile_value(f"kernel-trace_{profile_name}", "dispatches_per_frame")
+                                 if profile_name else None),
         "kernels": kernels,
         "roofline": roofline,
         "scene_build_s": round(t_build, 2),
