@@ -1647,20 +1647,24 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
     unsigned char *p = G ? pp.walk_scratch + (uint64_t)blockIdx.x * pp.walk_stride : lds_raw;
     // two queues (the next column's is written while the current one is
     // read) and one or two sets of run arrays (pp.walk_run_bufs; with two,
-    // the next column's runs are stored while the current ones are read)
+    // the next column's runs are stored while the current ones are read).
+    // Array k of a pair follows array 0 (queues: cap entries, runs: rcap), so
+    // each is addressed from one base: offsets from lds_raw keep the accesses
+    // LDS instructions (pointers picked from an array by a run-time index
+    // became generic pointers, whose flat accesses also wait for every
+    // outstanding global load and store).
     const uint32_t nb = pp.walk_run_bufs;
-    uint64_t *qc[2], *rcs[2];
-    qc[0] = (uint64_t *)p; p += (size_t)cap * 8;
-    qc[1] = (uint64_t *)p; p += (size_t)cap * 8;
-    rcs[0] = (uint64_t *)p; p += (size_t)rcap * 8;
-    rcs[1] = nb == 2 ? (uint64_t *)p : rcs[0]; p += (size_t)rcap * 8 * (nb - 1);
-    uint32_t *qk[2], *rks[2], *rbs[2];
-    qk[0] = (uint32_t *)p; p += (size_t)cap * 4;
-    qk[1] = (uint32_t *)p; p += (size_t)cap * 4;
-    rks[0] = (uint32_t *)p; p += (size_t)rcap * 4;
-    rks[1] = nb == 2 ? (uint32_t *)p : rks[0]; p += (size_t)rcap * 4 * (nb - 1);
-    rbs[0] = (uint32_t *)p; p += (size_t)rcap * 4;
-    rbs[1] = nb == 2 ? (uint32_t *)p : rbs[0]; p += (size_t)rcap * 4 * (nb - 1);
+    uint64_t *const qc0 = (uint64_t *)p; p += (size_t)cap * 8 * 2;
+    uint64_t *const rc0 = (uint64_t *)p; p += (size_t)rcap * 8 * nb;
+    uint32_t *const qk0 = (uint32_t *)p; p += (size_t)cap * 4 * 2;
+    uint32_t *const rk0 = (uint32_t *)p; p += (size_t)rcap * 4 * nb;
+    uint32_t *const rb0 = (uint32_t *)p; p += (size_t)rcap * 4 * nb;
+    // (buf is 0 when there is one set of run arrays)
+    auto qk = [&](uint32_t k) { return qk0 + (size_t)k * cap; };
+    auto qc = [&](uint32_t k) { return qc0 + (size_t)k * cap; };
+    auto rks = [&](uint32_t b) { return rk0 + (size_t)b * rcap; };
+    auto rbs = [&](uint32_t b) { return rb0 + (size_t)b * rcap; };
+    auto rcs = [&](uint32_t b) { return rc0 + (size_t)b * rcap; };
     uint32_t *ts_l = (uint32_t *)p; p += (size_t)tstride * 4;
     uint32_t *nr_l = (uint32_t *)p; p += (size_t)tstride * 4;
     uint32_t *r0_l = (uint32_t *)p;
@@ -1735,15 +1739,15 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
             return false;
         }
         if (tid < f.n) {
-            rks[buf][tid] = f.key;
-            rbs[buf][tid] = f.beg;
-            rcs[buf][tid] = f.cov;
+            rks(buf)[tid] = f.key;
+            rbs(buf)[tid] = f.beg;
+            rcs(buf)[tid] = f.cov;
         }
         const uint32_t r0 = r0_l[col];
         for (uint32_t k = f.n + tid; k < nr; k += T) {
-            rks[buf][k] = pp.run_key[r0 + k];
-            rbs[buf][k] = pp.run_beg[r0 + k];
-            rcs[buf][k] = pp.run_cov[r0 + k];
+            rks(buf)[k] = pp.run_key[r0 + k];
+            rbs(buf)[k] = pp.run_beg[r0 + k];
+            rcs(buf)[k] = pp.run_cov[r0 + k];
         }
         return true;
     };
@@ -1766,10 +1770,10 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
             return false;
         }
         const uint32_t ts = ts_l[colp], te = ts_l[colp + 1];
-        const uint32_t *rk = rks[buf], *rb = rbs[buf];
-        const uint64_t *rc = rcs[buf];
-        const uint32_t *qkc = qk[cur];
-        const uint64_t *qcc = qc[cur];
+        const uint32_t *rk = rks(buf), *rb = rbs(buf);
+        const uint64_t *rc = rcs(buf);
+        const uint32_t *qkc = qk(cur);
+        const uint64_t *qcc = qc(cur);
         const uint32_t total = nq + nr;
         const uint32_t per = (total + T - 1) / T;
         const uint32_t d0 = tid * per < total ? tid * per : total;
@@ -1884,8 +1888,8 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
                 ++ou;
             }
             if (keep) {
-                qk[nxt][oq] = lay_eo;
-                qc[nxt][oq] = v;
+                qk(nxt)[oq] = lay_eo;
+                qc(nxt)[oq] = v;
                 ++oq;
             }
         };
