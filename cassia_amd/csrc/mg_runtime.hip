@@ -815,11 +815,22 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     const uint32_t rcap_fit = (uint32_t)((orders + 63) & ~63ull);
     const bool fits = orders <= 2048 && walk_lds_bytes_host(2 * rcap_fit, rcap_fit, tstride, 1) + 256 <= lds_max;
     uint32_t walk_threads = spec.walk_threads;
-    if (walk_threads != 64 && walk_threads != 256 && walk_threads != 512 && walk_threads != 1024)
-        // (a 3840x2160 frame of 200,000 polygons, about 700 queue entries
-        // and runs per tile: 1024 threads 2.16 ms, 512 threads 1.95 ms, 256
-        // threads 4.9 ms, which need the whole LDS)
-        walk_threads = fits ? 256 : 512;
+    if (walk_threads != 64 && walk_threads != 256 && walk_threads != 512 && walk_threads != 1024) {
+        // Each row is a chain of dependent steps, so the fewer rows a frame
+        // has per CU, the more threads a row takes. Measured on MI355X: a
+        // 64x64 canvas of 330,000 lines (8 rows) walks in 0.34 ms with 1,024
+        // threads, 0.49 with 512 (a 1024x1024 one, 128 rows: 0.32 against
+        // 0.27 with 512); 3840x2160 with 200,000 polygons (270 rows)
+        // 1.81 / 1.71 / 4.1 ms with 1,024 / 512 / 256 threads (256 need the
+        // whole LDS there); a 1920x1080 animation (135 rows, small share)
+        // 0.32 ms with 512, 0.36 with 256; frames of many more rows than CUs
+        // gain nothing from more than 256 threads when their arrays fit a
+        // small share.
+        const uint32_t cus = (uint32_t)std::max(cus_, 1);
+        if (global_rows <= cus / 4) walk_threads = 1024;
+        else if (global_rows <= 4 * cus) walk_threads = 512;
+        else walk_threads = fits ? 256 : 512;
+    }
     const bool small_walk = fits;
     // The large configuration keeps two sets of run arrays so that the
     // next column's runs are stored while the current column is merged (one
