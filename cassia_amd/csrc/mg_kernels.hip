@@ -1638,7 +1638,7 @@ __device__ __forceinline__ uint32_t merge_split(const uint32_t *qk, uint32_t nq,
 // 1024 threads in LDS: at most 64 VGPRs and few enough SGPRs for eight
 // waves per SIMD, two work groups (two tile rows) per CU.
 template <uint32_t T, bool G>
-__global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintParams pp) {
+__global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     __shared__ uint64_t s_wsum[T / 64];
     __shared__ uint32_t s_rec[3];
@@ -1779,59 +1779,90 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
         const uint32_t d0 = tid * per < total ? tid * per : total;
         const uint32_t d1 = d0 + per < total ? d0 + per : total;
         const uint32_t i0 = merge_split(qkc, nq, rk, nr, d0);
-        // union element -> (q index or -1, r index or -1); second halves of
-        // equal-layer pairs are skipped
+        // One element of the queue ∪ runs union: layer | kRunEvenOdd, its
+        // queue and run indices (-1: none), the carried cover `a` and the run's
+        // cover `b`, and the run's words [beg, beg + len). Second halves of
+        // equal-layer pairs are skipped.
+        struct Elem {
+            uint32_t lay_eo;
+            int32_t qi, ri;
+            uint64_t a, b;
+            uint32_t beg, len;
+        };
+        // The element at merge position (i, r), from one round of LDS reads
+        // (positions past an array's end read its entry 0, unused), built with
+        // selects: `take_q` when it comes from the queue (else from the runs),
+        // `skip` for a run whose layer the previous queue element had (merged
+        // there).
+        auto element = [&](uint32_t i, uint32_t r, bool &take_q, bool &skip) {
+            const uint32_t iq = i < nq ? i : 0u, ir = r < nr ? r : 0u, ip = i > 0 ? i - 1u : 0u;
+            const uint32_t ir1 = ir + 1u < nr ? ir + 1u : ir;
+            const uint32_t kq = qkc[iq], kr = rk[ir], kp = qkc[ip];
+            const uint64_t ca = qcc[iq], cb = rc[ir];
+            const uint32_t b0 = rb[ir], b1 = ir + 1u < nr ? rb[ir1] : te;
+            const uint32_t lq = kq & 0xFFFFu, lr = kr & 0xFFFFu;
+            take_q = i < nq && (r >= nr || lq <= lr);
+            const bool with_run = r < nr && (!take_q || lr == lq);
+            skip = !take_q && i > 0 && (kp & 0xFFFFu) == lr;
+            Elem e;
+            e.lay_eo = (take_q ? kq : kr) & (0xFFFFu | kRunEvenOdd);
+            e.qi = take_q ? (int32_t)i : -1;
+            e.ri = with_run ? (int32_t)r : -1;
+            e.a = take_q ? ca : 0ull;
+            e.b = with_run ? cb : 0ull;
+            e.beg = b0;
+            e.len = b1 - b0;
+            return e;
+        };
         auto walk = [&](auto &&emit) {
             uint32_t i = i0, r = d0 - i0;
             for (uint32_t m = d0; m < d1; ++m) {
-                const bool take_q = i < nq && (r >= nr || (qkc[i] & 0xFFFFu) <= (rk[r] & 0xFFFFu));
-                if (take_q) {
-                    const uint32_t lay = qkc[i] & 0xFFFFu;
-                    const int32_t ri = (r < nr && (rk[r] & 0xFFFFu) == lay) ? (int32_t)r : -1;
-                    emit(lay, (int32_t)i, ri);
-                    ++i;
-                } else {
-                    const uint32_t lay = rk[r] & 0xFFFFu;
-                    if (!(i > 0 && (qkc[i - 1] & 0xFFFFu) == lay)) emit(lay, -1, (int32_t)r);
-                    ++r;
-                }
+                bool take_q, skip;
+                const Elem e = element(i, r, take_q, skip);
+                if (!skip) emit(e);
+                i += take_q ? 1u : 0u;
+                r += take_q ? 0u : 1u;
             }
         };
-        // next cover of a union element, and its fill rule (kRunEvenOdd bit)
-        auto next_cover = [&](int32_t qi, int32_t ri, bool *keep, uint32_t *eo_bit) {
-            const uint64_t a = qi >= 0 ? qcc[qi] : 0ull, b = ri >= 0 ? rc[ri] : 0ull;
-            const uint64_t v = add_bytes(a, b);
-            *eo_bit = (qi >= 0 ? qkc[qi] : rk[ri]) & kRunEvenOdd;
-            *keep = fold || !cover_empty(v, *eo_bit != 0);
+        // next cover of a union element
+        auto next_cover = [&](const Elem &e, bool *keep) {
+            const uint64_t v = add_bytes(e.a, e.b);
+            *keep = fold || !cover_empty(v, (e.lay_eo & kRunEvenOdd) != 0);
             return v;
         };
-        // The first two elements of the thread's slice keep what the writing
-        // pass needs in registers (one element per thread is the common case
-        // with 1,024 threads); longer slices are walked again.
-        uint32_t e_lay[2] = {0u, 0u};
-        int32_t e_qi[2] = {-1, -1}, e_ri[2] = {-1, -1};
-        uint64_t e_v[2] = {0ull, 0ull};
+        // The first two elements of the thread's slice stay in registers for
+        // the writing pass (one element per thread is the common case);
+        // longer slices are walked again.
+        Elem e0{}, e1{};
+        uint64_t ev0 = 0, ev1 = 0;
         uint32_t cu = 0, cq = 0, e_keep = 0;
-        walk([&](uint32_t lay, int32_t qi, int32_t ri) {
-            bool keep;
-            uint32_t eo_bit;
-            const uint64_t v = next_cover(qi, ri, &keep, &eo_bit);
-            if (cu == 0) {
-                e_lay[0] = lay | eo_bit;
-                e_qi[0] = qi;
-                e_ri[0] = ri;
-                e_v[0] = v;
-                e_keep |= keep ? 1u : 0u;
-            } else if (cu == 1) {
-                e_lay[1] = lay | eo_bit;
-                e_qi[1] = qi;
-                e_ri[1] = ri;
-                e_v[1] = v;
-                e_keep |= keep ? 2u : 0u;
+        if (per == 1) {
+            // (uniform) at most one element per thread: no loop
+            if (d0 < d1) {
+                bool take_q, skip, keep;
+                e0 = element(i0, d0 - i0, take_q, skip);
+                ev0 = next_cover(e0, &keep);
+                cu = skip ? 0u : 1u;
+                cq = !skip && keep ? 1u : 0u;
+                e_keep = cq;
             }
-            ++cu;
-            cq += keep ? 1u : 0u;
-        });
+        } else {
+            walk([&](const Elem &e) {
+                bool keep;
+                const uint64_t v = next_cover(e, &keep);
+                if (cu == 0) {
+                    e0 = e;
+                    ev0 = v;
+                    e_keep |= keep ? 1u : 0u;
+                } else if (cu == 1) {
+                    e1 = e;
+                    ev1 = v;
+                    e_keep |= keep ? 2u : 0u;
+                }
+                ++cu;
+                cq += keep ? 1u : 0u;
+            });
+        }
         // (union entries, kept queue entries) in the low and high halves:
         // 16 bits each when the LDS arrays bound them (cap < 2^16)
         uint64_t ex, all;
@@ -1869,39 +1900,39 @@ __global__ __launch_bounds__(T, T == 1024 && !G ? 8 : 1) void k_row_walk(PaintPa
         const bool write_rec = !fold && !counting;
         const uint32_t nxt = cur ^ 1u;
         uint32_t ou = rec_off + (uint32_t)ex, oq = (uint32_t)(ex >> 32);
-        // lay_eo: layer | kRunEvenOdd bit (the queue entry's key)
-        auto put = [&](uint32_t lay_eo, int32_t qi, int32_t ri, uint64_t v, bool keep) {
+        // the element's id record and, when its cover stays, its queue entry
+        // (key: layer | kRunEvenOdd)
+        auto put = [&](const Elem &e, uint64_t v, bool keep) {
             if (write_rec) {
-                uint32_t key = (lay_eo & 0xFFFFu) | ((lay_eo & kRunEvenOdd) ? kRecEvenOdd : 0u), beg = 0, len = 0;
+                uint32_t key = (e.lay_eo & 0xFFFFu) | ((e.lay_eo & kRunEvenOdd) ? kRecEvenOdd : 0u), beg = 0, len = 0;
                 uint64_t cv = 0;
-                if (ri >= 0) {
+                if (e.ri >= 0) {
                     key |= kRecRun;
-                    beg = rb[ri];
-                    len = ((uint32_t)ri + 1 < nr ? rb[ri + 1] : te) - beg;
+                    beg = e.beg;
+                    len = e.len;
                 }
-                if (qi >= 0) {
+                if (e.qi >= 0) {
                     key |= kRecCarry;
-                    cv = qcc[qi];
+                    cv = e.a;
                 }
                 pp.rec[ou] = make_uint4(key, beg, (uint32_t)cv, (uint32_t)(cv >> 32));
                 pp.rec_len[ou] = len;
                 ++ou;
             }
             if (keep) {
-                qk(nxt)[oq] = lay_eo;
+                qk(nxt)[oq] = e.lay_eo;
                 qc(nxt)[oq] = v;
                 ++oq;
             }
         };
         if (cu <= 2) {
-            if (cu > 0) put(e_lay[0], e_qi[0], e_ri[0], e_v[0], (e_keep & 1u) != 0);
-            if (cu > 1) put(e_lay[1], e_qi[1], e_ri[1], e_v[1], (e_keep & 2u) != 0);
+            if (cu > 0) put(e0, ev0, (e_keep & 1u) != 0);
+            if (cu > 1) put(e1, ev1, (e_keep & 2u) != 0);
         } else {
-            walk([&](uint32_t lay, int32_t qi, int32_t ri) {
+            walk([&](const Elem &e) {
                 bool keep;
-                uint32_t eo_bit;
-                const uint64_t v = next_cover(qi, ri, &keep, &eo_bit);
-                put(lay | eo_bit, qi, ri, v, keep);
+                const uint64_t v = next_cover(e, &keep);
+                put(e, v, keep);
             });
         }
         if (write_rec && tid == 0) {
