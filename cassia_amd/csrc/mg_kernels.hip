@@ -1846,6 +1846,31 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
                 cq = !skip && keep ? 1u : 0u;
                 e_keep = cq;
             }
+        } else if (per == 2) {
+            // (uniform) at most two elements per thread: the second after the
+            // first, moved into the first slot when the first is skipped
+            bool skip_a = true, skip_b = true, keep_a = false, keep_b = false;
+            Elem ea{}, eb{};
+            uint64_t va = 0, vb = 0;
+            if (d0 < d1) {
+                bool take_q;
+                ea = element(i0, d0 - i0, take_q, skip_a);
+                va = next_cover(ea, &keep_a);
+                if (d0 + 1 < d1) {
+                    bool take_q_b;
+                    const uint32_t i1 = i0 + (take_q ? 1u : 0u);
+                    eb = element(i1, d0 + 1 - i1, take_q_b, skip_b);
+                    vb = next_cover(eb, &keep_b);
+                }
+            }
+            cu = (skip_a ? 0u : 1u) + (skip_b ? 0u : 1u);
+            cq = (!skip_a && keep_a ? 1u : 0u) + (!skip_b && keep_b ? 1u : 0u);
+            e0 = skip_a ? eb : ea;
+            ev0 = skip_a ? vb : va;
+            e1 = eb;
+            ev1 = vb;
+            const bool k0 = skip_a ? keep_b : keep_a;
+            e_keep = (k0 ? 1u : 0u) | (!skip_a && keep_b ? 2u : 0u);
         } else {
             walk([&](const Elem &e) {
                 bool keep;
