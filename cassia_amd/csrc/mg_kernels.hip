@@ -1207,8 +1207,16 @@ constexpr uint32_t kMaskBit = 1u << 30, kSkipClipBit = 1u << 31;
 constexpr uint32_t kRunEvenOdd = 1u << 16;                    // run key: layer | even_odd << 16
 constexpr uint32_t kRecRun = 1u << 16, kRecCarry = 1u << 17;  // record key flags
 constexpr uint32_t kRecEvenOdd = 1u << 18;                     // record key: the layer's fill rule is even-odd
-constexpr uint32_t kStageWin = 512;                           // staged words per paint wave
-constexpr uint32_t kPaintWaveLds = kStageWin * 8 + 2 * 64 * 4;  // stage window + cells of one wave
+constexpr uint32_t kStageWin = 256;                           // staged words per paint wave
+// Run groups of the paint (group_build): slots of kGroupStride words, runs of
+// at most kGroupLenMax words. A stride of 8 mod 32 words puts the 8 slots'
+// rows in different banks for the prefix pass.
+constexpr uint32_t kGroupRuns = 8, kGroupStride = 72, kGroupLenMax = 256;
+// stage window + run-group slots of one wave (the per-run cells share the
+// slots): 4.25 KB, so that LDS allows more than the 8 waves per SIMD that the
+// paint's 64 VGPRs allow
+constexpr uint32_t kPaintWaveLds = kStageWin * 8 + kGroupRuns * kGroupStride * 4;
+static_assert(kGroupRuns * kGroupStride >= 2 * 64 && kGroupRuns <= 8, "per-run cells fit in the group slots");
 constexpr uint32_t kDeepFlag = 1u << 31;                      // record header: tile of the deep list
 constexpr uint32_t kDeepChunk = 192;                          // deep paint: records per round
 
@@ -1216,7 +1224,7 @@ size_t walk_lds_bytes_host(uint32_t cap, uint32_t rcap, uint32_t tstride, uint32
     return (size_t)cap * 24 + (size_t)rcap * 16 * run_bufs + (size_t)tstride * 12 + 64;
 }
 size_t paint_lds_bytes_host(uint32_t cap, bool clips) {
-    return (size_t)kStageWin * 8 + 2 * 64 * 4 + (clips ? (size_t)cap * 4 : 0);
+    return (size_t)kPaintWaveLds + (clips ? (size_t)cap * 4 : 0);
 }
 
 __device__ __forceinline__ int32_t cov_byte(uint64_t c, uint32_t y) { return as_i8((int32_t)(c >> (8 * y))); }
@@ -2088,6 +2096,8 @@ struct Batch {
     uint32_t cnt;    // records of the batch (wave-uniform)
     uint32_t first;  // first word of the batch's runs
     uint32_t lim;    // end of the batch's last run (lim == first: no run)
+    uint64_t rm;     // lanes of the batch whose record has a run
+    uint64_t longm;  // of those, runs longer than kGroupLenMax words
 };
 
 // Without Window the batch is simply the next 64 records (for a reader that
@@ -2117,6 +2127,7 @@ __device__ __forceinline__ void load_batch(const PaintParams &pp, const PropsEnt
     const uint64_t rm = __ballot(run);
     b.cnt = (uint32_t)__builtin_amdgcn_readfirstlane((int)avail);
     b.first = b.lim = 0;
+    b.rm = b.longm = 0;
     if (Window && rm) {
         b.first = rlane(b.beg, (uint32_t)__ffsll((long long)rm) - 1u);
         // the first run ending beyond the window closes the batch (a run
@@ -2130,6 +2141,8 @@ __device__ __forceinline__ void load_batch(const PaintParams &pp, const PropsEnt
         }
         const uint32_t last = 63u - (uint32_t)__clzll((long long)used);
         b.lim = used ? rlane(b.beg + b.len, last) : b.first;
+        b.rm = used;
+        b.longm = used & __ballot(b.len > kGroupLenMax);
     }
 }
 
@@ -2146,6 +2159,109 @@ __device__ __forceinline__ int32_t record_area(const Stage &st, int32_t *cells, 
     const int32_t cy = as_i8((int32_t)(rlane64(b.cov, q) >> (8 * py)));
     if (!(key & kRecRun)) return 16 * cy;
     return run_area(st, cells, segs, rlane(b.beg, q), rlane(b.len, q), cy);
+}
+
+// Run groups: the area sums of up to kGroupRuns consecutive runs of a batch
+// for every pixel, accumulated in one pass over the runs' staged words (64
+// words per step, lane = word) instead of one pass per run. The word of
+// (slot, pixel row y, column x), at y*8 + x in the slot, holds
+//   bits  0..15  the pixel's area sum mod 2^16 (the painter's i16 area),
+//   bits 16..23  the carries out of bit 15: at most one per word of the
+//                pixel, so a run of at most kGroupLenMax words never carries
+//                into
+//   bits 24..31  the cover sum mod 2^8 (the painter's i8 cover),
+// and a pass per (slot, pixel row) then replaces each cover byte with the sum
+// of the row's covers in the columns to its left, so that a record reads
+// 16 * P + area (run_area's value) from one word. The runs of a batch are
+// consecutive stretches of the sorted words, one key each, so a word's slot
+// is the number of key changes between the group's first word and it.
+//
+// group_build serves the records of a staged batch from record q on and
+// returns the record after those it serves:
+//   - no run at or after q: the batch's end (carried covers only);
+//   - the first run from q on is longer than kGroupLenMax words: the record
+//     after it, which takes run_area (whose cells share the slots);
+//   - else it fills the slots with up to kGroupRuns runs from that one on,
+//     ending before the next longer run, sets *r0 to the rank of the first
+//     (slot 0) and returns the record of the first run after the group.
+__device__ __forceinline__ uint32_t group_build(uint32_t *slots, const Stage &st, const Batch &b, uint32_t q,
+                                                uint32_t *r0) {
+    const uint32_t lane = lane_id();
+    const uint64_t from_q = b.rm & (~0ull << q);  // q < 64
+    if (!from_q) return b.cnt;
+    const uint32_t q1 = (uint32_t)__ffsll((long long)from_q) - 1u;
+    if ((b.longm >> q1) & 1ull) return q1 + 1u;
+    const uint32_t r = (uint32_t)__popcll(b.rm & ~(~0ull << q1));
+    const uint32_t rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(b.rm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)b.rm, 0u));
+    uint64_t sel = from_q & __ballot(rank - r < kGroupRuns);
+    const uint64_t lg = sel & b.longm;
+    if (lg) sel &= (1ull << (__ffsll((long long)lg) - 1)) - 1ull;
+    const uint64_t after = from_q & ~sel;
+    const uint32_t last = 63u - (uint32_t)__clzll((long long)sel);
+    const uint32_t n = (uint32_t)__popcll(sel);
+    const uint32_t from = rlane(b.beg, q1), to = rlane(b.beg, last) + rlane(b.len, last);
+    // lane = slot*8 + y clears, and later prefixes, the row y of a slot
+    const bool slot_lane = (lane >> 3) < kGroupRuns;
+    uint4 *row = (uint4 *)(slots + (lane >> 3) * kGroupStride + (lane & 7u) * 8u);
+    if (slot_lane) {
+        row[0] = make_uint4(0, 0, 0, 0);
+        row[1] = make_uint4(0, 0, 0, 0);
+    }
+    wave_lds_sync();
+    const uint64_t *w = st.w - st.base;
+    uint32_t prev_key = (uint32_t)(w[from] >> kPayloadBits), sbase = 0;
+    for (uint32_t s0 = from; s0 < to; s0 += 64) {
+        const uint32_t s = s0 + lane;
+        // lanes past the group's last word read that word again: no key
+        // change, and they add nothing
+        const uint64_t word = w[min(s, to - 1u)];
+        const uint32_t hi = (uint32_t)(word >> kPayloadBits), lo = (uint32_t)word;
+        // the key of the word before (lane 0: the previous step's last word)
+        const uint32_t before = (uint32_t)__builtin_amdgcn_update_dpp((int)prev_key, (int)hi, 0x138, 0xF, 0xF,
+                                                                      false);  // wave_shr:1
+        const bool start = hi != before;
+        const uint64_t sm = __ballot(start);
+        uint32_t slot = __builtin_amdgcn_mbcnt_hi((uint32_t)(sm >> 32),
+                                                  __builtin_amdgcn_mbcnt_lo((uint32_t)sm, sbase + (start ? 1u : 0u)));
+        slot = min(slot, n - 1u);  // (always below n: the runs are consecutive)
+        if (s < to) {
+            const uint32_t v = ((uint32_t)((int32_t)(lo << 16) >> 22) & 0xFFFFu) | (uint32_t)((int32_t)(lo << 26) >> 2);
+            atomicAdd(&slots[__umul24(slot, kGroupStride) + ((lo >> 16) & 63u)], v);
+        }
+        sbase += (uint32_t)__popcll(sm);
+        prev_key = rlane(hi, 63);
+    }
+    wave_lds_sync();
+    // cover bytes -> exclusive prefixes over the row's columns
+    if (slot_lane) {
+        uint4 c0 = row[0], c1 = row[1];
+        uint32_t e = 0;
+        auto pre = [&](uint32_t &v) {
+            const uint32_t t = v & 0xFF000000u;
+            v = (v & 0x00FFFFFFu) | e;
+            e += t;
+        };
+        pre(c0.x), pre(c0.y), pre(c0.z), pre(c0.w), pre(c1.x), pre(c1.y), pre(c1.z), pre(c1.w);
+        row[0] = c0;
+        row[1] = c1;
+    }
+    wave_lds_sync();
+    *r0 = r;
+    return after ? (uint32_t)__ffsll((long long)after) - 1u : b.cnt;
+}
+
+// record_area for a record that group_build has served (r0: the rank in slot
+// 0): a run of at most kGroupLenMax words reads its pixel's word of the
+// slots, a longer one takes run_area.
+__device__ __forceinline__ int32_t grouped_area(uint32_t *slots, const Stage &st, const uint64_t *segs,
+                                                const Batch &b, uint32_t q, uint32_t r0, uint32_t py) {
+    const uint32_t key = rlane(b.key, q);
+    const int32_t cy = as_i8((int32_t)(rlane64(b.cov, q) >> (8 * py)));
+    if (!(key & kRecRun)) return 16 * cy;
+    if ((b.longm >> q) & 1ull) return run_area(st, (int32_t *)slots, segs, rlane(b.beg, q), rlane(b.len, q), cy);
+    const uint32_t slot = (uint32_t)__popcll(b.rm & ~(~0ull << q)) - r0, lane = lane_id();
+    const uint32_t v = slots[__umul24(slot, kGroupStride) + (lane & 7u) * 8u + (lane >> 3)];  // (y, x)
+    return 16 * as_i8(cy + (int32_t)(v >> 24)) + as_i16((int32_t)v);
 }
 
 // The source terms of blend_at for one pixel (painter/mod.rs:364-400):
@@ -2256,13 +2372,16 @@ __device__ __forceinline__ PropsHead props_head(const PropsEntry *props, uint32_
 // The painter over records [skipped, n) of a tile (painter/mod.rs:260-400):
 // coverage, clip masks (only in scenes with clips), column skipping and
 // blending into this lane's pixel colour.
-// `area(b, k0, q)` gives the accumulated area A of record k0 + q for this
-// lane's pixel; with Staged the batch's runs are staged for it first.
-template <bool Clips, bool Staged, class AreaFn>
+// `group(b, q, r0)` prepares the area of the batch's records from q on and
+// returns the record after those it prepared (setting r0 for them);
+// `area(b, k0, q, r0)` then gives the accumulated area A of record k0 + q
+// for this lane's pixel. With Staged the batch's runs are staged first.
+template <bool Clips, bool Staged, class AreaFn, class GroupFn>
 __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsEntry *props, const uint32_t *fl,
                                              const uint4 *rec, const uint32_t *rlen, uint32_t n, uint32_t skipped,
                                              Stage &st, const uint64_t *segs, uint32_t px, uint32_t py, uint32_t c,
-                                             uint32_t j, float &c0, float &c1, float &c2, float &ca, AreaFn &&area) {
+                                             uint32_t j, float &c0, float &c1, float &c2, float &ca, AreaFn &&area,
+                                             GroupFn &&group) {
     bool has_clip = false;
     uint32_t clip_last = 0;
     float clip_mask = 0.0f;
@@ -2271,7 +2390,10 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
         load_batch<Staged>(pp, props, fl, rec, rlen, n, k0, b);
         if (Staged) stage_batch(st, segs, b);
         cnt = b.cnt;
-        for (uint32_t q = 0; q < cnt; ++q) {
+        for (uint32_t q = 0; q < cnt;) {
+        uint32_t r0 = 0;
+        const uint32_t qe = group(b, q, r0);
+        for (; q < qe; ++q) {
             // with clips the flags are the clip pass's (per record); without,
             // the layer's, read with its colour through scalar loads, which
             // the coverage computation below leaves time to arrive
@@ -2282,7 +2404,7 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
             const uint32_t f = Clips ? rlane(b.flags, q) : ph[0];
             if (Clips && !(f & kMaskBit)) continue;
             if (Clips && has_clip && clip_last < lay) has_clip = false;
-            const int32_t A = area(b, k0, q);
+            const int32_t A = area(b, k0, q, r0);
             const float cov = coverage_from_area(A, (key & kRecEvenOdd) != 0);
             if (Clips && (f & PF_CLIP)) {
                 // clip_at (painter/mod.rs:402-416)
@@ -2318,6 +2440,7 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
                 blend_terms_uniform(mode, c0, c1, c2, s0, s1, s2, a, t0, t1, t2);
             }
             blend_apply(c0, c1, c2, ca, t0, t1, t2, a);
+        }
         }
     }
 }
@@ -2365,9 +2488,11 @@ __device__ __forceinline__ void paint_deep(const PaintParams &pp, const PropsEnt
             const uint32_t base = skipped + ch * kDeepChunk;
             const uint32_t end = base + kDeepChunk < n ? base + kDeepChunk : n;
             paint_layers<false, false>(pp, props, nullptr, rec, rlen, end, base, st, segs, px, py, c, j, c0, c1, c2,
-                                       ca, [&](const Batch &, uint32_t k0, uint32_t q) {
+                                       ca,
+                                       [&](const Batch &, uint32_t k0, uint32_t q, uint32_t) {
                                            return (int32_t)A[(k0 + q - base) * 64 + lane];
-                                       });
+                                       },
+                                       [](const Batch &b, uint32_t, uint32_t &) { return b.cnt; });
         }
         __syncthreads();
     }
@@ -2398,7 +2523,8 @@ __device__ __forceinline__ void paint_tile(const PaintParams &pp, uint32_t grow,
     st.w = (uint64_t *)(lds_raw + (size_t)wv * kPaintWaveLds);
     st.base = st.end = 0;
     int32_t *cells = (int32_t *)(lds_raw + (size_t)wv * kPaintWaveLds + kStageWin * 8);
-    uint32_t *fl = clips ? (uint32_t *)(lds_raw + kStageWin * 8 + 2 * 64 * 4) : nullptr;
+    uint32_t *slots = (uint32_t *)cells;  // run-group slots (shared with the per-run cells)
+    uint32_t *fl = clips ? (uint32_t *)(lds_raw + kPaintWaveLds) : nullptr;
     const uint64_t *segs = pp.segs;
     if (fl) {
         for (uint32_t k = lane; k < n; k += 64)
@@ -2413,8 +2539,9 @@ __device__ __forceinline__ void paint_tile(const PaintParams &pp, uint32_t grow,
             load_batch(pp, props, fl, rec, rlen, n, k0, b);
             stage_batch(st, segs, b);
             cnt = b.cnt;
-            for (uint32_t q = 0; q < cnt; ++q) {
-                const int32_t A = record_area(st, cells, segs, b, q, py);
+            for (uint32_t q = 0, qe = 0, r0 = 0; q < cnt; ++q) {
+                if (q == qe) qe = group_build(slots, st, b, q, &r0);
+                const int32_t A = grouped_area(slots, st, segs, b, q, r0, py);
                 uint32_t r = 0;
                 if (lane == 0) r = atomicAdd(pp.cov_count, 1u);
                 r = __shfl(r, 0);
@@ -2569,16 +2696,19 @@ __device__ __forceinline__ void paint_tile(const PaintParams &pp, uint32_t grow,
     uint32_t pix = 0;
     if (op == 2) {
         float c0 = S.clear[0], c1 = S.clear[1], c2 = S.clear[2], ca = S.clear[3];
-        auto area = [&](const Batch &b, uint32_t, uint32_t q) { return record_area(st, cells, segs, b, q, py); };
+        auto area = [&](const Batch &b, uint32_t, uint32_t q, uint32_t r0) {
+            return grouped_area(slots, st, segs, b, q, r0, py);
+        };
+        auto group = [&](const Batch &b, uint32_t q, uint32_t &r0) { return group_build(slots, st, b, q, &r0); };
         if (Deep)
             paint_deep(pp, props, rec, rlen, n, skipped, st, cells, segs, px, py, c, j, c0, c1, c2, ca,
                        (int16_t *)(lds_raw + 4 * kPaintWaveLds));
         else if (clips)
             paint_layers<true, true>(pp, props, fl, rec, rlen, n, skipped, st, segs, px, py, c, j, c0, c1, c2, ca,
-                                     area);
+                                     area, group);
         else
             paint_layers<false, true>(pp, props, nullptr, rec, rlen, n, skipped, st, segs, px, py, c, j, c0, c1, c2,
-                                      ca, area);
+                                      ca, area, group);
         pix = pixel_bytes(c0, c1, c2, ca);
     } else if (op == 1) {
         const float sc4[4] = {solid0, solid1, solid2, solid3};
@@ -2594,8 +2724,8 @@ __device__ __forceinline__ void paint_tile(const PaintParams &pp, uint32_t grow,
     if (pp.tile_ops && threadIdx.x == 0) pp.tile_ops[tile] = (int8_t)op;
 }
 
-// One wave per tile of the work list.
-__global__ __launch_bounds__(64) void k_paint(PaintParams pp) {
+// One wave per tile of the work list; at most 64 VGPRs, for 8 waves per SIMD.
+__global__ __launch_bounds__(64, 8) void k_paint(PaintParams pp) {
     extern __shared__ __align__(16) unsigned char lds_raw[];
     // the row walk stopped early: its records are incomplete
     if (__hip_atomic_load(pp.overflow, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;

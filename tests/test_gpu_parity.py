@@ -287,6 +287,58 @@ def test_per_pixel_coverage_bit_exact(gpu, cpu, case):
         assert ar == ag, f"tile/layer {kr}: A differs"
 
 
+def fan(rng, cx, cy, k, r):
+    """k thin spikes of radius r around (cx, cy): 2k edges through its pixel."""
+    ang = np.sort(rng.uniform(0, 2 * np.pi, k))
+    L = []
+    for a in ang:
+        d = rng.uniform(0.05, 0.3)
+        p1 = (cx + r * np.cos(a), cy + r * np.sin(a))
+        p2 = (cx + r * np.cos(a + d), cy + r * np.sin(a + d))
+        L += [[cx, cy, *p1], [*p1, *p2], [*p2, cx, cy]]
+    return np.array(L, np.float32)
+
+
+@pytest.mark.parametrize("seed", [810, 811])
+def test_dense_pixels_and_long_runs(gpu, cpu, seed):
+    """Per-pixel A and RGBA8 against the oracle where one layer puts from a
+    few to a few hundred segments into one tile and up to ~2k edges through
+    one pixel (runs on both sides of the painter's 256-word limit for runs
+    accumulated in groups, cells with many area carries), under and over
+    many small polygons (tiles with more than eight runs, carried covers)."""
+    w, h = 64, 48
+    rng = np.random.default_rng(seed)
+    paths = []
+    for i, k in enumerate((3, 12, 40, 70, 100, 160, 400, 1000)):
+        cx = 4.5 + 8 * (i % 7) + rng.uniform(-2, 2)
+        cy = 4.5 + 8 * (i // 7) * 3 + rng.uniform(-2, 2)
+        paths.append(fan(rng, cx, cy, k, float(rng.uniform(1.5, 6.0))))
+    for _ in range(90):
+        paths.append(scenegen.random_polygon(rng, w, h, int(rng.integers(0, 2))))
+    order = rng.permutation(len(paths))
+    cols = rng.uniform(0.05, 1.0, (len(paths), 4)).astype(np.float32)
+    rules = rng.integers(0, 2, len(paths))
+    out = {}
+    for m in (cpu, gpu):
+        comp = m.Composition()
+        for i in order:
+            lid = comp.create_layer()
+            comp.insert_lines(lid, paths[i])
+            fr = m.FillRule.EvenOdd if rules[i] else m.FillRule.NonZero
+            comp.get_mut(lid).set_props(m.Props(fill_rule=fr,
+                                                func=m.Func.Draw(m.Style(fill=m.Fill.Solid(cols[i].tolist())))))
+        a, k = comp.render_coverage(w, h, [1.0, 1.0, 1.0, 1.0], None)
+        buf = np.zeros((w * h, 4), np.uint8)
+        comp.render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
+        out[m] = (sorted_coverage(a, k, m is gpu), buf)
+    ref, got = out[cpu][0], out[gpu][0]
+    assert len(ref) == len(got) > 0
+    for (kr, ar), (kg, ag) in zip(ref, got):
+        assert kr == kg, (kr, kg)
+        assert ar == ag, f"tile/layer {kr}: A differs"
+    assert_same_image(out[cpu][1], out[gpu][1], w)
+
+
 def test_line_changes_between_frames(gpu, cpu):
     """A composition changed between frames, against the oracle after each
     change: lines appended to an existing layer and to a new layer with a
