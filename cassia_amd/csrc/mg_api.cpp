@@ -20,10 +20,30 @@
 #include "mg_runtime.h"
 #include "mg_scratch.h"
 #include "mg_shared.h"
+#include "mg_exact.h"
 
 namespace {
 
 thread_local std::string g_error;
+
+// A solid Soft Light entry with finite colour channels carries the source
+// terms of its channels (soft_light_terms: u0 u1 u2 v0 v1 v2 in the six
+// words from grad_start on, which a solid fill does not use) and
+// PF_SOFT_TERMS; the painter then reads them instead of computing them per
+// record. Finite channels give finite terms, so no NaN is made here.
+void set_soft_light_terms(mg::PropsEntry &e) {
+    static_assert(offsetof(mg::PropsEntry, n_stops) + 4 == offsetof(mg::PropsEntry, grad_start) + 24,
+                  "six words from grad_start");
+    if ((e.flags & (mg::PF_GRADIENT | mg::PF_CLIP)) || ((e.flags >> mg::PF_BLEND_SHIFT) & 15u) != mg::BM_SOFT_LIGHT)
+        return;
+    float t[6];
+    for (int i = 0; i < 3; ++i) {
+        if (!std::isfinite(e.color[i])) return;
+        mg::soft_light_terms(e.color[i], t[i], t[3 + i]);
+    }
+    std::memcpy(reinterpret_cast<char *>(&e) + offsetof(mg::PropsEntry, grad_start), t, sizeof t);
+    e.flags |= mg::PF_SOFT_TERMS;
+}
 
 mg_status fail(mg_status s, const std::string &msg) {
     g_error = msg;
@@ -340,6 +360,7 @@ struct mg_composition {
             }
             if (cache_id >= 0 && ((l.is_unchanged >> cache_id) & 1u)) fl |= mg::PF_UNCHANGED;
             e.flags = fl;
+            set_soft_light_terms(e);
             if (l.enabled) max_order = std::max<uint32_t>(max_order, kv.first);
         }
         f.max_order = std::max<uint32_t>(max_order, n_props - 1);
@@ -1258,7 +1279,7 @@ mg_status pack_batch(mg_batch *b, mg_composition *const *comps, const mg_batch_f
             f.layers.push_back(e);
         }
         for (mg::PropsEntry e : cs.props) {
-            e.stops_offset += stops_base;
+            if (e.flags & mg::PF_GRADIENT) e.stops_offset += stops_base;
             f.props.push_back(e);
         }
         f.stops.insert(f.stops.end(), cs.stops.begin(), cs.stops.end());
@@ -1558,6 +1579,7 @@ MG_EXPORT void cassia_render(const uint64_t *psegments, size_t n, const CassiaSt
         e.flags = mg::PF_DEFINED | (s.fillRule == 1 ? mg::PF_EVENODD : 0) | (blend << mg::PF_BLEND_SHIFT);
         if (blend == MG_BLEND_OVER && s.fill[3] == 1.0f) e.flags |= mg::PF_OPAQUE;
         std::memcpy(e.color, s.fill, sizeof e.color);
+        set_soft_light_terms(e);
     }
     f.layers.assign(1, mg::LayerEntry{});
     f.max_order = m ? (uint32_t)m - 1 : 0;

@@ -100,6 +100,17 @@ enum : uint32_t {
     BM_COLOR_BURN, BM_HARD_LIGHT, BM_SOFT_LIGHT, BM_DIFFERENCE, BM_EXCLUSION
 };
 
+// The terms of Soft Light's formula that depend on the source channel s
+// alone (style.rs:288-418, as in blend_lane below): u = s·(1−s) and
+// v = D(s) − s, D being the polynomial for s ≤ 0.25 and the square root
+// above. Host and device compute them with the same IEEE operations.
+__host__ __device__ __forceinline__ void soft_light_terms(float s, float &u, float &v) {
+    const float poly = __builtin_fmaf(__builtin_fmaf(16.0f, s, -12.0f), s, 4.0f) * s;
+    const float dd = s <= 0.25f ? poly : __builtin_sqrtf(s);
+    u = s * (1.0f - s);
+    v = dd - s;
+}
+
 // Per-lane form of blend_function! (style.rs:288-418), used for painted
 // pixels. `select(a, b, m)` is `m ? a : b` (simd/auto.rs:459-502).
 __host__ __device__ __forceinline__ float blend_lane(uint32_t mode, float d, float s) {

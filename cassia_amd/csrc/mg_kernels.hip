@@ -2308,39 +2308,97 @@ __device__ __forceinline__ float overlay_uniform(float d, float s) {
     if (s <= 0.5f) return (d * s) * 2.0f;
     return 2.0f * ((s + d) - __builtin_fmaf(s, d, 0.5f));
 }
+// The props table seen as constant memory: read at a wave-uniform index, its
+// entries come through scalar loads (the table is not written while a frame
+// paints), so a record's flags and solid colour need no lane reads.
+typedef uint32_t PropsHead __attribute__((ext_vector_type(8)));  // flags, clip_layers, color[4], grad_start
+static_assert(offsetof(PropsEntry, color) == 8 && offsetof(PropsEntry, grad_start) == 24 &&
+                  sizeof(PropsEntry) == 48, "props entry layout");
+typedef const __attribute__((address_space(4))) PropsHead *ScalarPropsHead;
+
+// NaN test on the bits of a (wave-uniform) float: scalar instructions.
+__device__ __forceinline__ bool nan_bits(float x) { return (__float_as_uint(x) & 0x7FFFFFFFu) > 0x7F800000u; }
+
+// With PF_SOFT_TERMS (flags `f` of entry `pe`) Soft Light reads the source
+// terms of its three channels from the entry (words 6..11, one scalar load of
+// words 4..11) instead of computing them. Darken and Lighten with a source
+// that is not NaN take one comparison per channel: fmin_rs(d, s) is then
+// d <= s ? d : s (a NaN d fails the comparison and gives s), fmax_rs(d, s)
+// is d >= s ? d : s.
 __device__ __forceinline__ void blend_terms_uniform(uint32_t mode, float c0, float c1, float c2, float f0,
                                                     float f1, float f2, float a, float &b0, float &b1,
-                                                    float &b2) {
-    if (mode == BM_OVERLAY) {
-        b0 = overlay_uniform(c0, f0) * a;
-        b1 = overlay_uniform(c1, f1) * a;
-        b2 = overlay_uniform(c2, f2) * a;
-        return;
-    }
-    if (mode == BM_SOFT_LIGHT) {
-        const uint32_t l0 = (uint32_t)__builtin_ctzll(__ballot(true));
-        const uint32_t own = lane_id() - l0;
-        const float sl = own == 0 ? f0 : (own == 1 ? f1 : f2);
-        const float poly = __builtin_fmaf(__builtin_fmaf(16.0f, sl, -12.0f), sl, 4.0f) * sl;
-        const float dd = sl <= 0.25f ? poly : __builtin_sqrtf(sl);
-        const float u = sl * (1.0f - sl), v = dd - sl;
-        const float fs[3] = {f0, f1, f2};
-        const float cs[3] = {c0, c1, c2};
-        float out[3];
+                                                    float &b2, uint32_t f, const PropsEntry *pe) {
+    const float fs[3] = {f0, f1, f2};
+    const float cs[3] = {c0, c1, c2};
+    float out[3];
+    switch (mode) {
+    case BM_OVERLAY:
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) out[ch] = overlay_uniform(cs[ch], fs[ch]) * a;
+        break;
+    case BM_DARKEN:
+    case BM_LIGHTEN:
+        if (nan_bits(f0) || nan_bits(f1) || nan_bits(f2)) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) out[ch] = blend_lane(mode, cs[ch], fs[ch]) * a;
+        } else if (mode == BM_DARKEN) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) out[ch] = (cs[ch] <= fs[ch] ? cs[ch] : fs[ch]) * a;
+        } else {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) out[ch] = (cs[ch] >= fs[ch] ? cs[ch] : fs[ch]) * a;
+        }
+        break;
+    case BM_SOFT_LIGHT: {
+        float us[3], vs[3];
+        if (f & PF_SOFT_TERMS) {
+            const PropsHead w = *(ScalarPropsHead)((uintptr_t)pe + 16);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                us[ch] = __uint_as_float(w[2 + ch]);
+                vs[ch] = __uint_as_float(w[5 + ch]);
+            }
+        } else {
+            const uint32_t l0 = (uint32_t)__builtin_ctzll(__ballot(true));
+            const uint32_t own = lane_id() - l0;
+            const float sl = own == 0 ? f0 : (own == 1 ? f1 : f2);
+            float u, v;
+            soft_light_terms(sl, u, v);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                us[ch] = rlane_f(u, l0 + (uint32_t)ch);
+                vs[ch] = rlane_f(v, l0 + (uint32_t)ch);
+            }
+        }
 #pragma unroll
         for (int ch = 0; ch < 3; ++ch) {
-            const float uc = rlane_f(u, l0 + (uint32_t)ch), vc = rlane_f(v, l0 + (uint32_t)ch);
             const float k = __builtin_fmaf(2.0f, cs[ch], -1.0f);
-            const float lo = __builtin_fmaf(uc, k, fs[ch]);
-            const float hi = __builtin_fmaf(vc, k, fs[ch]);
+            const float lo = __builtin_fmaf(us[ch], k, fs[ch]);
+            const float hi = __builtin_fmaf(vs[ch], k, fs[ch]);
             out[ch] = (cs[ch] <= 0.5f ? lo : hi) * a;
         }
-        b0 = out[0];
-        b1 = out[1];
-        b2 = out[2];
-        return;
+        break;
     }
-    blend_terms(mode, c0, c1, c2, f0, f1, f2, a, b0, b1, b2);
+#define MG_BLEND_UNIFORM_CASE(m)                                                     \
+    case m:                                                                          \
+        _Pragma("unroll") for (int ch = 0; ch < 3; ++ch) out[ch] = blend_lane(m, cs[ch], fs[ch]) * a; \
+        break;
+        MG_BLEND_UNIFORM_CASE(BM_MULTIPLY)
+        MG_BLEND_UNIFORM_CASE(BM_SCREEN)
+        MG_BLEND_UNIFORM_CASE(BM_COLOR_DODGE)
+        MG_BLEND_UNIFORM_CASE(BM_COLOR_BURN)
+        MG_BLEND_UNIFORM_CASE(BM_HARD_LIGHT)
+        MG_BLEND_UNIFORM_CASE(BM_DIFFERENCE)
+        MG_BLEND_UNIFORM_CASE(BM_EXCLUSION)
+#undef MG_BLEND_UNIFORM_CASE
+    default:  // Over, and modes outside the table (blend_lane's default)
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) out[ch] = fs[ch] * a;
+        break;
+    }
+    b0 = out[0];
+    b1 = out[1];
+    b2 = out[2];
 }
 
 // dst * (1 - alpha) + term (painter/mod.rs:364-400), as fused multiply-adds
@@ -2358,12 +2416,6 @@ __device__ __forceinline__ void blend_apply(float &c0, float &c1, float &c2, flo
     fma_in_place(ca, inv, a);
 }
 
-// The props table seen as constant memory: read at a wave-uniform index, its
-// entries come through scalar loads (the table is not written while a frame
-// paints), so a record's flags and solid colour need no lane reads.
-typedef uint32_t PropsHead __attribute__((ext_vector_type(8)));  // flags, clip_layers, color[4], grad_start
-static_assert(offsetof(PropsEntry, color) == 8 && sizeof(PropsEntry) % 16 == 0, "props entry layout");
-typedef const __attribute__((address_space(4))) PropsHead *ScalarPropsHead;
 // The first 32 bytes of entry `lay` (flags and solid colour), one scalar load.
 __device__ __forceinline__ PropsHead props_head(const PropsEntry *props, uint32_t lay) {
     return *(ScalarPropsHead)((uintptr_t)(props + lay));
@@ -2437,7 +2489,7 @@ __device__ __forceinline__ void paint_layers(const PaintParams &pp, const PropsE
                 const float s3 = Clips ? rlane_f(b.color.w, q) : __uint_as_float(ph[5]);
                 a = s3 * cov;
                 if (apply_clip) a *= clip_mask;
-                blend_terms_uniform(mode, c0, c1, c2, s0, s1, s2, a, t0, t1, t2);
+                blend_terms_uniform(mode, c0, c1, c2, s0, s1, s2, a, t0, t1, t2, f, props + lay);
             }
             blend_apply(c0, c1, c2, ca, t0, t1, t2, a);
         }

@@ -44,6 +44,10 @@ enum : uint32_t {
     PF_OPAQUE = 1u << 6,    // Draw + Solid + Over + alpha == 1.0
     PF_RADIAL = 1u << 7,
     PF_BLEND_SHIFT = 8,     // 4 bits of BlendMode
+    // a solid Soft Light layer with finite colour channels: the six words
+    // from grad_start on hold soft_light_terms of channels 0..2 (u0 u1 u2
+    // v0 v1 v2), computed when the table is built
+    PF_SOFT_TERMS = 1u << 12,
 };
 struct PropsEntry {
     uint32_t flags;

@@ -339,6 +339,38 @@ def test_dense_pixels_and_long_runs(gpu, cpu, seed):
     assert_same_image(out[cpu][1], out[gpu][1], w)
 
 
+def test_solid_blend_special_colours(gpu, cpu):
+    """RGBA8 against the oracle for solid layers in the blend modes whose
+    source-only terms the painter specializes (Darken, Lighten, Soft Light
+    with terms carried by the props table, Overlay), with colour channels
+    that are NaN, infinite, negative or above one, beside ordinary ones."""
+    w, h = 64, 40
+    rng = np.random.default_rng(77)
+    special = [float("nan"), float("inf"), -float("inf"), -0.5, 1.75, 0.0, 1.0, 0.25, 0.5, -0.0]
+    modes = ["Darken", "Lighten", "SoftLight", "Overlay", "ColorDodge", "Multiply"]
+    layers = []
+    for i in range(36):
+        col = rng.uniform(0.0, 1.0, 4)
+        for c in range(3):
+            if rng.uniform() < 0.3:
+                col[c] = special[int(rng.integers(len(special)))]
+        col[3] = rng.uniform(0.3, 1.0)
+        layers.append((scenegen.random_polygon(rng, w, h, int(rng.integers(0, 2))), col.astype(np.float32).tolist(),
+                       modes[i % len(modes)]))
+    out = {}
+    for m in (cpu, gpu):
+        comp = m.Composition()
+        for lines, col, mode in layers:
+            lid = comp.create_layer()
+            comp.insert_lines(lid, lines)
+            comp.get_mut(lid).set_props(m.Props(func=m.Func.Draw(m.Style(fill=m.Fill.Solid(col),
+                                                                          blend_mode=getattr(m.BlendMode, mode)))))
+        buf = np.zeros((w * h, 4), np.uint8)
+        comp.render(m.Buffer(buf, w), [0.6, 0.4, 0.2, 1.0], None)
+        out[m] = buf
+    assert_same_image(out[cpu], out[gpu], w)
+
+
 def test_line_changes_between_frames(gpu, cpu):
     """A composition changed between frames, against the oracle after each
     change: lines appended to an existing layer and to a new layer with a
