@@ -20,6 +20,11 @@ HIP_FLAGS = [
     "-ffp-contract=off", "-fhip-fp32-correctly-rounded-divide-sqrt",
     "-fno-gpu-flush-denormals-to-zero", "-fwrapv", "-fno-fast-math",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # Branches on wave-uniform values stay plain scalar branches instead of
+    # being restructured into flag-guarded regions: the paint's per-record
+    # decisions (run or carried cover, fill rule, blend mode) then cost one
+    # scalar compare and branch each (3840x2160 paint 5.40 -> 4.41 ms).
+    "-mllvm", "-structurizecfg-skip-uniform-regions=true",
 ]
 
 
