@@ -32,9 +32,11 @@ sys.path.insert(0, ROOT)
 
 METRIC = "frames/s & pixel-segments/s batched vs plain on 1 MI355X, bit-exact; p99 ms"
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E peak
-# vector-ALU issue peak: 256 CUs x 4 SIMDs, one wave64 VALU instruction per
-# 4 cycles per SIMD (16 lanes), 2.4 GHz
-VALU_PEAK_PER_S = 256 * 4 * 2.4e9 / 4
+# vector-ALU issue: 256 CUs x 4 SIMDs at 2.4 GHz. A wave64 f32 or 3-operand
+# instruction takes 4 cycles of its 16-lane SIMD; 2-operand 32-bit integer and
+# logic instructions issue about every 2 cycles on this device.
+SIMD_CYCLES_PER_S = 256 * 4 * 2.4e9
+VALU_PEAK_PER_S = SIMD_CYCLES_PER_S / 4
 
 
 # kernel -> stats field with its per-frame time (hipEvents on the library's
@@ -486,6 +488,11 @@ def measured_valu(workload):
     return profile_value(f"sq-instructions_{workload}", "valu_per_launch", {})
 
 
+def measured_valu_f32(workload):
+    """Of those, f32 arithmetic instructions (add, fma, mul, cvt, trans), or {}."""
+    return profile_value(f"sq-instructions_{workload}", "valu_f32_per_launch", {})
+
+
 def bench_circles(cassia_amd, workloads, steps, warmup):
     """The plain configuration on the GPU: 1024x1024, 10,000 circles x 64
     lines, one frame per step into a device-resident frame."""
@@ -620,10 +627,13 @@ def main():
                       "bytes_per_launch": b,
                       "gbs": round(b / (per_launch_ms * 1e-3) / 1e9, 2) if per_launch_ms > 0 else None}
     valu = measured_valu(profile_name) if profile_name else {}
+    valu_f32 = measured_valu_f32(profile_name) if profile_name else {}
     for k, v in kernels.items():
         v["hbm_frac"] = round(v["gbs"] / HBM_PEAK_GBS, 4) if v["gbs"] else None
-        if k in valu and v["ms_per_launch"] > 0:
-            v["valu_frac"] = round(valu[k] / (v["ms_per_launch"] * 1e-3) / VALU_PEAK_PER_S, 4)
+        if k in valu and k in valu_f32 and v["ms_per_launch"] > 0:
+            # VALU issue floor (f32 at 4 cycles, the rest at 2) over the measured time
+            floor_ms = (4 * valu_f32[k] + 2 * (valu[k] - valu_f32[k])) / SIMD_CYCLES_PER_S * 1e3
+            v["valu_frac"] = round(floor_ms / v["ms_per_launch"], 4)
     dominant = max(kms, key=kms.get)
     d = kernels[dominant]
     traffic = measured_traffic(profile_name).get(dominant) if profile_name else None
@@ -636,13 +646,21 @@ def main():
         roofline["copy_ceiling_gbs"] = profile_value(f"hbm-traffic_{profile_name}", "copy_ceiling_gbs")
     if dominant in valu:
         # vector-ALU issue: the kernel's wave-level VALU instructions (SQ
-        # counters of the committed profile) per second of its measured time
+        # counters of the committed profile) per second of its measured time,
+        # and the shortest time that instruction stream can issue in (f32
+        # instructions at 4 cycles, all others at 2); frac = that floor over
+        # the measured time
         rate = valu[dominant] / (d["ms_per_launch"] * 1e-3)
-        roofline["valu"] = {"achieved": rate, "peak": VALU_PEAK_PER_S, "unit": "wave64 VALU instructions/s",
-                            "frac": round(rate / VALU_PEAK_PER_S, 4), "instructions_per_launch": valu[dominant],
+        f32 = measured_valu_f32(profile_name).get(dominant)
+        roofline["valu"] = {"achieved": rate, "unit": "wave64 VALU instructions/s",
+                            "cadence_4_cycles": VALU_PEAK_PER_S, "instructions_per_launch": valu[dominant],
                             "source": f"profiles/sq-instructions_{profile_name}.md"}
-        if rate / VALU_PEAK_PER_S > hbm_frac:
-            roofline["bound"] = "valu"
+        if f32 is not None:
+            floor_ms = (4 * f32 + 2 * (valu[dominant] - f32)) / SIMD_CYCLES_PER_S * 1e3
+            roofline["valu"].update({"f32_per_launch": f32, "issue_floor_ms": round(floor_ms, 4),
+                                     "frac": round(floor_ms / d["ms_per_launch"], 4)})
+            if floor_ms / d["ms_per_launch"] > hbm_frac:
+                roofline["bound"] = "valu"
     out = {
         "metric": METRIC,
         "value": segs * fps,
