@@ -95,17 +95,19 @@ __global__ void k_path_emit(const PathCommand *cmds, const flat::PathDesc *paths
 __global__ void k_part_lines(const uint32_t *offsets, const uint32_t *lengths, uint64_t n_lines,
                              uint32_t part_size, uint32_t *part_line, uint64_t n_parts);
 __global__ void k_clamp_line_pixels(FrameCounters *counters, uint64_t cap, uint32_t *flag);
-__global__ void k_histogram(const uint64_t *words, FrameCounters *counters, FrameDims dims, uint32_t *hist);
 __global__ void k_row_starts(const uint64_t *sorted, const FrameCounters *counters, uint32_t rows,
                              uint32_t row_shift, uint32_t *row_start);
 __global__ void k_ingest(const uint64_t *in, uint64_t n_in, const SceneDesc *scenes, FrameDims dims,
                          const FrameCounters *counters, uint64_t *out, SlotCounts sc);
-__global__ void k_count_reduce(const uint32_t *acc, FrameDims dims, uint32_t *hist, FrameCounters *counters);
-__global__ void k_digit_bases(const uint32_t *hist, uint32_t *bases, uint32_t radix);
+__global__ void k_count_reduce(const uint32_t *acc, FrameCounters *counters);
+// the type a sort pass stores the next pass's R-bit digit in
+template <int R> struct DigitOf { using T = uint8_t; };
+template <> struct DigitOf<9> { using T = uint16_t; };
 template <int R>
-__global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters,
-                            uint32_t shift, const uint32_t *digit_base, uint32_t *status,
-                            uint32_t *stall, const uint32_t *first_off, const uint32_t *part_kept);
+__global__ void k_digit_count(const typename DigitOf<R>::T *dig, const FrameCounters *counters, uint32_t *cnt);
+template <int R>
+__global__ void k_sort_pass(const uint64_t *in, uint64_t *out, const FrameCounters *counters, uint32_t shift,
+                            const uint32_t *offsets, const uint32_t *part_kept, typename DigitOf<R>::T *dig_out);
 __global__ void k_tile_starts(PaintParams pp, uint32_t global_rows, uint32_t stream_runs);
 __global__ void k_tile_runs(PaintParams pp, const uint2 *work, uint32_t n_work);
 __global__ void k_tile_nr(PaintParams pp, uint32_t global_rows);
@@ -123,7 +125,12 @@ size_t paint_lds_bytes_host(uint32_t cap, bool clips);
 
 constexpr uint32_t kStatSlices = 32;
 constexpr uint32_t kEmitTileHost = 2048;
-constexpr uint32_t kSortTileHost = 4096;
+constexpr uint32_t kSortTileHost = 8192;
+constexpr uint32_t kSortThreadsHost = 512;  // threads per sort pass work group
+constexpr uint32_t kXcdsHost = 8;           // XCD-mapped launches round their grids to multiples of this
+inline uint32_t xcd_grid(uint64_t parts) {
+    return (uint32_t)(parts ? (parts + kXcdsHost - 1) / kXcdsHost * kXcdsHost : kXcdsHost);
+}
 constexpr uint32_t kScanTileHost = 2048;
 constexpr uint32_t kRunsTileHost = 4096;  // words per k_runs_scan partition
 

@@ -225,7 +225,7 @@ private:
     bool status_pinned_ = false, status_final_ = false;
     alignas(16) unsigned char status_local_[512] = {};
     bool read_status(size_t bytes, std::string &err);
-    DevBuf segs_a_, segs_b_, hist_, bases_, sort_status_;
+    DevBuf segs_a_, segs_b_, dig_;  // ping-pong words; the next sort pass's digits
     DevBuf row_start_, row_scene_;
     DevBuf tile_ops_, cov_a_, cov_key_, cov_count_;
     DevBuf part_line_;

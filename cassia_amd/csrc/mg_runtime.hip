@@ -116,7 +116,7 @@ Renderer::Renderer() {}
 std::vector<DevBuf *> Renderer::all_buffers() {
     return {&scenes_, &scene_line_base_, &layers_, &props_, &stops_,
             &setup_, &lengths_, &offsets_, &scan_status_, &status_, &segs_a_,
-            &segs_b_, &hist_, &bases_, &sort_status_,
+            &segs_b_, &dig_,
             &row_start_, &row_scene_,
             &tile_ops_, &cov_a_, &cov_key_, &cov_count_, &work_, &part_line_, &cnt0_, &cnt_acc_, &cnt_status_, &tile_start_, &tile_run0_,
             &run_key_, &run_beg_, &run_cov_, &rec_, &rec_len_, &rec_hdr_, &runs_status_, &runs_work_, &tile_nr_,
@@ -531,20 +531,19 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         return false;
     }
     const uint64_t nparts_scan = (L + kScanTileHost - 1) / kScanTileHost;
-    const uint32_t count_rows = dims.passes * radix + 2, rest_rows = count_rows - radix;
-    if (!scan_status_.reserve(std::max<uint64_t>(nparts_scan, 1) * 8) || !hist_.reserve(16 * radix * 4) ||
-        !cnt_acc_.reserve((uint64_t)kCountSlicesHost * rest_rows * 4)) {
+    const uint32_t count_rows = radix + 2;  // first-pass digits, kept, culled
+    if (!scan_status_.reserve(std::max<uint64_t>(nparts_scan, 1) * 8) ||
+        !cnt_acc_.reserve((uint64_t)kCountSlicesHost * 2 * 4)) {
         err = "out of device memory";
         return false;
     }
     MG_HIP(mark(ev0_), "event");
     MG_HIP(mark(evs_[0]), "event");
-    // the status block, the lines' lengths, the scan's look-back words, the
-    // digit histograms and the slot-count slices the emission adds into
+    // the status block, the lines' lengths, the scan's look-back words and
+    // the kept/culled slices the emission adds into
     zero_words(stream_, (uint32_t)cus_,
                {{fs, sizeof(FrameStatus) / 4}, {lengths_.ptr, lines && order_lines ? L : 0},
-                {scan_status_.ptr, lines ? 2 * nparts_scan : 0}, {hist_.ptr, (uint64_t)dims.passes * radix},
-                {cnt_acc_.ptr, (uint64_t)kCountSlicesHost * rest_rows}});
+                {scan_status_.ptr, lines ? 2 * nparts_scan : 0}, {cnt_acc_.ptr, (uint64_t)kCountSlicesHost * 2}});
     if (lines) {
         k_setup<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(
             pts, ids, ranks, layers_.as<LayerEntry>(), scenes_.as<SceneDesc>(), scene_line_base_.as<uint32_t>(), dims,
@@ -579,17 +578,24 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     const uint64_t cap_segs = std::max<uint64_t>(2 * P, 1);
     const uint64_t nparts_emit = (P + kEmitTileHost - 1) / kEmitTileHost;
     const uint64_t nparts_sort = (cap_segs + kSortTileHost - 1) / kSortTileHost;
-    if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) || !bases_.reserve(16 * radix * 4) || !sort_status_.reserve(nparts_sort * radix * 4 * dims.passes) ||
+    // the next pass's digits, padded to whole partitions (k_digit_count reads whole ones)
+    const size_t dig_size = R == 9 ? 2 : 1;
+    if (!segs_a_.reserve(cap_segs * 8) || !segs_b_.reserve(cap_segs * 8) ||
+        !dig_.reserve(std::max<uint64_t>(nparts_sort, 1) * kSortTileHost * dig_size) ||
         !row_start_.reserve(((size_t)global_rows + 1) * 4)) {
         err = "out of device memory (segments)";
         return false;
     }
-    // Per-partition slot counts: pass 0 digits digit-major (scanned in place
-    // into the first pass's offsets), the rest in kCountSlicesHost slices.
-    const uint64_t n_cnt0 = (uint64_t)radix * nparts_sort;
+    // Per-partition digit counts, digit-major, scanned in place into a
+    // pass's output offsets: the emission's for the first pass, then
+    // k_digit_count's for each later one (each scan has its own look-back
+    // words).
+    // (the emission's counts are per emission partition, twice as many as
+    // the sort partitions whose counts the later passes scan)
+    const uint64_t n_cnt0 = (uint64_t)radix * std::max<uint64_t>(nparts_emit, 1);
     const uint64_t cnt_parts = (n_cnt0 + kScanTileHost - 1) / kScanTileHost;
-    if (!cnt0_.reserve(std::max<uint64_t>(n_cnt0, 1) * 4) || !cnt_status_.reserve(std::max<uint64_t>(cnt_parts, 1) * 8) ||
-        !part_kept_.reserve(std::max<uint64_t>(nparts_sort, 1) * 16)) {
+    if (!cnt0_.reserve(n_cnt0 * 4) || !cnt_status_.reserve(std::max<uint64_t>(cnt_parts, 1) * 8 * dims.passes) ||
+        !part_kept_.reserve(std::max<uint64_t>(std::max(4 * nparts_emit, 8 * nparts_sort), 1) * 4)) {
         err = "out of device memory (slot counts)";
         return false;
     }
@@ -600,50 +606,62 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     // every line-pixel owns two word slots; the ingest path uses the same layout
     if (ingest && spec.n_in) {
         fill64((uint64_t *)&ctr->line_pixels, P, 1, stream_);
-        k_ingest<<<(uint32_t)nparts_sort, 256, count_lds, stream_>>>(spec.in_segs, spec.n_in, scenes_.as<SceneDesc>(),
+        k_ingest<<<(uint32_t)nparts_emit, 256, count_lds, stream_>>>(spec.in_segs, spec.n_in, scenes_.as<SceneDesc>(),
                                                                    dims, ctr, segs_a_.as<uint64_t>(), counts);
     } else if (P) {
         if (!part_line_.reserve(nparts_emit * 4)) { err = "out of device memory"; return false; }
         k_part_lines<<<(uint32_t)((L + 255) / 256), 256, 0, stream_>>>(offsets_.as<uint32_t>(), lengths_.as<uint32_t>(), L,
                                                                       kEmitTileHost, part_line_.as<uint32_t>(),
                                                                       nparts_emit);
-        k_emit<<<(uint32_t)nparts_emit, 256, count_lds, stream_>>>(setup_.as<LineSetup>(), offsets_.as<uint32_t>(),
+        k_emit<<<xcd_grid(nparts_emit), 256, count_lds, stream_>>>(setup_.as<LineSetup>(), offsets_.as<uint32_t>(),
                                                                  scenes_.as<SceneDesc>(), dims, ctr,
                                                                  segs_a_.as<uint64_t>(), part_line_.as<uint32_t>(),
                                                                  counts);
     }
     MG_HIP(hipGetLastError(), "emit launch");
     MG_HIP(mark(evs_[2]), "event");
-    // look-back words of the slot-count scan and of every digit pass,
-    // cleared outside the sort's timing
-    zero_words(stream_, (uint32_t)cus_,
-               {{cnt_status_.ptr, P ? 2 * cnt_parts : 0}, {sort_status_.ptr, nparts_sort * radix * dims.passes}});
+    // look-back words of the digit-count scans, cleared outside the sort's timing
+    zero_words(stream_, (uint32_t)cus_, {{cnt_status_.ptr, P ? 2 * cnt_parts * dims.passes : 0}});
     if (P) {
-        k_count_reduce<<<rest_rows, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), dims, hist_.as<uint32_t>(), ctr);
+        k_count_reduce<<<2, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), ctr);
         k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
                                                            cnt_status_.as<uint64_t>(), stall, &ctr->order_total);
     }
-    k_digit_bases<<<dims.passes, 256, 0, stream_>>>(hist_.as<uint32_t>(), bases_.as<uint32_t>(), radix);
-    MG_HIP(hipGetLastError(), "histogram launch");
+    MG_HIP(hipGetLastError(), "count launch");
     MG_HIP(mark(evs_[3]), "event");
     uint64_t *in = segs_a_.as<uint64_t>(), *out = segs_b_.as<uint64_t>();
-    const uint32_t sort_grid = (uint32_t)std::max<uint64_t>(nparts_sort, 1);
+    const uint32_t sort_grid = xcd_grid(nparts_sort);
     for (uint32_t p = 0; p < dims.passes; ++p) {
         const uint32_t shift = dims.sort_shift + p * R;
-        uint32_t *status = sort_status_.as<uint32_t>() + (size_t)p * nparts_sort * radix;
-        const uint32_t *from_slots = p == 0 ? cnt0_.as<uint32_t>() : nullptr;
+        if (p && P) {
+            // offsets of this pass: the digits the previous pass stored, counted and scanned
+            if (R == 9)
+                k_digit_count<9><<<sort_grid, 256, 0, stream_>>>(dig_.as<uint16_t>(), ctr, cnt0_.as<uint32_t>());
+            else if (R == 8)
+                k_digit_count<8><<<sort_grid, 256, 0, stream_>>>(dig_.as<uint8_t>(), ctr, cnt0_.as<uint32_t>());
+            else if (R == 6)
+                k_digit_count<6><<<sort_grid, 256, 0, stream_>>>(dig_.as<uint8_t>(), ctr, cnt0_.as<uint32_t>());
+            else
+                k_digit_count<4><<<sort_grid, 256, 0, stream_>>>(dig_.as<uint8_t>(), ctr, cnt0_.as<uint32_t>());
+            k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
+                                                               cnt_status_.as<uint64_t>() + (size_t)p * cnt_parts,
+                                                               stall, &ctr->order_total);
+        }
+        const uint32_t *kept = p == 0 ? part_kept_.as<uint32_t>() : nullptr;
+        const bool next = p + 1 < dims.passes;
+        const uint32_t *offs = cnt0_.as<uint32_t>();
         if (R == 9)
-            k_sort_pass<9><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
+            k_sort_pass<9><<<sort_grid, kSortThreadsHost, 0, stream_>>>(in, out, ctr, shift, offs, kept,
+                                                                         next ? dig_.as<uint16_t>() : nullptr);
         else if (R == 8)
-            k_sort_pass<8><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
+            k_sort_pass<8><<<sort_grid, kSortThreadsHost, 0, stream_>>>(in, out, ctr, shift, offs, kept,
+                                                                         next ? dig_.as<uint8_t>() : nullptr);
         else if (R == 6)
-            k_sort_pass<6><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
+            k_sort_pass<6><<<sort_grid, kSortThreadsHost, 0, stream_>>>(in, out, ctr, shift, offs, kept,
+                                                                         next ? dig_.as<uint8_t>() : nullptr);
         else
-            k_sort_pass<4><<<sort_grid, 256, 0, stream_>>>(in, out, ctr, shift, bases_.as<uint32_t>() + p * radix,
-                                                            status, stall, from_slots, part_kept_.as<uint32_t>());
+            k_sort_pass<4><<<sort_grid, kSortThreadsHost, 0, stream_>>>(in, out, ctr, shift, offs, kept,
+                                                                         next ? dig_.as<uint8_t>() : nullptr);
         std::swap(in, out);
     }
     MG_HIP(hipGetLastError(), "sort launch");
@@ -1015,8 +1033,8 @@ bool Renderer::upload_work(const FrameSpec &spec, std::string &err) {
 // while none of them moved.
 std::vector<const void *> Renderer::frame_buffers() const {
     const DevBuf *all[] = {&scenes_, &scene_line_base_, &layers_, &props_, &stops_, &row_scene_, &setup_,
-                           &lengths_, &offsets_, &scan_status_, &status_, &segs_a_, &segs_b_, &hist_,
-                           &bases_, &sort_status_, &row_start_, &part_line_,
+                           &lengths_, &offsets_, &scan_status_, &status_, &segs_a_, &segs_b_, &dig_,
+                           &row_start_, &part_line_,
                            &cnt0_, &cnt_acc_,
                            &cnt_status_, &work_, &runs_work_, &tile_nr_, &runs_status_, &tile_start_, &tile_run0_, &run_key_, &run_beg_,
                            &run_cov_, &rec_, &rec_len_, &rec_hdr_, &deep_list_, &part_kept_};
