@@ -75,8 +75,14 @@ __global__ void k_setup(const float4 *pts, const uint16_t *ids, const uint32_t *
                         const SceneDesc *scenes, const uint32_t *scene_line_base, FrameDims dims,
                         LineSetup *out, uint32_t *lengths, uint32_t *err, uint32_t order_lines);
 __global__ void k_line_ranks(const uint16_t *ids, uint64_t first, uint64_t n, uint32_t *counts, uint32_t *rank);
+// device-side bound of a scan (see k_scan_u32)
+struct ScanLimit {
+    const uint64_t *src = nullptr;
+    uint32_t mul = 0, div = 1;
+};
 __global__ void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n, uint64_t *status,
-                           uint32_t *stall, uint64_t *total);
+                           uint32_t *stall, uint64_t *total,
+                           ScanLimit lim = ScanLimit{});
 // Per-partition slot counts written by the emission: pass 0's digit counts
 // (digit-major, one column per partition) and slices of the other totals.
 struct SlotCounts {

@@ -440,13 +440,21 @@ __global__ __launch_bounds__(256) void k_path_emit(const PathCommand *__restrict
 
 // Single-pass exclusive scan of u32 values (decoupled look-back), 2048
 // values per partition. out may alias in. total receives the sum (u64).
+// With lim.src the scan covers only the first min(n, lim.mul *
+// ceil(*lim.src / lim.div)) values: the partitions a frame's device-side
+// count leaves (digit counts of a frame smaller than its buffers' capacity).
 __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *out, uint64_t n,
                                                   uint64_t *status, uint32_t *stall,
-                                                  uint64_t *total) {
+                                                  uint64_t *total, ScanLimit lim) {
     __shared__ uint32_t wsum[4];
     __shared__ uint64_t s_excl;
     constexpr uint32_t IPT = 8;
+    if (lim.src) {
+        const uint64_t m = (*lim.src + lim.div - 1) / lim.div * lim.mul;
+        n = m < n ? m : n;
+    }
     const uint32_t part = blockIdx.x;
+    if ((uint64_t)part * 256 * IPT >= n && part) return;  // past the values (partition 0 writes total)
     const uint64_t base = (uint64_t)part * 256 * IPT + (uint64_t)threadIdx.x * IPT;
     uint32_t v[IPT], sum = 0;
     for (uint32_t k = 0; k < IPT; ++k) {
@@ -1024,8 +1032,14 @@ __global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(kS
         const uint32_t li = w * (64 * kSortKPT) + i * 64 + lane;
         keys[i] = li < valid_n && li < wave_lim ? in[base + li] : ~0ull;
     }
+    // steps past the wave's last word (the slots of the first pass) rank nothing
+    const uint32_t wave_end = (wave_lim < valid_n ? wave_lim : valid_n);
 #pragma unroll
     for (uint32_t i = 0; i < kSortKPT; ++i) {
+        if (w * (64 * kSortKPT) + i * 64 >= wave_end) {  // uniform over the wave
+            rank[i] = 0xFFFFFFFFu;
+            continue;
+        }
         bool valid = true;
         uint64_t key = keys[i];
         if (key >> 63) {

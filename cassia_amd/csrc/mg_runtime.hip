@@ -624,8 +624,9 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     zero_words(stream_, (uint32_t)cus_, {{cnt_status_.ptr, P ? 2 * cnt_parts * dims.passes : 0}});
     if (P) {
         k_count_reduce<<<2, 256, 0, stream_>>>(cnt_acc_.as<uint32_t>(), ctr);
-        k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
-                                                           cnt_status_.as<uint64_t>(), stall, &ctr->order_total);
+        k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(
+            cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0, cnt_status_.as<uint64_t>(), stall, &ctr->order_total,
+            ScanLimit{&ctr->line_pixels, radix, kEmitTileHost});
     }
     MG_HIP(hipGetLastError(), "count launch");
     MG_HIP(mark(evs_[3]), "event");
@@ -643,9 +644,9 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
                 k_digit_count<6><<<sort_grid, 256, 0, stream_>>>(dig_.as<uint8_t>(), ctr, cnt0_.as<uint32_t>());
             else
                 k_digit_count<4><<<sort_grid, 256, 0, stream_>>>(dig_.as<uint8_t>(), ctr, cnt0_.as<uint32_t>());
-            k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0,
-                                                               cnt_status_.as<uint64_t>() + (size_t)p * cnt_parts,
-                                                               stall, &ctr->order_total);
+            k_scan_u32<<<(uint32_t)cnt_parts, 256, 0, stream_>>>(
+                cnt0_.as<uint32_t>(), cnt0_.as<uint32_t>(), n_cnt0, cnt_status_.as<uint64_t>() + (size_t)p * cnt_parts,
+                stall, &ctr->order_total, ScanLimit{&ctr->kept, radix, kSortTileHost});
         }
         const uint32_t *kept = p == 0 ? part_kept_.as<uint32_t>() : nullptr;
         const bool next = p + 1 < dims.passes;
