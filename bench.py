@@ -40,9 +40,18 @@ VALU_PEAK_PER_S = SIMD_CYCLES_PER_S / 4
 
 
 # kernel -> stats field with its per-frame time (hipEvents on the library's
-# stream, bracketing only that kernel's launches)
-KERNELS = {"k_emit": "emit_ms", "k_sort_pass": "sort_ms", "k_tile_runs": "runs_ms",
+# stream, bracketing only that kernel's launches; the sort passes' own events
+# in sort_pass_ms)
+KERNELS = {"k_emit": "emit_ms", "k_sort_pass": "sort_pass_ms", "k_tile_runs": "runs_ms",
            "k_row_walk": "walk_ms", "k_paint": "paint_ms"}
+
+
+def kernel_ms(st, field):
+    """Per-frame ms of one KERNELS entry: the stage time, or the sum of the
+    digit pass kernels."""
+    if field == "sort_pass_ms":
+        return sum(st.sort_pass_ms[p] for p in range(st.sort_passes))
+    return getattr(st, field)
 
 
 def kernel_bytes(st, width, height, n_layers):
@@ -597,6 +606,8 @@ def main():
         comp.render_device(W, H, scene.background)
     lat = []
     kms = {k: 0.0 for k in KERNELS}
+    pass_ms = [0.0] * 16
+    sort_stage_ms = 0.0
     dev_ms = 0.0
     addr = None
     t0 = time.perf_counter()
@@ -606,7 +617,10 @@ def main():
         lat.append((time.perf_counter() - a) * 1e3)
         st = comp.stats()
         for k, f in KERNELS.items():
-            kms[k] += getattr(st, f)
+            kms[k] += kernel_ms(st, f)
+        for p in range(st.sort_passes):
+            pass_ms[p] += st.sort_pass_ms[p]
+        sort_stage_ms += st.hist_ms + st.sort_ms
         dev_ms += st.device_ms
     elapsed = time.perf_counter() - t0
     progress(f"{args.steps} frames in {elapsed:.2f} s")
@@ -626,6 +640,16 @@ def main():
         kernels[k] = {"ms_per_frame": round(ms, 4), "launches": n, "ms_per_launch": round(per_launch_ms, 4),
                       "bytes_per_launch": b,
                       "gbs": round(b / (per_launch_ms * 1e-3) / 1e9, 2) if per_launch_ms > 0 else None}
+    kernels["k_sort_pass"]["pass_ms"] = [round(pass_ms[p] / K, 4) for p in range(st.sort_passes)]
+    # the whole sort stage: the passes, the later passes' digit counts and
+    # the digit-count scans, against the passes' algorithmic bytes
+    n_sort, b_sort = kb["k_sort_pass"]
+    stage = sort_stage_ms / K
+    sort_stage = {"ms_per_frame": round(stage, 4), "bytes_per_frame": n_sort * b_sort,
+                  "gbs": round(n_sort * b_sort / (stage * 1e-3) / 1e9, 2) if stage > 0 else None,
+                  "includes": "digit passes, the later passes' digit counts (k_digit_count) and the "
+                              "digit-count scans (k_scan_u32)"}
+    sort_stage["hbm_frac"] = round(sort_stage["gbs"] / HBM_PEAK_GBS, 4) if sort_stage["gbs"] else None
     valu = measured_valu(profile_name) if profile_name else {}
     valu_f32 = measured_valu_f32(profile_name) if profile_name else {}
     for k, v in kernels.items():
@@ -696,6 +720,7 @@ def main():
         "dispatches_per_frame": (profile_value(f"kernel-trace_{profile_name}", "dispatches_per_frame")
                                  if profile_name else None),
         "kernels": kernels,
+        "sort_stage": sort_stage,
         "roofline": roofline,
         "scene_build_s": round(t_build, 2),
     }

@@ -70,7 +70,7 @@ class FrameStats(C.Structure):
                 ("emit_ms", C.c_double), ("hist_ms", C.c_double), ("sort_ms", C.c_double),
                 ("runs_ms", C.c_double), ("walk_ms", C.c_double), ("paint_ms", C.c_double),
                 ("runs", C.c_uint64), ("records", C.c_uint64), ("walk_threads", C.c_uint32), ("deep_tiles", C.c_uint32),
-                ("walk_lds", C.c_uint32), ("walk_rewalks", C.c_uint32)]
+                ("walk_lds", C.c_uint32), ("walk_rewalks", C.c_uint32), ("sort_pass_ms", C.c_double * 16)]
 
 WALK_SHARE, WALK_HALF, WALK_WHOLE, WALK_GLOBAL = 0, 1, 2, 3
 

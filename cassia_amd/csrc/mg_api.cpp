@@ -780,6 +780,7 @@ static void fill_stats(mg_frame_stats &st, const mg::FrameResult &r, size_t line
     st.deep_tiles = r.deep_tiles;
     st.walk_lds = r.walk_lds;
     st.walk_rewalks = r.walk_rewalks;
+    for (uint32_t p = 0; p < 16; ++p) st.sort_pass_ms[p] = p < r.passes ? r.pass_ms[p] : 0.0;
 }
 
 // Scene descriptor of one frame of `width` x `rows_px` pixels at `stride`

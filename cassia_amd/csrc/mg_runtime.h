@@ -17,6 +17,8 @@
 
 namespace mg {
 
+constexpr uint32_t kMaxSortPasses = 16;  // digit passes of the widest sort key
+
 // A fixed block of device memory handed out front to back; nothing is freed
 // until reset. An engine in arena mode takes every buffer from one.
 struct Arena {
@@ -105,6 +107,7 @@ struct FrameResult {
     double device_ms = 0.0;
     // setup+scan, emit, histogram+bases, sort passes, tile runs, row walk, paint
     double stage_ms[7] = {0, 0, 0, 0, 0, 0, 0};
+    double pass_ms[kMaxSortPasses] = {};  // each sort pass kernel alone
     uint64_t runs = 0, records = 0;     // tile runs and tile id records of the painted frame
     const uint64_t *sorted = nullptr;   // device pointer to the sorted words (mode 1)
 };
@@ -213,6 +216,7 @@ private:
     hipStream_t stream_ = nullptr;
     hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
     hipEvent_t evs_[8] = {};
+    hipEvent_t evp_[2 * kMaxSortPasses] = {};  // before and after each sort pass kernel
     uint64_t uploaded_owner_ = 0, uploaded_version_ = 0;
     std::vector<SceneDesc> uploaded_scenes_;
     DevBuf scenes_, scene_line_base_, layers_, props_, stops_;

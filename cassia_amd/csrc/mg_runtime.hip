@@ -135,6 +135,8 @@ Renderer::~Renderer() {
     if (ev1_) (void)hipEventDestroy(ev1_);
     for (hipEvent_t e : evs_)
         if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : evp_)
+        if (e) (void)hipEventDestroy(e);
     if (stream_) (void)hipStreamDestroy(stream_);
     if (status_pinned_) (void)hipHostFree(status_host_);
 }
@@ -179,6 +181,7 @@ bool Renderer::init(std::string &err) {
     MG_HIP(hipEventCreate(&ev0_), "hipEventCreate");
     MG_HIP(hipEventCreate(&ev1_), "hipEventCreate");
     for (hipEvent_t &e : evs_) MG_HIP(hipEventCreate(&e), "hipEventCreate");
+    for (hipEvent_t &e : evp_) MG_HIP(hipEventCreate(&e), "hipEventCreate");
     if (!status_.reserve(sizeof(FrameStatus)) || !cov_count_.reserve(16)) {
         err = "out of device memory";
         return false;
@@ -500,7 +503,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
     dims.radix_bits = R;
     dims.passes = (sort_bits + R - 1) / R;
     dims.mode = spec.mode;
-    if (kPayloadBits + dims.key_bits > 64 || dims.passes > 16) {
+    if (kPayloadBits + dims.key_bits > 64 || dims.passes > kMaxSortPasses) {
         err = "frame too large for the 64-bit render key";
         return false;
     }
@@ -651,6 +654,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         const uint32_t *kept = p == 0 ? part_kept_.as<uint32_t>() : nullptr;
         const bool next = p + 1 < dims.passes;
         const uint32_t *offs = cnt0_.as<uint32_t>();
+        MG_HIP(mark(evp_[2 * p]), "event");
         if (R == 9)
             k_sort_pass<9><<<sort_grid, kSortThreadsHost, 0, stream_>>>(in, out, ctr, shift, offs, kept,
                                                                          next ? dig_.as<uint16_t>() : nullptr);
@@ -663,6 +667,7 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         else
             k_sort_pass<4><<<sort_grid, kSortThreadsHost, 0, stream_>>>(in, out, ctr, shift, offs, kept,
                                                                          next ? dig_.as<uint8_t>() : nullptr);
+        MG_HIP(mark(evp_[2 * p + 1]), "event");
         std::swap(in, out);
     }
     MG_HIP(hipGetLastError(), "sort launch");
@@ -712,6 +717,11 @@ bool Renderer::run(const FrameSpec &spec, const float4 *pts, const uint16_t *ids
         float sm = 0.0f;
         MG_HIP(hipEventElapsedTime(&sm, evs_[k], evs_[k + 1]), "elapsed");
         res.stage_ms[k] = sm;
+    }
+    for (uint32_t p = 0; p < dims.passes; ++p) {
+        float sm = 0.0f;
+        MG_HIP(hipEventElapsedTime(&sm, evp_[2 * p], evp_[2 * p + 1]), "elapsed");
+        res.pass_ms[p] = sm;
     }
     res.segments = hs.ctr.segments;
     res.kept = hs.ctr.kept;
@@ -1095,6 +1105,7 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
                 for (uint32_t k = 0; k < kStatSlices; ++k) res.records += ov[kStatSlices + k];
                 res.runs = ov[3];
                 for (double &v : res.stage_ms) v = 0.0;
+                for (double &v : res.pass_ms) v = 0.0;
                 replayed = true;
                 return true;
             }

@@ -314,8 +314,9 @@ typedef struct mg_frame_stats {
     /* per stage (hipEvents on the library's stream) */
     double setup_ms;           /* line setup + line-pixel scan           */
     double emit_ms;            /* pixel-segment emission                 */
-    double hist_ms;            /* digit histograms + digit bases         */
-    double sort_ms;            /* all LSD digit passes                   */
+    double hist_ms;            /* first sort pass's digit-count scan     */
+    double sort_ms;            /* all LSD digit passes, with the later
+                                  passes' digit counts and scans         */
     double runs_ms;            /* row/tile ranges + per-tile layer runs  */
     double walk_ms;            /* row-serial cover carry (id records)    */
     double paint_ms;           /* tile paint: workbench + painter        */
@@ -326,6 +327,8 @@ typedef struct mg_frame_stats {
        LDS, 2 the whole LDS, 3 global memory; walks repeated because a tile
        outgrew the arrays */
     uint32_t walk_lds, walk_rewalks;
+    double sort_pass_ms[16];   /* each digit pass kernel alone (hipEvents
+                                  around it; 0 past sort_passes)          */
 } mg_frame_stats;
 MG_EXPORT mg_status mg_last_frame_stats(const mg_composition *comp, mg_frame_stats *out);
 
