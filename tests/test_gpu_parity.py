@@ -478,3 +478,60 @@ def test_far_and_non_finite_coordinates(gpu, cpu):
     assert len(out[cpu][0]) > 500
     assert np.array_equal(out[cpu][0], out[gpu][0])
     assert_same_image(out[cpu][1], out[gpu][1], w)
+
+
+def _lines_with_line_pixels(rng, target, w, h):
+    """Vertical lines (x0 = x1) whose line-pixel counts, per the oracle's
+    LinesBuilder::build, add up to exactly `target`."""
+    import oracle
+    # line-pixels of a vertical line of k pixels' height starting inside a pixel
+    ks = np.arange(1, h - 2)
+    probe = oracle.lines_build(np.full(len(ks), 3.25), np.full(len(ks), 0.5), np.full(len(ks), 3.25),
+                               0.5 + ks.astype(np.float64), np.zeros(len(ks), np.uint16))
+    by_len = {}
+    for k, n in zip(ks, probe["lengths"]):
+        by_len.setdefault(int(n), int(k))
+    lines, total = [], 0
+    while total < target:
+        need = target - total
+        n = min(need, int(rng.integers(40, h - 8)))
+        while n not in by_len:
+            n -= 1
+        k = by_len[n]
+        x = float(rng.integers(1, w - 1)) + 0.25
+        y0 = float(rng.integers(0, h - k - 1)) + 0.5
+        lines.append((x, y0, x, y0 + k))
+        total += n
+    return np.array(lines, np.float32)
+
+
+@pytest.mark.parametrize("target", [2047, 2048, 2049, 4095, 4096, 4097, 6143, 6145, 8191, 8193, 10247])
+def test_sort_partition_boundaries(gpu, cpu, target):
+    """Frames whose line-pixel counts sit on either side of the emission
+    partition (2,048 line-pixels, 4,096 word slots) and sort partition (two
+    emission partitions, 8,192 slots) edges: a sort partition with one
+    emission partition, partial emission waves, partial sort partitions."""
+    import oracle
+    w, h = 96, 128
+    rng = np.random.default_rng(target)
+    lines = _lines_with_line_pixels(rng, target, w, h)
+    got = oracle.lines_build(lines[:, 0], lines[:, 1], lines[:, 2], lines[:, 3], np.zeros(len(lines), np.uint16))
+    assert int(got["lengths"].sum()) == target
+    props = scenegen.random_props(cpu, np.random.default_rng(target), 6)
+    gprops = scenegen.random_props(gpu, np.random.default_rng(target), 6)
+    layer_of = rng.integers(0, 6, len(lines))
+    outs = []
+    for m, pr in ((cpu, props), (gpu, gprops)):
+        comp = m.Composition()
+        ids = [comp.create_layer() for _ in range(6)]
+        for i, lid in enumerate(ids):
+            sel = lines[layer_of == i]
+            if len(sel):
+                comp.insert_lines(lid, sel)
+            comp.get_mut(lid).set_props(pr[i])
+        buf = np.zeros((w * h, 4), np.uint8)
+        comp.render(m.Buffer(buf, w), [1.0, 1.0, 1.0, 1.0], None)
+        outs.append(buf)
+        if m is gpu:
+            assert comp.stats().line_pixels == target
+    assert_same_image(outs[0], outs[1], w)
