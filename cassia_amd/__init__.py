@@ -7,6 +7,7 @@ on the GPU through ``libmoldgpu.so``; importing fails loudly when the library
 has not been built, and rendering raises when no gfx950 device is present.
 """
 import ctypes as C
+import itertools
 import enum
 import os
 from dataclasses import dataclass, field
@@ -98,6 +99,15 @@ _PROPS_DTYPE = _props_dtype()
 class _BatchFrame(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("background", C.c_float * 4),
                 ("has_crop", C.c_int32), ("crop", _Rect)]
+
+
+# numpy view of an array of _BatchFrame
+_BATCH_FRAME_DTYPE = np.dtype({"names": ["width", "height", "background", "has_crop", "crop"],
+                               "formats": ["<u4", "<u4", ("<f4", 4), "<i4", ("<u4", 4)],
+                               "offsets": [_BatchFrame.width.offset, _BatchFrame.height.offset,
+                                           _BatchFrame.background.offset, _BatchFrame.has_crop.offset,
+                                           _BatchFrame.crop.offset],
+                               "itemsize": C.sizeof(_BatchFrame)})
 
 
 class BatchTuning(C.Structure):
@@ -806,22 +816,28 @@ class Batch:
         n = len(comps)
         if n != len(frames):
             raise ValueError("one frame per composition")
-        hs = (_vp * max(n, 1))(*[c._h for c in comps])
-        fs = (_BatchFrame * max(n, 1))()
-        for i, fr in enumerate(frames):
-            w, h, bg = fr[0], fr[1], fr[2]
-            crop = fr[3] if len(fr) > 3 else None
-            fs[i].width, fs[i].height = w, h
-            for k in range(4):
-                fs[i].background[k] = float(bg[k])
-            if crop is not None:
-                fs[i].has_crop = 1
-                fs[i].crop = _Rect(*crop.px[:2], *crop.px[2:])
+        # the tables are packed with numpy (a per-field ctypes loop over
+        # thousands of frames costs milliseconds of host time per render)
+        hs = np.fromiter((c._h.value for c in comps), dtype=np.uint64, count=n)
+        fs = np.zeros(max(n, 1), dtype=_BATCH_FRAME_DTYPE)
+        if n:
+            chain = itertools.chain.from_iterable
+            wh = np.fromiter(chain((fr[0], fr[1]) for fr in frames), dtype=np.uint32, count=2 * n)
+            fs["width"][:n] = wh[0::2]
+            fs["height"][:n] = wh[1::2]
+            fs["background"][:n] = np.fromiter(chain(fr[2] for fr in frames), dtype=np.float32,
+                                               count=4 * n).reshape(n, 4)
+            for i, fr in enumerate(frames):
+                if len(fr) > 3 and fr[3] is not None:
+                    px = fr[3].px
+                    fs["has_crop"][i] = 1
+                    fs["crop"][i] = (px[0], px[1], px[2], px[3])
         dev = _vp()
         offs = C.POINTER(C.c_uint64)()
-        _check(_lib.mg_batch_render(self._h, hs, fs, n, C.byref(dev), C.byref(offs)))
+        _check(_lib.mg_batch_render(self._h, hs.ctypes.data_as(C.POINTER(_vp)),
+                                    fs.ctypes.data_as(C.POINTER(_BatchFrame)), n, C.byref(dev), C.byref(offs)))
         self._frames = [(fr[0], fr[1]) for fr in frames]
-        self._offsets = [offs[i] for i in range(n)]
+        self._offsets = np.ctypeslib.as_array(offs, shape=(n,)).tolist() if n else []
         return dev.value
 
     @property
