@@ -322,7 +322,16 @@ __global__ __launch_bounds__(256) void k_setup(const float4 *__restrict__ pts, c
                                                uint32_t order_lines) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= dims.n_lines) return;
-    uint32_t sc = dims.n_scenes == 1 ? 0 : scene_of_line(scene_line_base, dims.n_scenes, i);
+    uint32_t sc = 0;
+    if (dims.n_scenes > 1) {
+        // the wave's lines are consecutive and mostly of one scene: one
+        // search for its first line, then lanes past that scene's lines
+        // search on their own
+        const uint32_t i0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)i);
+        sc = scene_of_line(scene_line_base, dims.n_scenes, i0);
+        const uint32_t next = sc + 1 < dims.n_scenes ? scene_line_base[sc + 1] : 0xFFFFFFFFu;
+        if (i >= next) sc = scene_of_line(scene_line_base, dims.n_scenes, i);
+    }
     const SceneDesc &S = scenes[sc];
     // the scene's own resident lines, or the frame's line arrays
     const float4 *P = S.lines_pts ? (const float4 *)S.lines_pts : pts;
