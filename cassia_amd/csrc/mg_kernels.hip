@@ -498,15 +498,23 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
 //   area_cover() (rasterizer/mod.rs:149-172): x/y swapped for the same
 //               octants, the area axis reversed for octants 2, 6 (0x44) and
 //               the cover axis for octants 3, 4, 5, 6 (0x78).
-__device__ __forceinline__ uint64_t pack_segment(uint32_t layer, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
-                                                 int32_t bx, int32_t by, uint32_t octant) {
+// The fields of one piece's PixelSegment, each truncated as its bit field
+// of the u64 layout truncates it.
+struct SegFields {
+    int32_t tj;          // tile row: i16, stored in 12 bits
+    int32_t tib;         // tile column + 256 (at least -1 + 256): i16, stored in 13 bits
+    uint32_t layer;      // 16 bits
+    uint32_t tx, ty;     // pixel in the tile, 3 bits each
+    uint32_t area, cover;  // i16 and i8, stored in 10 and 6 bits
+};
+
+__device__ __forceinline__ SegFields seg_fields(uint32_t layer, int32_t x0, int32_t x1, int32_t y0, int32_t y1,
+                                                int32_t bx, int32_t by, uint32_t octant) {
     const bool sw = (0x66u >> octant) & 1u;
     const bool fa = (0x44u >> octant) & 1u;
     const bool fb = (0x78u >> octant) & 1u;
     const int32_t tbx = sw ? by : bx, tby = sw ? bx : by;
     const int32_t ti = as_i16(tbx >> 3);
-    const int32_t tj = as_i16(tby >> 3);
-    const uint32_t tx = (uint32_t)(tbx & 7), ty = (uint32_t)(tby & 7);
     const int32_t a0 = sw ? y0 : x0, a1 = sw ? y1 : x1;
     const int32_t c0 = sw ? x0 : y0, c1 = sw ? x1 : y1;
     const int32_t ax0 = fa ? a1 : a0, ax1 = fa ? a0 : a1;
@@ -515,23 +523,35 @@ __device__ __forceinline__ uint64_t pack_segment(uint32_t layer, int32_t x0, int
     const int32_t h = wrap_sub(ay1, ay0);
     const int32_t tri = as_i16(wrap_mul(h, wrap_sub(ax1, ax0)) / 2);
     const int32_t rect = as_i16(wrap_mul(h, wrap_sub(border, ax1)));
-    const int32_t area = as_i16(tri + rect);
-    const int32_t cover = as_i8(h);
-    const int32_t tib = as_i16((ti < -1 ? -1 : ti) + 256);
-    uint64_t w = ((uint64_t)(uint32_t)tj & 0xFFFull) << 51;
-    w |= ((uint64_t)(uint32_t)tib & 0x1FFFull) << 38;
-    w |= (uint64_t)(layer & 0xFFFFu) << 22;
-    w |= (uint64_t)ty << 19;
-    w |= (uint64_t)tx << 16;
-    w |= ((uint64_t)(uint32_t)area & 0x3FFull) << 6;
-    w |= (uint64_t)(uint32_t)cover & 0x3Full;
+    SegFields f;
+    f.tj = as_i16(tby >> 3);
+    f.tib = as_i16((ti < -1 ? -1 : ti) + 256);
+    f.layer = layer & 0xFFFFu;
+    f.tx = (uint32_t)(tbx & 7);
+    f.ty = (uint32_t)(tby & 7);
+    f.area = (uint32_t)as_i16(tri + rect);
+    f.cover = (uint32_t)as_i8(h);
+    return f;
+}
+
+// The PixelSegment (raster_segment.rs:142-160) of one piece.
+__device__ __forceinline__ uint64_t pack_segment(const SegFields &f) {
+    uint64_t w = ((uint64_t)(uint32_t)f.tj & 0xFFFull) << 51;
+    w |= ((uint64_t)(uint32_t)f.tib & 0x1FFFull) << 38;
+    w |= (uint64_t)f.layer << 22;
+    w |= (uint64_t)f.ty << 19;
+    w |= (uint64_t)f.tx << 16;
+    w |= ((uint64_t)f.area & 0x3FFull) << 6;
+    w |= (uint64_t)f.cover & 0x3Full;
     return w;
 }
 
 // Rasterizer::rasterize for one line-pixel (rasterizer/mod.rs:233-281): one
 // piece, or two when the span crosses a pixel-row border. Both pieces are
 // always computed (no divergence); the return value says how many count.
-__device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, uint64_t out[2]) {
+// `emit` turns a piece's fields into its slot word.
+template <class Emit>
+__device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, uint64_t out[2], Emit &&emit) {
     const uint32_t layer = l.packed & 0xFFFFu, octant = (l.packed >> 16) & 7u;
     const int32_t x0i = wrap_add(l.start, pi);
     const int32_t x1i = wrap_add(x0i, 1);
@@ -546,8 +566,8 @@ __device__ __forceinline__ int rasterize_pixel(const LineSetup &l, int32_t pi, u
     const float x = __builtin_fmaf(l.slope_recip, (float)y, l.px_slope_recip_py);
     const int32_t xs = sub_pixel_hw(x);
     const int32_t ex = split ? xs : x1s, ey = split ? ys : y1s;
-    out[0] = pack_segment(layer, x0s, ex, y0s, ey, x0i, (ey < y0s ? ey : y0s) >> 4, octant);
-    out[1] = pack_segment(layer, xs, x1s, ys, y1s, x0i, (ys < y1s ? ys : y1s) >> 4, octant);
+    out[0] = emit(seg_fields(layer, x0s, ex, y0s, ey, x0i, (ey < y0s ? ey : y0s) >> 4, octant));
+    out[1] = emit(seg_fields(layer, xs, x1s, ys, y1s, x0i, (ys < y1s ? ys : y1s) >> 4, octant));
     return split ? 2 : 1;
 }
 
@@ -601,15 +621,19 @@ struct SceneCull {
 __device__ __forceinline__ SceneCull scene_cull(const SceneDesc &S) {
     return SceneCull{(int32_t)S.crop_v0, (int32_t)S.crop_v1, (int32_t)S.crop_h1, S.row_base};
 }
-__device__ __forceinline__ uint64_t render_word_cull(uint64_t w, const SceneCull &c, const FrameDims &d) {
-    const int32_t tj = (int32_t)((int64_t)(w << 1) >> 52);  // sign-extended 12-bit field
-    const int32_t col = (int32_t)((w >> 38) & 0x1FFFu) - 256;
+// The render word of a piece (render_word's key over the payload; rows
+// outside the scene and tiles right of its crop culled) straight from its
+// fields, truncated as the PixelSegment's bit fields truncate them (tile row
+// to 12 bits, sign-extended; biased column to 13 bits).
+__device__ __forceinline__ uint64_t render_word_fields(const SegFields &f, const SceneCull &c, const FrameDims &d) {
+    const int32_t tj = (int32_t)((uint32_t)f.tj << 20) >> 20;  // sign-extended 12-bit field
+    const int32_t col = (int32_t)((uint32_t)f.tib & 0x1FFFu) - 256;
     const uint32_t colp = col < 0 ? 0u : (uint32_t)col + 1u;
-    const uint32_t layer = (uint32_t)(w >> 22) & 0xFFFFu;
     const uint32_t row = c.row_base + (uint32_t)tj;
-    const uint64_t key = ((uint64_t)row << (d.col_bits + d.layer_bits)) | ((uint64_t)colp << d.layer_bits) | layer;
+    const uint64_t key = ((uint64_t)row << (d.col_bits + d.layer_bits)) | ((uint64_t)colp << d.layer_bits) | f.layer;
+    const uint32_t payload = f.ty << 19 | f.tx << 16 | (f.area & 0x3FFu) << 6 | (f.cover & 0x3Fu);
     const bool keep = tj >= c.v0 && tj < c.v1 && col < c.h1;
-    return keep ? (key << kPayloadBits) | (w & ((1ull << kPayloadBits) - 1)) : kSlotCull;
+    return keep ? (key << kPayloadBits) | payload : kSlotCull;
 }
 
 // Emission: every line-pixel p owns the two slots 2p, 2p+1 of the word
@@ -625,7 +649,7 @@ __device__ __forceinline__ uint64_t render_word_cull(uint64_t w, const SceneCull
 // from global memory). 256 keeps the emission's LDS near 22 KB, seven work
 // groups per CU instead of four at 512 (3840x2160 frame: 2.60 -> 2.10 ms).
 constexpr uint32_t kEmitLines = 256;
-constexpr uint32_t kCountSlices = 256;  // accumulation slices of the later passes' digit totals
+constexpr uint32_t kCountSlices = 256;  // accumulation slices of the kept and culled counts
 
 // Per-partition counts of the word slots (a partition is kSortTile slots).
 // LDS layout `s_cnt`: the first sort pass's digits, then kept words, then
@@ -718,13 +742,15 @@ __device__ __forceinline__ uint32_t emit_span(const LineSetup *lines, const Scen
             const LineSetup L = lines[li - base];
             const SceneCull sc = Staged ? culls[li - base] : scene_cull(scenes[L.packed >> 19]);
             uint64_t wd[2];
-            const int n = rasterize_pixel(L, (int32_t)(p - first), wd);
+            // render frames: the render word straight from the fields; the
+            // canonical stream: the PixelSegment itself
+            const int n = dims.mode == 0
+                              ? rasterize_pixel(L, (int32_t)(p - first), wd,
+                                                [&](const SegFields &f) { return render_word_fields(f, sc, dims); })
+                              : rasterize_pixel(L, (int32_t)(p - first), wd,
+                                                [](const SegFields &f) { return pack_segment(f); });
             v[0] = wd[0];
             v[1] = wd[1];
-            if (dims.mode == 0) {
-#pragma unroll
-                for (int q = 0; q < 2; ++q) v[q] = render_word_cull(wd[q], sc, dims);
-            }
             if (n < 2) v[1] = kSlotNone;
             count_slot_pair(s_cnt, v[0], v[1], dims, kept);
         }
