@@ -54,7 +54,7 @@ struct PaintParams {
     uint2 *rec_hdr;
     uint32_t *rec_used;
     uint64_t rec_cap;
-    uint32_t *n_runs;            // statistics: runs of the frame
+    uint32_t *n_runs;            // statistics: runs of the frame, in kStatSlices counters
     uint32_t *n_records;         // statistics: records of the frame, in kStatSlices counters
     const uint2 *work;           // (global row, tile column) per paint work group
 };

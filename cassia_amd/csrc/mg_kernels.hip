@@ -1585,7 +1585,9 @@ __global__ __launch_bounds__(256) void k_tile_runs(PaintParams pp, const uint2 *
         const uint2 wk = work[t];
         runs += tile_runs_one(pp, wk.x, wk.y, cells);
     }
-    if (lane_id() == 0 && runs) atomicAdd(pp.n_runs, runs);
+    // (one of kStatSlices counters per work group: the waves of a frame of
+    // many small tiles would otherwise queue on one address)
+    if (lane_id() == 0 && runs) atomicAdd(pp.n_runs + (blockIdx.x % kStatSlices), runs);
 }
 
 // Merge-path split of two layer-sorted lists at diagonal d: q elements first

@@ -11,7 +11,7 @@ namespace mg {
 // words of the paint flags: overflow flags, record store use, run count,
 // deep tiles and the walk's maxima at 0..6, record counter slices from
 // kStatSlices on
-constexpr uint32_t kOverflowWords = 2 * kStatSlices;
+constexpr uint32_t kOverflowWords = 3 * kStatSlices;  // flags, record slices, run slices
 
 // The frame's status block on the device, cleared by the frame's first
 // kernel and read back by one copy at each host synchronisation: counters;
@@ -787,7 +787,7 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         pp.cov_count = cov_count_.as<uint32_t>();
         pp.cov_cap = spec.coverage_cap;
     }
-    pp.n_runs = ov_dev + 3;
+    pp.n_runs = ov_dev + 2 * kStatSlices;
     pp.n_records = ov_dev + kStatSlices;
     // Tiles of 8,192 records or more go to the four-wave deep painter (at most
     // two per CU; more stay with the one-wave painter). A tile holds one wave
@@ -956,8 +956,10 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
         res.walk_max_ids = ov[6];
         if (!ov[0]) {
             n_records_ = 0;
-            for (uint32_t k = 0; k < kStatSlices; ++k) n_records_ += ov[kStatSlices + k];
-            n_runs_ = ov[3];
+            for (uint32_t k = 0; k < kStatSlices; ++k) {
+                n_records_ += ov[kStatSlices + k];
+                n_runs_ += ov[2 * kStatSlices + k];
+            }
             n_deep_ = std::min(ov[4], pp.deep_cap);
             status_final_ = true;
             return true;
@@ -1102,8 +1104,11 @@ bool Renderer::run_graph(const FrameSpec &spec, const float4 *pts, const uint16_
                 res.kept = hc.kept;
                 res.device_ms = ms;
                 res.records = 0;
-                for (uint32_t k = 0; k < kStatSlices; ++k) res.records += ov[kStatSlices + k];
-                res.runs = ov[3];
+                res.runs = 0;
+                for (uint32_t k = 0; k < kStatSlices; ++k) {
+                    res.records += ov[kStatSlices + k];
+                    res.runs += ov[2 * kStatSlices + k];
+                }
                 for (double &v : res.stage_ms) v = 0.0;
                 for (double &v : res.pass_ms) v = 0.0;
                 replayed = true;
