@@ -1667,12 +1667,10 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
         while (x <= h0 && nr_l[x] == 0) ++x;
         return x;
     };
-    // Runs are prefetched into registers two columns ahead (one run per
-    // thread, two register sets taking turns), so their loads overlap a
-    // whole column's merge; runs beyond T are loaded when stored. A column's
-    // runs are stored at the end of the column before it: into the other set
-    // of run arrays without waiting (one barrier per column), or, with one
-    // set, after a barrier.
+    // Runs are prefetched into registers (one run per thread); runs beyond T
+    // are loaded when stored. With two sets of run arrays a column's runs
+    // are stored into the other set during the column before it (one barrier
+    // per column); with one set, after a barrier at the end of that column.
     struct Prefetch {
         uint32_t key, beg, n;
         uint64_t cov;
@@ -1727,9 +1725,11 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
         return true;
     };
     // One column: the cover queue entering tile colp - 1 meets its runs (in
-    // run arrays `buf`). Returns false when the work group must stop
-    // (capacity exceeded). Ends without a barrier.
-    auto step = [&](uint32_t colp, uint32_t buf) -> bool {
+    // run arrays `buf`). `mid` runs between the scan and the stores of the
+    // records (the two-buffer walk stores the next column's runs there).
+    // Returns false when the work group must stop (capacity exceeded). Ends
+    // without a barrier.
+    auto step = [&](uint32_t colp, uint32_t buf, auto &&mid) -> bool {
         const bool fold = colp <= h0;
         const uint32_t nr = nr_l[colp];
         {
@@ -1897,6 +1897,7 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
             rec_off = rec_next;
             rec_next += n_union;
         }
+        if (!mid()) return false;
         const bool write_rec = !fold && !counting;
         const uint32_t nxt = cur ^ 1u;
         uint32_t ou = rec_off + (uint32_t)ex, oq = (uint32_t)(ex >> 32);
@@ -1952,35 +1953,60 @@ __global__ __launch_bounds__(T) void k_row_walk(PaintParams pp) {
         cur = nxt;
         return true;
     };
-    uint32_t c0 = nr_l[0] ? 0u : next_colp(0), c1 = next_colp(c0), c2 = next_colp(c1);
-    Prefetch A, B;
-    issue(A, c0);
-    issue(B, c1);
-    if (!store(A, c0, 0)) return;
-    issue(A, c2);
-    __syncthreads();
-    uint32_t buf = 0;
-    while (c0 <= h1) {
-        // column c0 in arrays buf; B holds c1, A holds c2
-        if (!step(c0, buf)) return;
-        const uint32_t c3 = next_colp(c2);
-        if (nb == 1) __syncthreads();
-        if (!store(B, c1, buf ^ (nb - 1))) return;
-        issue(B, c3);
+    uint32_t c0 = nr_l[0] ? 0u : next_colp(0), c1 = next_colp(c0);
+    const auto none = [] { return true; };
+    if (nb == 2) {
+        // Two sets of run arrays: a column's runs are loaded into registers
+        // after the records of the column before it are stored, and stored
+        // into the other set in the middle of that column's step, before its
+        // own records. Loads waited for while record stores are outstanding
+        // wait for those stores too (one counter for both), so no record
+        // store lies between a load and its use.
+        Prefetch B;
+        issue(B, c0);
+        if (!store(B, c0, 0)) return;
+        issue(B, c1);
         __syncthreads();
-        buf ^= nb - 1;
-        if (c1 > h1) break;
-        // column c1 in arrays buf; A holds c2, B holds c3
-        if (!step(c1, buf)) return;
-        const uint32_t c4 = next_colp(c3);
-        if (nb == 1) __syncthreads();
-        if (!store(A, c2, buf ^ (nb - 1))) return;
-        issue(A, c4);
+        uint32_t buf = 0;
+        while (c0 <= h1) {
+            // column c0 in arrays buf; B holds c1
+            if (!step(c0, buf, [&] { return store(B, c1, buf ^ 1u); })) return;
+            c0 = c1;
+            c1 = next_colp(c1);
+            issue(B, c1);
+            __syncthreads();
+            buf ^= 1u;
+        }
+    } else {
+        // One set: the next column's runs are stored after a barrier; they
+        // are loaded two columns ahead.
+        uint32_t c2 = next_colp(c1);
+        Prefetch A, B;
+        issue(A, c0);
+        issue(B, c1);
+        if (!store(A, c0, 0)) return;
+        issue(A, c2);
         __syncthreads();
-        buf ^= nb - 1;
-        c0 = c2;
-        c1 = c3;
-        c2 = c4;
+        while (c0 <= h1) {
+            // column c0; B holds c1, A holds c2
+            if (!step(c0, 0, none)) return;
+            const uint32_t c3 = next_colp(c2);
+            __syncthreads();
+            if (!store(B, c1, 0)) return;
+            issue(B, c3);
+            __syncthreads();
+            if (c1 > h1) break;
+            // column c1; A holds c2, B holds c3
+            if (!step(c1, 0, none)) return;
+            const uint32_t c4 = next_colp(c3);
+            __syncthreads();
+            if (!store(A, c2, 0)) return;
+            issue(A, c4);
+            __syncthreads();
+            c0 = c2;
+            c1 = c3;
+            c2 = c4;
+        }
     }
     // (one of kStatSlices counters per row: rows of a large batch would
     // queue on one address)
