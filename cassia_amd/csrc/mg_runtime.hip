@@ -810,7 +810,11 @@ bool Renderer::paint(const FrameSpec &spec, uint32_t global_rows, uint32_t col_b
     const bool stream_runs = words_cap > (uint64_t)8192 * std::max<uint32_t>(n_runs_work_, 1);
     if (!stream_runs) {
         k_tile_starts<<<(uint32_t)((n_cols + 255) / 256), 256, 0, stream_>>>(pp, global_rows, 0u);
-        const uint32_t blocks = (uint32_t)std::min<uint64_t>((n_runs_work_ + 3) / 4, (uint64_t)cus_ * 16);
+        // four waves per work group, at least four columns per wave (a wave
+        // clears its LDS cells once) and at most eight work groups per CU,
+        // which all fit at once (1024x1024 circles 0.140 -> 0.073 ms,
+        // 3840x2160 polygons 1.10 -> 1.09 ms against sixteen per CU)
+        const uint32_t blocks = (uint32_t)std::min<uint64_t>((n_runs_work_ + 15) / 16, (uint64_t)cus_ * 8);
         if (blocks) k_tile_runs<<<blocks, 256, 0, stream_>>>(pp, runs_work_.as<uint2>(), n_runs_work_);
     } else {
         // runs over the whole sorted stream, then each tile's first run
