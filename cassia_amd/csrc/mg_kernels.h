@@ -137,7 +137,7 @@ constexpr uint32_t kXcdsHost = 8;           // XCD-mapped launches round their g
 inline uint32_t xcd_grid(uint64_t parts) {
     return (uint32_t)(parts ? (parts + kXcdsHost - 1) / kXcdsHost * kXcdsHost : kXcdsHost);
 }
-constexpr uint32_t kScanTileHost = 2048;
+constexpr uint32_t kScanTileHost = 4096;  // values per k_scan_u32 partition (16 per thread)
 constexpr uint32_t kRunsTileHost = 4096;  // words per k_runs_scan partition
 
 }  // namespace mg

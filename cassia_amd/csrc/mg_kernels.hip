@@ -457,8 +457,8 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
                                                   uint64_t *total, ScanLimit lim) {
     __shared__ uint32_t wsum[4];
     __shared__ uint64_t s_excl;
-    constexpr uint32_t IPT = 8;
-    static_assert(IPT == 8, "two uint4 per thread");
+    constexpr uint32_t IPT = kScanTileHost / 256;
+    static_assert(IPT % 4 == 0 && 256 * IPT == kScanTileHost, "whole uint4 per thread");
     if (lim.src) {
         const uint64_t m = (*lim.src + lim.div - 1) / lim.div * lim.mul;
         n = m < n ? m : n;
@@ -466,14 +466,16 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
     const uint32_t part = blockIdx.x;
     if ((uint64_t)part * 256 * IPT >= n && part) return;  // past the values (partition 0 writes total)
     const uint64_t base = (uint64_t)part * 256 * IPT + (uint64_t)threadIdx.x * IPT;
-    // a thread's IPT values are contiguous: two 16-byte loads and stores
-    // where all IPT are in range (the buffers are 256-byte aligned)
+    // a thread's IPT values are contiguous: 16-byte loads and stores where
+    // all IPT are in range (the buffers are 256-byte aligned)
     const bool whole = base + IPT <= n;
     uint32_t v[IPT], sum = 0;
     if (whole) {
-        const uint4 a = *reinterpret_cast<const uint4 *>(in + base);
-        const uint4 b = *reinterpret_cast<const uint4 *>(in + base + 4);
-        v[0] = a.x, v[1] = a.y, v[2] = a.z, v[3] = a.w, v[4] = b.x, v[5] = b.y, v[6] = b.z, v[7] = b.w;
+#pragma unroll
+        for (uint32_t q = 0; q < IPT / 4; ++q) {
+            const uint4 a = reinterpret_cast<const uint4 *>(in + base)[q];
+            v[4 * q] = a.x, v[4 * q + 1] = a.y, v[4 * q + 2] = a.z, v[4 * q + 3] = a.w;
+        }
     } else {
         for (uint32_t k = 0; k < IPT; ++k) v[k] = base + k < n ? in[base + k] : 0u;
     }
@@ -497,8 +499,9 @@ __global__ __launch_bounds__(256) void k_scan_u32(const uint32_t *in, uint32_t *
         run += v[k];
     }
     if (whole) {
-        *reinterpret_cast<uint4 *>(out + base) = make_uint4(o[0], o[1], o[2], o[3]);
-        *reinterpret_cast<uint4 *>(out + base + 4) = make_uint4(o[4], o[5], o[6], o[7]);
+#pragma unroll
+        for (uint32_t q = 0; q < IPT / 4; ++q)
+            reinterpret_cast<uint4 *>(out + base)[q] = make_uint4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
     } else {
         for (uint32_t k = 0; k < IPT; ++k)
             if (base + k < n) out[base + k] = o[k];
